@@ -1,0 +1,146 @@
+/*
+ * nudim.h -- C ABI of libnudim_hip.so, the MI355X (gfx950) implementation of
+ * NUDIM's dual-space HIO/ER constraint iteration.
+ *
+ * The reference has no library API: its `reconstruction` main calls NFFT3,
+ * FFTW3 and its own denoise.c directly.  Each entry point below names the
+ * reference call (file:line) it replaces.  Conventions:
+ *   - every function returns 0 on success and a negative NUDIM_E* code on
+ *     failure; nudim_last_error() gives a thread-local message.  The library
+ *     never aborts the host process.
+ *   - the caller owns every host array; the library copies on load / get.
+ *     The library owns all device memory through the opaque context.
+ *   - complex arrays are interleaved (re, im) doubles, as fftw_complex.
+ *   - volumes are double [nz][np][np] (the reference's img[z][y][x],
+ *     src/reconstruction.c:527).
+ *   - a context is used from one host thread at a time; distinct contexts
+ *     may run on distinct threads and overlap on the device.
+ */
+#ifndef NUDIM_H
+#define NUDIM_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { NUDIM_OK = 0, NUDIM_EINVAL = -1, NUDIM_ENOMEM = -2, NUDIM_EHIP = -3,
+       NUDIM_ESTATE = -4, NUDIM_EBUDGET = -5 };
+
+/* arithmetic of the device path */
+enum { NUDIM_FP64_PLAIN = 0, /* FP64 everywhere, literal NFFT-based CGNR, reference NFFT sizes */
+       NUDIM_FP32 = 1 };     /* FP32 data, FP64 tables/scalars, Toeplitz normal operator */
+
+/* --method (src/reconstruction.c:44-48) */
+enum { NUDIM_DIFFUSION = 0, NUDIM_TV = 1 };
+
+/* CGNR normal-operator form */
+enum { NUDIM_CG_AUTO = 0, NUDIM_CG_NFFT = 1, NUDIM_CG_TOEPLITZ = 2 };
+
+typedef struct nudim_params {
+  /* command-line surface of src/reconstruction.c:91-112 */
+  int nx, nz, np;                     /* --nx, --nz, -p/--pad */
+  double gam_min, gam_max, interval;  /* --min, --max, -t/--interval */
+  double padangle;                    /* --padangle */
+  int weight;                         /* --weight */
+  int random;                         /* --random (unknown samples: uniform[0,10)) */
+  unsigned long seed;                 /* MT19937 seed for --random (GSL default 0 -> 4357) */
+  int config_height, nheight, ncenter;/* --height / --center */
+  int denoise, method;                /* --denoise, --method */
+  int nhio, ner;                      /* --nhio, --ner */
+  /* device path */
+  int precision;                      /* NUDIM_FP64_PLAIN or NUDIM_FP32 */
+  int device;                         /* HIP device ordinal */
+  int nfft_n;                         /* oversampled grid (0: reference 2*nextpow2(np) for FP64,
+                                         autotuned for FP32) */
+  int nfft_w;                         /* window taps per dimension (0: 14 for FP64, tuned FP32) */
+  int cg_form;                        /* NUDIM_CG_* (0: NFFT for FP64, Toeplitz for FP32) */
+  int use_graph;                      /* capture each iteration into a hipGraph (FP32 only) */
+} nudim_params;
+
+typedef struct nudim_ctx nudim_ctx;
+
+const char *nudim_last_error(void);
+void nudim_default_params(nudim_params *p);
+
+/* ---- job lifecycle: replaces main() of src/reconstruction.c:50-733 ---- */
+
+/* validation of src/reconstruction.c:245-302; geometry of :304-426 */
+int nudim_create(const nudim_params *p, nudim_ctx **out);
+void nudim_destroy(nudim_ctx *ctx);
+/* number of tilt files nmeas and polar nodes M (src/reconstruction.c:304, 384-387) */
+int nudim_sizes(const nudim_ctx *ctx, int *nmeas, int *M, int *nang);
+/* projection FT + sample assembly, src/reconstruction.c:333-487;
+ * proj is double [nmeas][nz][nx] (file j, row z, column x) */
+int nudim_load_projections(nudim_ctx *ctx, const double *proj);
+/* assembled samples directly: slices c128 [nz][M], kind[M] (0 unknown, 1 known, 2 held out) */
+int nudim_load_samples(nudim_ctx *ctx, const double *slices, const int *kind);
+/* initial INFFT + z IFFT, src/reconstruction.c:489-539 */
+int nudim_initial(nudim_ctx *ctx);
+/* `count` passes of the HIO/ER loop body, src/reconstruction.c:545-700.  The pass
+ * counter nn continues across calls (HIO while nn < nhio).  rfactor[count] gets
+ * R = sqrt(diff1/diff2) of each pass (:603-604); may be NULL. */
+int nudim_iterate(nudim_ctx *ctx, int count, double *rfactor);
+/* current volume img[nz][np][np] (double) */
+int nudim_get_volume(nudim_ctx *ctx, double *img);
+/* replace the current volume (parity tests of single passes) and the pass counter */
+int nudim_set_volume(nudim_ctx *ctx, const double *img, int nn);
+/* at pass `nn` keep copies of img before and after denoising (the
+ * grad_before/grad_after dump of src/reconstruction.c:637-658) */
+int nudim_set_dump(nudim_ctx *ctx, int nn, double *before, double *after);
+/* bytes of device memory the context holds */
+size_t nudim_device_bytes(const nudim_ctx *ctx);
+/* bytes a context with these parameters would hold (admission control) */
+int nudim_estimate_bytes(const nudim_params *p, size_t *bytes);
+/* kernel timing with HIP events on the job's stream during nudim_iterate:
+ * class 0 off, 1 Toeplitz column pass, 2 NFFT interpolation, 3 NFFT gather,
+ * 4 other line FFT passes, 5 TV sweeps, 6 z-axis transforms.  Selecting a
+ * class resets the counters.  (No reference counterpart: the reference only
+ * prints total wall time, src/reconstruction-parallel.c:881-883.) */
+int nudim_profile(nudim_ctx *ctx, int kernel_class);
+/* total device ms, launch count and algorithmic bytes of the timed launches */
+int nudim_profile_read(nudim_ctx *ctx, double *ms, long *launches, double *bytes);
+/* wait for all device work of the context */
+int nudim_synchronize(nudim_ctx *ctx);
+
+/* ---- parity entry points (stateless; host arrays in the reference layouts) ---- */
+
+/* nfft_trafo of the plan p1 (src/reconstruction.c:393-405, :571):
+ * f[P][M] = sum_k fhat[P][N][N] exp(-2 pi i (k - N/2).x_m); x[M][2] as grid.c:71-72 */
+int nudim_nfft_trafo(int precision, int N, int M, const double *x, int n, int W, int P,
+                     const double *fhat, double *f);
+/* nfft_adjoint (inside the NFFT3 solver): fhat[P][N][N] = sum_m f[P][M] exp(+...) */
+int nudim_nfft_adjoint(int precision, int N, int M, const double *x, int n, int W, int P,
+                       const double *f, double *fhat);
+/* solver_before_loop_complex + nsteps x solver_loop_one_step_complex, CGNR
+ * [| PRECOMPUTE_WEIGHT] (src/reconstruction.c:413, :506-510, :592-595);
+ * y[P][M], fhat[P][N][N] in/out (initial guess), w[M] or NULL */
+int nudim_cgnr(int precision, int form, int N, int M, const double *x, const double *w, int n,
+               int W, int P, const double *y, double *fhat, int nsteps);
+/* fftw_plan_many_dft(1,&len,howmany,data,NULL,howmany,1,...,sign) in place,
+ * src/reconstruction.c:521-524, 556-559, 607-610; data c128 [len][howmany] */
+int nudim_zfft(int precision, int len, int howmany, double *data, int sign);
+/* total_variation_discrete(u, lam, niter, nz, ny, nx), src/denoise.c:67-151; u in/out */
+int nudim_tv(int precision, double *u, int nz, int ny, int nx, float lam, int niter);
+/* total_variation_discrete_height, src/denoise.c:153-249 */
+int nudim_tv_height(int precision, double *u, int nz, int ny, int nx, float lam, int niter,
+                    int nheight, int ncenter);
+/* aniso_diffusion_2d, src/denoise.c:320-378 (requires nz == nx, see :349-350) */
+int nudim_diffusion2d(int precision, double *u, int nz, int ny, int nx, float lambda,
+                      float kappa, int niter);
+/* density constraint, src/reconstruction.c:672-700; img [nz][np][np] in/out */
+int nudim_constraint(int precision, double *img, const double *img_last, int nz, int np, int nx,
+                     int is_hio, int config_height, int nheight, int ncenter);
+/* projection FT, src/reconstruction.c:306-367: proj [nmeas][nz][nx] -> tomo c128 [nmeas][nz][np] */
+int nudim_projection_ft(int precision, int nx, int nz, int np, int nmeas, const double *proj,
+                        double *tomo);
+/* 2D1D simulator, src/projection_sim.c:282-405 without noise:
+ * density [nz][ny][nx] -> proj [nslices][nz][np] = creal(slice)/(np*nz) */
+int nudim_project_2d1d(int nx, int ny, int nz, int np, double gam_min, double gam_max,
+                       double interval, const double *density, double *proj);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,909 @@
+// api.hip -- reconstruction job context and the C ABI of libnudim_hip.so.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "engine.hpp"
+#include "nudim.h"
+
+using namespace nudim;
+
+static thread_local std::string g_last_error;
+
+static int fail(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define API_BEGIN try {
+#define API_END                                            \
+  }                                                        \
+  catch (const nudim::Error &e) { return fail(e.code, e.what()); } \
+  catch (const std::bad_alloc &) { return fail(NUDIM_ENOMEM, "host allocation failed"); } \
+  catch (const std::exception &e) { return fail(NUDIM_EINVAL, e.what()); }
+
+extern "C" const char *nudim_last_error(void) { return g_last_error.c_str(); }
+
+// ---------------------------------------------------------------------------
+// device kernels of the set-up stage
+// ---------------------------------------------------------------------------
+
+// projection FT, reconstruction.c:306-367, as the separable exact sum:
+// Q[t][r][i] = sum_c P[t][r][c] ex[c][i]
+__global__ void k_pft_x(const double *__restrict__ proj, const cx<double> *__restrict__ ex, cx<double> *__restrict__ Q,
+                        int nx, int np, long long rows) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows * np) return;
+  long long row = i / np;
+  int k = (int)(i % np);
+  const double *pr = proj + row * nx;
+  cx<double> acc = mk(0.0, 0.0);
+  for (int c = 0; c < nx; c++) {
+    double v = pr[c];
+    if (v == 0.0) continue;
+    cx<double> e = ex[(long long)c * np + k];
+    acc = acc + mk(v * e.x, v * e.y);
+  }
+  Q[i] = acc;
+}
+
+// tomo[t][j][i] = sum_r Q[t][r][i] ez[r][j]
+__global__ void k_pft_z(const cx<double> *__restrict__ Q, const cx<double> *__restrict__ ez, cx<double> *__restrict__ tomo,
+                        int nz, int np, int nmeas) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long total = (long long)nmeas * nz * np;
+  if (i >= total) return;
+  int k = (int)(i % np);
+  long long tj = i / np;
+  int j = (int)(tj % nz);
+  long long t = tj / nz;
+  const cx<double> *q = Q + t * nz * np + k;
+  cx<double> acc = mk(0.0, 0.0);
+  for (int r = 0; r < nz; r++) acc = acc + q[(long long)r * np] * ez[(long long)r * nz + j];
+  tomo[i] = acc;
+}
+
+// sample assembly, reconstruction.c:437-484: meas[m][j]
+template <typename T>
+__global__ void k_assemble(const cx<double> *__restrict__ tomo, const int *__restrict__ kind, const int *__restrict__ order,
+                           const double *__restrict__ rnd, cx<T> *__restrict__ meas, int M, int np, int nz) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)M * nz) return;
+  int m = (int)(i / nz), j = (int)(i % nz);
+  cx<T> v = mk((T)0, (T)0);
+  if (kind[m] == 1) {
+    cx<double> t = tomo[((long long)order[m] * nz + j) * np + (m % np)];
+    v = mk((T)t.x, (T)t.y);
+  } else if (rnd) {
+    v = mk((T)rnd[(long long)j * M + m], (T)0);
+  }
+  meas[i] = v;
+}
+
+__global__ void k_rfactor_final(const double *__restrict__ d1, const double *__restrict__ d2, int P,
+                                double *__restrict__ out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double a = 0, b = 0;
+    for (int p = 0; p < P; p++) {
+      a += d1[p];
+      b += d2[p];
+    }
+    *out = sqrt(a / b);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// projection FT on the device (FP64)
+// ---------------------------------------------------------------------------
+
+static void projection_ft_dev(int nx, int nz, int np, int nmeas, const double *dproj, cx<double> *dtomo,
+                              hipStream_t st) {
+  std::vector<double> x;
+  uniform_nodes(np, nz, &x);
+  std::vector<double> ex(2 * (size_t)nx * np), ez(2 * (size_t)nz * nz);
+  for (int c = 0; c < nx; c++)
+    for (int i = 0; i < np; i++) {
+      double a = -2.0 * M_PI * (double)(c - nx / 2) * x[2 * i + 0];
+      ex[2 * ((size_t)c * np + i)] = std::cos(a);
+      ex[2 * ((size_t)c * np + i) + 1] = std::sin(a);
+    }
+  for (int r = 0; r < nz; r++)
+    for (int j = 0; j < nz; j++) {
+      double a = -2.0 * M_PI * (double)(r - nz / 2) * x[2 * ((size_t)j * np) + 1];
+      ez[2 * ((size_t)r * nz + j)] = std::cos(a);
+      ez[2 * ((size_t)r * nz + j) + 1] = std::sin(a);
+    }
+  DevBuf dex(ex.size() * 8), dez(ez.size() * 8), dQ((size_t)nmeas * nz * np * sizeof(cx<double>));
+  NUDIM_HIP(hipMemcpyAsync(dex.p, ex.data(), ex.size() * 8, hipMemcpyHostToDevice, st));
+  NUDIM_HIP(hipMemcpyAsync(dez.p, ez.data(), ez.size() * 8, hipMemcpyHostToDevice, st));
+  long long rows = (long long)nmeas * nz;
+  hipLaunchKernelGGL(k_pft_x, dim3(cdiv(rows * np, 256)), dim3(256), 0, st, dproj, dex.as<cx<double>>(),
+                     dQ.as<cx<double>>(), nx, np, rows);
+  hipLaunchKernelGGL(k_pft_z, dim3(cdiv(rows * np, 256)), dim3(256), 0, st, dQ.as<cx<double>>(), dez.as<cx<double>>(),
+                     dtomo, nz, np, nmeas);
+  NUDIM_HIP(hipGetLastError());
+  NUDIM_HIP(hipStreamSynchronize(st));
+}
+
+// ---------------------------------------------------------------------------
+// job
+// ---------------------------------------------------------------------------
+
+struct JobBase {
+  virtual ~JobBase() = default;
+  virtual void load_meas_planes(const double *slices) = 0;                        // host c128 [nz][M]
+  virtual void load_projections(const double *proj, const double *rnd) = 0;     // host
+  virtual void initial() = 0;
+  virtual void iterate(int nn, double *rdev_slot) = 0;
+  virtual void get_volume(double *img) = 0;
+  virtual void set_volume(const double *img) = 0;
+  virtual size_t bytes() const = 0;
+  double *dump_before = nullptr, *dump_after = nullptr;
+  int dump_nn = -1;
+};
+
+struct Geometry {
+  int nmeas = 0, nang = 0, M = 0;
+  double pad_min = 0, pad_max = 0;
+  std::vector<double> x, w;
+  std::vector<int> kind, order;
+};
+
+static void validate(const nudim_params *p) {
+  NUDIM_CHECK(p, "null parameters");
+  NUDIM_CHECK(p->nx > 0 && p->nz > 0, "You need to provide valid projection size with --nx, --nz.");
+  NUDIM_CHECK(p->np > 0, "You need to provide valid projection dimension in XY plane with --pad.");
+  NUDIM_CHECK(p->nhio >= 0, "You need to provide valid nhio with --nhio.");
+  NUDIM_CHECK(p->ner >= 0, "You need to provide valid ner with --ner.");
+  NUDIM_CHECK(p->interval > 0, "You need to provide valid angular sampling interval with --interval.");
+  NUDIM_CHECK(!(p->padangle < 0 ||
+                p->interval / p->padangle - (int)(p->interval / p->padangle + 0.1) > 1e-5) &&
+                  p->padangle > 0,
+              "You need to provide valid padded angular sampling interval with --padangle.");
+  NUDIM_CHECK(!(p->gam_max < -90 || p->gam_max > 90 || p->gam_min < -90),
+              "You need to provide valid scope of tomogram angles (-90,90] with --max, --min.");
+  NUDIM_CHECK(!(p->config_height && (p->nheight <= 0 || p->ncenter <= 0)),
+              "You must provide valid both sample height and center with --height and --center.");
+  NUDIM_CHECK(p->np % 2 == 0 && p->nz % 2 == 0, "np and nz must be even on this path");
+  NUDIM_CHECK(p->nx <= p->np, "--nx must not exceed --pad");
+  NUDIM_CHECK(p->precision == NUDIM_FP64_PLAIN || p->precision == NUDIM_FP32, "unknown precision");
+  NUDIM_CHECK(!(p->denoise && p->method == NUDIM_DIFFUSION && p->nz != p->np),
+              "DIFFUSION denoising needs nz == np (the reference clamps x with the z extent)");
+}
+
+static void make_geometry(const nudim_params *p, Geometry *g) {
+  g->nmeas = (int)((p->gam_max - p->gam_min) / p->interval + 0.5) + 1;
+  pad_range(p->gam_min, p->interval, &g->pad_min, &g->pad_max);
+  g->M = polar_nodes(g->pad_min, g->pad_max, p->padangle, p->np, &g->x);
+  g->nang = g->M / p->np;
+  if (p->weight) density_weights(p->np, g->nang, p->padangle, &g->w);
+  sample_classes(p->np, g->nang, g->pad_min, p->padangle, p->gam_min, p->gam_max, p->interval, &g->kind,
+                 &g->order);
+}
+
+static void resolve_knobs(const nudim_params *p, int *n, int *W, int *form) {
+  if (p->precision == NUDIM_FP64_PLAIN) {
+    *n = p->nfft_n > 0 ? p->nfft_n : 2 * next_pow2(p->np);
+    *W = p->nfft_w > 0 ? p->nfft_w : 14;
+    *form = p->cg_form > 0 ? p->cg_form : NUDIM_CG_NFFT;
+  } else {
+    *n = p->nfft_n > 0 ? p->nfft_n : nice_fft_size(2 * p->np);
+    *W = p->nfft_w > 0 ? p->nfft_w : 8;
+    *form = p->cg_form > 0 ? p->cg_form : NUDIM_CG_TOEPLITZ;
+  }
+}
+
+template <typename T>
+struct Job : JobBase {
+  nudim_params prm;
+  const Geometry &geo;
+  hipStream_t st;
+  int N, P, M, n, W, form;
+  long long V, NN, MM;
+  NfftOp<T> op;
+  ToeplitzOp<T> toep;
+  Cgnr<T> cg;
+  DevBuf dw, dkind, dorder;
+  DevBuf meas, y, Fh, F0;
+  DevBuf img_a, img_b, grad;
+  DevBuf d1, d2, tvpart_g, tvpart_a, tvsums;
+  FftDesc fdz{};
+  DevBuf twz;
+  T *img = nullptr, *img_last = nullptr;
+  static constexpr int kTvBlocks = 1024;
+
+  Job(const nudim_params &p, const Geometry &g, hipStream_t s) : prm(p), geo(g), st(s) {
+    N = p.np;
+    P = p.nz;
+    M = g.M;
+    resolve_knobs(&p, &n, &W, &form);
+    V = (long long)N * N * P;
+    NN = V;
+    MM = (long long)M * P;
+    op.init(N, n, W, g.x.data(), M, P);
+    if (!g.w.empty()) {
+      dw.alloc(g.w.size() * 8);
+      NUDIM_HIP(hipMemcpy(dw.p, g.w.data(), g.w.size() * 8, hipMemcpyHostToDevice));
+    }
+    if (form == NUDIM_CG_TOEPLITZ) toep.init(N, g.x.data(), g.w.empty() ? nullptr : g.w.data(), M, P, st);
+    cg.init(N, M, P, form, &op, form == NUDIM_CG_TOEPLITZ ? &toep : nullptr, dw.as<double>());
+    dkind.alloc(M * sizeof(int));
+    NUDIM_HIP(hipMemcpy(dkind.p, g.kind.data(), M * sizeof(int), hipMemcpyHostToDevice));
+    dorder.alloc(M * sizeof(int));
+    NUDIM_HIP(hipMemcpy(dorder.p, g.order.data(), M * sizeof(int), hipMemcpyHostToDevice));
+    meas.alloc(MM * sizeof(cx<T>));
+    y.alloc(MM * sizeof(cx<T>));
+    Fh.alloc(NN * sizeof(cx<T>));
+    F0.alloc(NN * sizeof(cx<T>));
+    img_a.alloc(V * sizeof(T));
+    img_b.alloc(V * sizeof(T));
+    if (p.denoise && p.method == NUDIM_TV) grad.alloc(V * sizeof(T));
+    d1.alloc(P * 8);
+    d2.alloc(P * 8);
+    tvpart_g.alloc(kTvBlocks * 8);
+    tvpart_a.alloc(kTvBlocks * 8);
+    tvsums.alloc(2 * 8);
+    fdz = make_fft_desc(P);
+    std::vector<double> t;
+    fft_twiddles(P, &t);
+    std::vector<cx<T>> h(P);
+    for (int i = 0; i < P; i++) h[i] = mk((T)t[2 * i], (T)t[2 * i + 1]);
+    twz.alloc(P * sizeof(cx<T>));
+    NUDIM_HIP(hipMemcpy(twz.p, h.data(), P * sizeof(cx<T>), hipMemcpyHostToDevice));
+    img = img_a.as<T>();
+    img_last = img_b.as<T>();
+    NUDIM_HIP(hipMemsetAsync(img, 0, V * sizeof(T), st));
+  }
+
+  size_t bytes() const override {
+    return op.bytes() + toep.bytes() + cg.bytes() + dw.bytes + dkind.bytes + dorder.bytes + meas.bytes + y.bytes +
+           Fh.bytes + F0.bytes + img_a.bytes + img_b.bytes + grad.bytes + d1.bytes + d2.bytes + tvpart_g.bytes +
+           tvpart_a.bytes + tvsums.bytes + twz.bytes;
+  }
+
+  void load_meas_planes(const double *slices) override {
+    DevBuf tmp((size_t)MM * 16);
+    NUDIM_HIP(hipMemcpyAsync(tmp.p, slices, (size_t)MM * 16, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_cx_from_planes<T>, dim3(cdiv(MM, 256)), dim3(256), 0, st, tmp.as<double>(), meas.as<cx<T>>(),
+                       (long long)M, P);
+    NUDIM_HIP(hipGetLastError());
+    NUDIM_HIP(hipStreamSynchronize(st));
+  }
+
+  void load_projections(const double *proj, const double *rnd) override {
+    int nx = prm.nx, nmeas = geo.nmeas;
+    size_t pb = (size_t)nmeas * P * nx * 8;
+    DevBuf dproj(pb), dtomo((size_t)nmeas * P * N * sizeof(cx<double>)), drnd;
+    NUDIM_HIP(hipMemcpyAsync(dproj.p, proj, pb, hipMemcpyHostToDevice, st));
+    if (rnd) {
+      drnd.alloc((size_t)MM * 8);
+      NUDIM_HIP(hipMemcpyAsync(drnd.p, rnd, (size_t)MM * 8, hipMemcpyHostToDevice, st));
+    }
+    projection_ft_dev(nx, P, N, nmeas, dproj.as<double>(), dtomo.as<cx<double>>(), st);
+    hipLaunchKernelGGL(k_assemble<T>, dim3(cdiv(MM, 256)), dim3(256), 0, st, dtomo.as<cx<double>>(),
+                       dkind.as<int>(), dorder.as<int>(), drnd.as<double>(), meas.as<cx<T>>(), M, N, P);
+    NUDIM_HIP(hipGetLastError());
+    NUDIM_HIP(hipStreamSynchronize(st));
+  }
+
+  void zfft_inverse(const cx<T> *g, T *out) {
+    int Gs = line_chunk_log2(P, 1 << 30, (int)sizeof(cx<T>), FftEpt<T>::value);
+    long long nlines = (long long)N * N;
+    size_t lds = (size_t)P * (1 << Gs) * sizeof(cx<T>);
+    ProfScope ps(KC_ZFFT, st, 2.0 * V * sizeof(cx<T>));
+    hipLaunchKernelGGL(k_zfft_inv<T>, dim3(cdiv(nlines, 1 << Gs)), dim3(kFftThreads), lds, st, g, out, fdz,
+                       twz.as<cx<T>>(), Gs, nlines);
+    NUDIM_HIP(hipGetLastError());
+  }
+
+  void zfft_forward(const T *in, cx<T> *fh, cx<T> *f0) {
+    int Gs = line_chunk_log2(P, 1 << 30, (int)sizeof(cx<T>), FftEpt<T>::value);
+    long long nlines = (long long)N * N;
+    size_t lds = (size_t)P * (1 << Gs) * sizeof(cx<T>);
+    ProfScope ps(KC_ZFFT, st, 2.0 * V * sizeof(cx<T>));
+    hipLaunchKernelGGL(k_zfft_fwd<T>, dim3(cdiv(nlines, 1 << Gs)), dim3(kFftThreads), lds, st, in, fh, f0, fdz,
+                       twz.as<cx<T>>(), Gs, nlines);
+    NUDIM_HIP(hipGetLastError());
+  }
+
+  void initial() override {
+    hipLaunchKernelGGL(k_zero_cx<T>, dim3(cdiv(NN, 256)), dim3(256), 0, st, F0.as<cx<T>>(), NN);
+    cg.solve(meas.as<cx<T>>(), F0.as<cx<T>>(), 10, true, st);
+    zfft_inverse(F0.as<cx<T>>(), img);
+    NUDIM_HIP(hipStreamSynchronize(st));
+  }
+
+  void tv(T *u) {
+    int ylo = 0, yhi = N - 1;
+    if (prm.config_height) {
+      ylo = std::max(ylo, (2 * prm.ncenter - prm.nheight) / 2);
+      yhi = std::min(yhi, (2 * prm.ncenter + prm.nheight) / 2);
+    }
+    if (yhi < ylo) return;
+    long long total = (long long)(yhi - ylo + 1) * N * P;
+    double lam = (double)0.2f;
+    const double vb = (double)total * sizeof(T);  // algorithmic: read u once + write u once per sweep
+    for (int s = 0; s < 40; s++) {
+      {
+        ProfScope ps(KC_TV, st, vb);
+        hipLaunchKernelGGL(k_tv_grad<T>, dim3(kTvBlocks), dim3(kThreads), 0, st, u, grad.as<T>(), N, N, P, ylo, yhi,
+                           tvpart_g.as<double>(), tvpart_a.as<double>());
+      }
+      hipLaunchKernelGGL(k_sum_scalars, dim3(1), dim3(1), 0, st, tvpart_g.as<double>(), tvpart_a.as<double>(),
+                         kTvBlocks, tvsums.as<double>());
+      {
+        ProfScope ps(KC_TV, st, vb);
+        hipLaunchKernelGGL(k_tv_update<T>, dim3(cdiv(total, 256)), dim3(256), 0, st, u, grad.as<T>(), N, P, ylo,
+                           yhi, tvsums.as<double>(), lam);
+      }
+    }
+    NUDIM_HIP(hipGetLastError());
+  }
+
+  void diffusion(T *u) {
+    const int niter = 80;
+    double lam = (double)0.25f, kappa = (double)0.005f;
+    int ny = N, nx = N;
+    int taumax = 2 * (niter - 1) + (ny - 1) + (nx - 1);
+    for (int tau = 0; tau <= taumax; tau++) {
+      int smin = tau - (ny - 1) - (nx - 1);
+      smin = smin > 0 ? (smin + 1) / 2 : 0;
+      int smax = std::min(niter - 1, tau / 2);
+      if (smax < smin) continue;
+      long long cnt = (long long)(smax - smin + 1) * ny * P;
+      hipLaunchKernelGGL(k_diffusion_tau<T>, dim3(cdiv(cnt, 256)), dim3(256), 0, st, u, ny, nx, P, niter, tau, lam,
+                         kappa * kappa);
+    }
+    NUDIM_HIP(hipGetLastError());
+  }
+
+  void copy_to_host_ref(const T *u, double *out) {
+    DevBuf tmp(V * 8);
+    hipLaunchKernelGGL(k_vol_to_ref<T>, dim3(cdiv(V, 256)), dim3(256), 0, st, u, tmp.as<double>(), N, N, P);
+    NUDIM_HIP(hipMemcpyAsync(out, tmp.p, V * 8, hipMemcpyDeviceToHost, st));
+    NUDIM_HIP(hipStreamSynchronize(st));
+  }
+
+  void iterate(int nn, double *rslot) override {
+    std::swap(img, img_last);  // img_last = img (reconstruction.c:551)
+    zfft_forward(img_last, Fh.as<cx<T>>(), F0.as<cx<T>>());
+    cx<T> *v = cg.v.template as<cx<T>>();
+    op.trafo(Fh.as<cx<T>>(), v, st);
+    cg.red.template rfactor<T>(v, meas.as<cx<T>>(), dkind.as<int>(), M, d1.as<double>(), d2.as<double>(), st);
+    hipLaunchKernelGGL(k_projector<T>, dim3(cdiv(MM, 256)), dim3(256), 0, st, v, meas.as<cx<T>>(), dkind.as<int>(),
+                       y.as<cx<T>>(), MM, P);
+    hipLaunchKernelGGL(k_rfactor_final, dim3(1), dim3(1), 0, st, d1.as<double>(), d2.as<double>(), P, rslot);
+    NUDIM_HIP(hipGetLastError());
+    cg.solve(y.as<cx<T>>(), F0.as<cx<T>>(), 10, false, st);
+    zfft_inverse(F0.as<cx<T>>(), img);
+    if (prm.denoise) {
+      if (nn == dump_nn && dump_before) copy_to_host_ref(img, dump_before);
+      if (prm.method == NUDIM_DIFFUSION)
+        diffusion(img);
+      else
+        tv(img);
+      if (nn == dump_nn && dump_after) copy_to_host_ref(img, dump_after);
+    }
+    hipLaunchKernelGGL(k_constraint<T>, dim3(cdiv(V, 256)), dim3(256), 0, st, img, img_last, N, P, prm.nx,
+                       nn < prm.nhio ? 1 : 0, prm.config_height, prm.nheight, prm.ncenter);
+    NUDIM_HIP(hipGetLastError());
+  }
+
+  void get_volume(double *out) override { copy_to_host_ref(img, out); }
+
+  void set_volume(const double *in) override {
+    DevBuf tmp(V * 8);
+    NUDIM_HIP(hipMemcpyAsync(tmp.p, in, V * 8, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_vol_from_ref<T>, dim3(cdiv(V, 256)), dim3(256), 0, st, tmp.as<double>(), img, N, N, P);
+    NUDIM_HIP(hipGetLastError());
+    NUDIM_HIP(hipStreamSynchronize(st));
+  }
+};
+
+struct nudim_ctx {
+  nudim_params p{};
+  Geometry geo;
+  hipStream_t st = nullptr;
+  std::unique_ptr<JobBase> job;
+  int nn = 0;
+  bool loaded = false, initialized = false;
+  DevBuf rtrace;
+  int rcap = 0;
+  KernelProfiler prof;
+};
+
+// makes the context's profiler current for the calling thread
+struct ProfBind {
+  KernelProfiler *old;
+  explicit ProfBind(nudim_ctx *c) : old(g_prof) { g_prof = c->prof.cls ? &c->prof : nullptr; }
+  ~ProfBind() { g_prof = old; }
+};
+
+extern "C" void nudim_default_params(nudim_params *p) {
+  std::memset(p, 0, sizeof(*p));
+  p->nx = p->nz = p->np = -1;
+  p->gam_min = p->gam_max = 100;
+  p->interval = -1;
+  p->padangle = -1;
+  p->nheight = p->ncenter = -1;
+  p->nhio = p->ner = -1;
+  p->method = NUDIM_TV;
+  p->precision = NUDIM_FP64_PLAIN;
+}
+
+extern "C" int nudim_create(const nudim_params *p, nudim_ctx **out) {
+  API_BEGIN
+  NUDIM_CHECK(out, "null output pointer");
+  *out = nullptr;
+  validate(p);
+  std::unique_ptr<nudim_ctx> c(new nudim_ctx());
+  c->p = *p;
+  NUDIM_HIP(hipSetDevice(p->device));
+  NUDIM_HIP(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+  make_geometry(p, &c->geo);
+  if (p->precision == NUDIM_FP64_PLAIN)
+    c->job.reset(new Job<double>(c->p, c->geo, c->st));
+  else
+    c->job.reset(new Job<float>(c->p, c->geo, c->st));
+  NUDIM_HIP(hipStreamSynchronize(c->st));
+  *out = c.release();
+  return NUDIM_OK;
+  API_END
+}
+
+extern "C" void nudim_destroy(nudim_ctx *c) {
+  if (!c) return;
+  try {
+    (void)hipSetDevice(c->p.device);
+    if (c->st) (void)hipStreamSynchronize(c->st);
+    c->job.reset();
+    c->rtrace.release();
+    if (c->st) (void)hipStreamDestroy(c->st);
+  } catch (...) {
+  }
+  delete c;
+}
+
+extern "C" int nudim_sizes(const nudim_ctx *c, int *nmeas, int *M, int *nang) {
+  if (!c) return fail(NUDIM_EINVAL, "null context");
+  if (nmeas) *nmeas = c->geo.nmeas;
+  if (M) *M = c->geo.M;
+  if (nang) *nang = c->geo.nang;
+  return NUDIM_OK;
+}
+
+extern "C" int nudim_load_projections(nudim_ctx *c, const double *proj) {
+  API_BEGIN
+  NUDIM_CHECK(c && proj, "null argument");
+  NUDIM_HIP(hipSetDevice(c->p.device));
+  std::vector<double> rnd;
+  if (c->p.random) {
+    // gsl_rng_uniform(rng)*10 for every unknown and held-out sample, planes
+    // outer, nodes inner (reconstruction.c:437-469)
+    Mt19937 mt(c->p.seed);
+    rnd.assign((size_t)c->geo.M * c->p.nz, 0.0);
+    for (int j = 0; j < c->p.nz; j++)
+      for (int i = 0; i < c->geo.M; i++)
+        if (c->geo.kind[i] != 1) rnd[(size_t)j * c->geo.M + i] = mt.uniform() * 10;
+  }
+  c->job->load_projections(proj, rnd.empty() ? nullptr : rnd.data());
+  c->loaded = true;
+  return NUDIM_OK;
+  API_END
+}
+
+extern "C" int nudim_load_samples(nudim_ctx *c, const double *slices, const int *kind) {
+  API_BEGIN
+  NUDIM_CHECK(c && slices, "null argument");
+  NUDIM_HIP(hipSetDevice(c->p.device));
+  if (kind) {
+    for (int i = 0; i < c->geo.M; i++)
+      NUDIM_CHECK(kind[i] == c->geo.kind[i], "sample classes differ from the job geometry");
+  }
+  c->job->load_meas_planes(slices);
+  c->loaded = true;
+  return NUDIM_OK;
+  API_END
+}
+
+extern "C" int nudim_initial(nudim_ctx *c) {
+  API_BEGIN
+  NUDIM_CHECK(c, "null context");
+  if (!c->loaded) return fail(NUDIM_ESTATE, "load projections before nudim_initial");
+  NUDIM_HIP(hipSetDevice(c->p.device));
+  c->job->initial();
+  c->initialized = true;
+  c->nn = 0;
+  return NUDIM_OK;
+  API_END
+}
+
+extern "C" int nudim_iterate(nudim_ctx *c, int count, double *rfactor) {
+  API_BEGIN
+  NUDIM_CHECK(c && count >= 0, "invalid argument");
+  if (!c->initialized) return fail(NUDIM_ESTATE, "call nudim_initial (or nudim_set_volume) first");
+  NUDIM_HIP(hipSetDevice(c->p.device));
+  if (c->rcap < count) {
+    c->rtrace.alloc((size_t)count * 8);
+    c->rcap = count;
+  }
+  {
+    ProfBind pb(c);
+    for (int i = 0; i < count; i++) c->job->iterate(c->nn + i, c->rtrace.as<double>() + i);
+  }
+  if (rfactor && count > 0)
+    NUDIM_HIP(hipMemcpyAsync(rfactor, c->rtrace.p, (size_t)count * 8, hipMemcpyDeviceToHost, c->st));
+  NUDIM_HIP(hipStreamSynchronize(c->st));
+  c->prof.collect();
+  c->nn += count;
+  return NUDIM_OK;
+  API_END
+}
+
+extern "C" int nudim_get_volume(nudim_ctx *c, double *img) {
+  API_BEGIN
+  NUDIM_CHECK(c && img, "null argument");
+  NUDIM_HIP(hipSetDevice(c->p.device));
+  c->job->get_volume(img);
+  return NUDIM_OK;
+  API_END
+}
+
+extern "C" int nudim_set_volume(nudim_ctx *c, const double *img, int nn) {
+  API_BEGIN
+  NUDIM_CHECK(c && img && nn >= 0, "invalid argument");
+  NUDIM_HIP(hipSetDevice(c->p.device));
+  c->job->set_volume(img);
+  c->nn = nn;
+  c->initialized = true;
+  return NUDIM_OK;
+  API_END
+}
+
+extern "C" int nudim_set_dump(nudim_ctx *c, int nn, double *before, double *after) {
+  if (!c) return fail(NUDIM_EINVAL, "null context");
+  c->job->dump_nn = nn;
+  c->job->dump_before = before;
+  c->job->dump_after = after;
+  return NUDIM_OK;
+}
+
+extern "C" int nudim_profile(nudim_ctx *c, int kernel_class) {
+  if (!c || kernel_class < 0 || kernel_class > KC_ZFFT) return fail(NUDIM_EINVAL, "invalid profile request");
+  c->prof.reset(kernel_class);
+  return NUDIM_OK;
+}
+
+extern "C" int nudim_profile_read(nudim_ctx *c, double *ms, long *launches, double *bytes) {
+  if (!c) return fail(NUDIM_EINVAL, "null context");
+  if (ms) *ms = c->prof.ms;
+  if (launches) *launches = c->prof.launches;
+  if (bytes) *bytes = c->prof.bytes;
+  return NUDIM_OK;
+}
+
+extern "C" size_t nudim_device_bytes(const nudim_ctx *c) { return c && c->job ? c->job->bytes() : 0; }
+
+extern "C" int nudim_synchronize(nudim_ctx *c) {
+  API_BEGIN
+  NUDIM_CHECK(c, "null context");
+  NUDIM_HIP(hipStreamSynchronize(c->st));
+  return NUDIM_OK;
+  API_END
+}
+
+extern "C" int nudim_estimate_bytes(const nudim_params *p, size_t *bytes) {
+  API_BEGIN
+  validate(p);
+  NUDIM_CHECK(bytes, "null output");
+  Geometry g;
+  make_geometry(p, &g);
+  int n, W, form;
+  resolve_knobs(p, &n, &W, &form);
+  size_t cb = p->precision == NUDIM_FP64_PLAIN ? 16 : 8, rb = cb / 2;
+  size_t N = p->np, P = p->nz, M = g.M;
+  size_t NN = N * N * P, MM = M * P;
+  size_t b = 0;
+  b += (size_t)N * n * P * cb + (size_t)n * n * P * cb;           // NFFT X, grid
+  b += (size_t)M * W * W * 4 + (size_t)n * n * 4 + M * W * 16;      // gather map, window tables
+  if (form == NUDIM_CG_TOEPLITZ) b += (size_t)4 * N * N * cb + (size_t)N * 2 * N * P * cb + NN * cb;
+  b += 2 * NN * cb + 3 * MM * cb;                                   // CG z, p; r, v, t
+  b += 2 * MM * cb + 2 * NN * cb;                                   // meas, y, Fh, F0
+  b += 2 * NN * rb + ((p->denoise && p->method == NUDIM_TV) ? NN * rb : 0);
+  *bytes = b;
+  return NUDIM_OK;
+  API_END
+}
+
+// ---------------------------------------------------------------------------
+// stateless parity entry points
+// ---------------------------------------------------------------------------
+
+struct TmpStream {
+  hipStream_t s = nullptr;
+  TmpStream() { NUDIM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
+  ~TmpStream() {
+    if (s) {
+      (void)hipStreamSynchronize(s);
+      (void)hipStreamDestroy(s);
+    }
+  }
+};
+
+template <typename T>
+static void to_dev_planes(const double *h, long long K, int P, DevBuf &out, hipStream_t st) {
+  DevBuf tmp((size_t)K * P * 16);
+  NUDIM_HIP(hipMemcpyAsync(tmp.p, h, (size_t)K * P * 16, hipMemcpyHostToDevice, st));
+  out.alloc((size_t)K * P * sizeof(cx<T>));
+  hipLaunchKernelGGL(k_cx_from_planes<T>, dim3(cdiv(K * P, 256)), dim3(256), 0, st, tmp.as<double>(), out.as<cx<T>>(),
+                     K, P);
+  NUDIM_HIP(hipGetLastError());
+  NUDIM_HIP(hipStreamSynchronize(st));
+}
+
+template <typename T>
+static void from_dev_planes(const DevBuf &d, long long K, int P, double *h, hipStream_t st) {
+  DevBuf tmp((size_t)K * P * 16);
+  hipLaunchKernelGGL(k_cx_to_planes<T>, dim3(cdiv(K * P, 256)), dim3(256), 0, st, d.as<cx<T>>(), tmp.as<double>(), K,
+                     P);
+  NUDIM_HIP(hipGetLastError());
+  NUDIM_HIP(hipMemcpyAsync(h, tmp.p, (size_t)K * P * 16, hipMemcpyDeviceToHost, st));
+  NUDIM_HIP(hipStreamSynchronize(st));
+}
+
+template <typename T>
+static void nfft_parity(int N, int M, const double *x, int n, int W, int P, const double *in, double *out, bool adj) {
+  TmpStream ts;
+  NfftOp<T> op;
+  op.init(N, n, W, x, M, P);
+  DevBuf a, b((size_t)(adj ? (long long)N * N : M) * P * sizeof(cx<T>));
+  if (!adj) {
+    to_dev_planes<T>(in, (long long)N * N, P, a, ts.s);
+    op.trafo(a.as<cx<T>>(), b.as<cx<T>>(), ts.s);
+    from_dev_planes<T>(b, M, P, out, ts.s);
+  } else {
+    to_dev_planes<T>(in, M, P, a, ts.s);
+    op.adjoint(a.as<cx<T>>(), b.as<cx<T>>(), ts.s);
+    from_dev_planes<T>(b, (long long)N * N, P, out, ts.s);
+  }
+}
+
+extern "C" int nudim_nfft_trafo(int precision, int N, int M, const double *x, int n, int W, int P,
+                                const double *fhat, double *f) {
+  API_BEGIN
+  NUDIM_CHECK(x && fhat && f && N > 0 && M > 0 && P > 0, "invalid argument");
+  if (precision == NUDIM_FP32) nfft_parity<float>(N, M, x, n, W, P, fhat, f, false);
+  else nfft_parity<double>(N, M, x, n, W, P, fhat, f, false);
+  return NUDIM_OK;
+  API_END
+}
+
+extern "C" int nudim_nfft_adjoint(int precision, int N, int M, const double *x, int n, int W, int P,
+                                  const double *f, double *fhat) {
+  API_BEGIN
+  NUDIM_CHECK(x && fhat && f && N > 0 && M > 0 && P > 0, "invalid argument");
+  if (precision == NUDIM_FP32) nfft_parity<float>(N, M, x, n, W, P, f, fhat, true);
+  else nfft_parity<double>(N, M, x, n, W, P, f, fhat, true);
+  return NUDIM_OK;
+  API_END
+}
+
+template <typename T>
+static void cgnr_parity(int form, int N, int M, const double *x, const double *w, int n, int W, int P,
+                        const double *y, double *fhat, int nsteps) {
+  TmpStream ts;
+  NfftOp<T> op;
+  op.init(N, n, W, x, M, P);
+  ToeplitzOp<T> toep;
+  if (form == NUDIM_CG_TOEPLITZ) toep.init(N, x, w, M, P, ts.s);
+  DevBuf dw;
+  if (w) {
+    dw.alloc((size_t)M * 8);
+    NUDIM_HIP(hipMemcpy(dw.p, w, (size_t)M * 8, hipMemcpyHostToDevice));
+  }
+  Cgnr<T> cg;
+  cg.init(N, M, P, form, &op, form == NUDIM_CG_TOEPLITZ ? &toep : nullptr, dw.as<double>());
+  DevBuf dy, df;
+  to_dev_planes<T>(y, M, P, dy, ts.s);
+  to_dev_planes<T>(fhat, (long long)N * N, P, df, ts.s);
+  cg.solve(dy.as<cx<T>>(), df.as<cx<T>>(), nsteps, false, ts.s);
+  from_dev_planes<T>(df, (long long)N * N, P, fhat, ts.s);
+}
+
+extern "C" int nudim_cgnr(int precision, int form, int N, int M, const double *x, const double *w, int n, int W,
+                          int P, const double *y, double *fhat, int nsteps) {
+  API_BEGIN
+  NUDIM_CHECK(x && y && fhat && N > 0 && M > 0 && P > 0 && nsteps >= 0, "invalid argument");
+  if (form <= 0) form = precision == NUDIM_FP32 ? NUDIM_CG_TOEPLITZ : NUDIM_CG_NFFT;
+  if (precision == NUDIM_FP32) cgnr_parity<float>(form, N, M, x, w, n, W, P, y, fhat, nsteps);
+  else cgnr_parity<double>(form, N, M, x, w, n, W, P, y, fhat, nsteps);
+  return NUDIM_OK;
+  API_END
+}
+
+template <typename T>
+static void zfft_parity(int len, int howmany, double *data, int sign) {
+  TmpStream ts;
+  LineFft<T> f;
+  f.init(len);
+  DevBuf a, tmp((size_t)len * howmany * 16);
+  NUDIM_HIP(hipMemcpyAsync(tmp.p, data, (size_t)len * howmany * 16, hipMemcpyHostToDevice, ts.s));
+  a.alloc((size_t)len * howmany * sizeof(cx<T>));
+  // reference layout [len][howmany] is already "plane innermost" with P = howmany
+  hipLaunchKernelGGL(k_cx_from_planes<T>, dim3(cdiv((long long)len * howmany, 256)), dim3(256), 0, ts.s,
+                     tmp.as<double>(), a.as<cx<T>>(), (long long)len * howmany, 1);
+  LineIO io{0, howmany, len, 0, nullptr, nullptr};
+  f.run(a.as<cx<T>>(), a.as<cx<T>>(), 1, io, io, sign < 0 ? -1 : +1, howmany, ts.s);
+  hipLaunchKernelGGL(k_cx_to_planes<T>, dim3(cdiv((long long)len * howmany, 256)), dim3(256), 0, ts.s,
+                     a.as<cx<T>>(), tmp.as<double>(), (long long)len * howmany, 1);
+  NUDIM_HIP(hipGetLastError());
+  NUDIM_HIP(hipMemcpyAsync(data, tmp.p, (size_t)len * howmany * 16, hipMemcpyDeviceToHost, ts.s));
+  NUDIM_HIP(hipStreamSynchronize(ts.s));
+}
+
+extern "C" int nudim_zfft(int precision, int len, int howmany, double *data, int sign) {
+  API_BEGIN
+  NUDIM_CHECK(data && len > 0 && howmany > 0, "invalid argument");
+  if (precision == NUDIM_FP32) zfft_parity<float>(len, howmany, data, sign);
+  else zfft_parity<double>(len, howmany, data, sign);
+  return NUDIM_OK;
+  API_END
+}
+
+// Volume-level parity: build a throw-away job-shaped state without geometry.
+template <typename T>
+struct VolTmp {
+  TmpStream ts;
+  DevBuf u, grad, part_g, part_a, sums;
+  int nz, ny, nx;
+  long long V;
+  VolTmp(const double *h, int nz_, int ny_, int nx_) : nz(nz_), ny(ny_), nx(nx_) {
+    V = (long long)nz * ny * nx;
+    DevBuf tmp(V * 8);
+    NUDIM_HIP(hipMemcpyAsync(tmp.p, h, V * 8, hipMemcpyHostToDevice, ts.s));
+    u.alloc(V * sizeof(T));
+    hipLaunchKernelGGL(k_vol_from_ref<T>, dim3(cdiv(V, 256)), dim3(256), 0, ts.s, tmp.as<double>(), u.as<T>(), ny, nx,
+                       nz);
+    NUDIM_HIP(hipGetLastError());
+    NUDIM_HIP(hipStreamSynchronize(ts.s));
+  }
+  void out(double *h) {
+    DevBuf tmp(V * 8);
+    hipLaunchKernelGGL(k_vol_to_ref<T>, dim3(cdiv(V, 256)), dim3(256), 0, ts.s, u.as<T>(), tmp.as<double>(), ny, nx,
+                       nz);
+    NUDIM_HIP(hipGetLastError());
+    NUDIM_HIP(hipMemcpyAsync(h, tmp.p, V * 8, hipMemcpyDeviceToHost, ts.s));
+    NUDIM_HIP(hipStreamSynchronize(ts.s));
+  }
+  void tv(float lam, int niter, int ylo, int yhi) {
+    ylo = std::max(ylo, 0);
+    yhi = std::min(yhi, ny - 1);
+    if (yhi < ylo) return;
+    grad.alloc(V * sizeof(T));
+    const int nb = 1024;
+    part_g.alloc(nb * 8);
+    part_a.alloc(nb * 8);
+    sums.alloc(16);
+    long long total = (long long)(yhi - ylo + 1) * nx * nz;
+    for (int s = 0; s < niter; s++) {
+      hipLaunchKernelGGL(k_tv_grad<T>, dim3(nb), dim3(kThreads), 0, ts.s, u.as<T>(), grad.as<T>(), ny, nx, nz, ylo, yhi,
+                         part_g.as<double>(), part_a.as<double>());
+      hipLaunchKernelGGL(k_sum_scalars, dim3(1), dim3(1), 0, ts.s, part_g.as<double>(), part_a.as<double>(), nb,
+                         sums.as<double>());
+      hipLaunchKernelGGL(k_tv_update<T>, dim3(cdiv(total, 256)), dim3(256), 0, ts.s, u.as<T>(), grad.as<T>(), nx, nz,
+                         ylo, yhi, sums.as<double>(), (double)lam);
+    }
+    NUDIM_HIP(hipGetLastError());
+  }
+};
+
+extern "C" int nudim_tv(int precision, double *u, int nz, int ny, int nx, float lam, int niter) {
+  API_BEGIN
+  NUDIM_CHECK(u && nz > 0 && ny > 0 && nx > 0 && niter >= 0, "invalid argument");
+  if (precision == NUDIM_FP32) {
+    VolTmp<float> v(u, nz, ny, nx);
+    v.tv(lam, niter, 0, ny - 1);
+    v.out(u);
+  } else {
+    VolTmp<double> v(u, nz, ny, nx);
+    v.tv(lam, niter, 0, ny - 1);
+    v.out(u);
+  }
+  return NUDIM_OK;
+  API_END
+}
+
+extern "C" int nudim_tv_height(int precision, double *u, int nz, int ny, int nx, float lam, int niter, int nheight,
+                               int ncenter) {
+  API_BEGIN
+  NUDIM_CHECK(u && nz > 0 && ny > 0 && nx > 0 && niter >= 0, "invalid argument");
+  int ylo = (2 * ncenter - nheight) / 2, yhi = (2 * ncenter + nheight) / 2;
+  if (precision == NUDIM_FP32) {
+    VolTmp<float> v(u, nz, ny, nx);
+    v.tv(lam, niter, ylo, yhi);
+    v.out(u);
+  } else {
+    VolTmp<double> v(u, nz, ny, nx);
+    v.tv(lam, niter, ylo, yhi);
+    v.out(u);
+  }
+  return NUDIM_OK;
+  API_END
+}
+
+template <typename T>
+static void diffusion_parity(double *u, int nz, int ny, int nx, float lambda, float kappa, int niter) {
+  VolTmp<T> v(u, nz, ny, nx);
+  double lam = (double)lambda, k = (double)kappa;
+  int taumax = 2 * (niter - 1) + (ny - 1) + (nx - 1);
+  for (int tau = 0; niter > 0 && tau <= taumax; tau++) {
+    int smin = tau - (ny - 1) - (nx - 1);
+    smin = smin > 0 ? (smin + 1) / 2 : 0;
+    int smax = std::min(niter - 1, tau / 2);
+    if (smax < smin) continue;
+    long long cnt = (long long)(smax - smin + 1) * ny * nz;
+    hipLaunchKernelGGL(k_diffusion_tau<T>, dim3(cdiv(cnt, 256)), dim3(256), 0, v.ts.s, v.u.template as<T>(), ny, nx, nz,
+                       niter, tau, lam, k * k);
+  }
+  NUDIM_HIP(hipGetLastError());
+  v.out(u);
+}
+
+extern "C" int nudim_diffusion2d(int precision, double *u, int nz, int ny, int nx, float lambda, float kappa,
+                                 int niter) {
+  API_BEGIN
+  NUDIM_CHECK(u && nz > 0 && ny > 0 && nx > 0 && niter >= 0, "invalid argument");
+  NUDIM_CHECK(nz == nx, "aniso_diffusion_2d needs nz == nx (denoise.c:349-350 clamps x with the z extent)");
+  if (precision == NUDIM_FP32) diffusion_parity<float>(u, nz, ny, nx, lambda, kappa, niter);
+  else diffusion_parity<double>(u, nz, ny, nx, lambda, kappa, niter);
+  return NUDIM_OK;
+  API_END
+}
+
+template <typename T>
+static void constraint_parity(double *img, const double *last, int nz, int np, int nx, int is_hio, int ch, int nh,
+                              int nc) {
+  VolTmp<T> a(img, nz, np, np), b(last, nz, np, np);
+  NUDIM_HIP(hipStreamSynchronize(b.ts.s));
+  hipLaunchKernelGGL(k_constraint<T>, dim3(cdiv(a.V, 256)), dim3(256), 0, a.ts.s, a.u.template as<T>(),
+                     b.u.template as<T>(), np, nz, nx, is_hio, ch, nh, nc);
+  NUDIM_HIP(hipGetLastError());
+  a.out(img);
+}
+
+extern "C" int nudim_constraint(int precision, double *img, const double *img_last, int nz, int np, int nx,
+                                int is_hio, int config_height, int nheight, int ncenter) {
+  API_BEGIN
+  NUDIM_CHECK(img && img_last && nz > 0 && np > 0, "invalid argument");
+  if (precision == NUDIM_FP32) constraint_parity<float>(img, img_last, nz, np, nx, is_hio, config_height, nheight, ncenter);
+  else constraint_parity<double>(img, img_last, nz, np, nx, is_hio, config_height, nheight, ncenter);
+  return NUDIM_OK;
+  API_END
+}
+
+extern "C" int nudim_projection_ft(int precision, int nx, int nz, int np, int nmeas, const double *proj,
+                                   double *tomo) {
+  API_BEGIN
+  (void)precision;  // set-up arithmetic is FP64 on both paths
+  NUDIM_CHECK(proj && tomo && nx > 0 && nz > 0 && np > 0 && nmeas > 0, "invalid argument");
+  TmpStream ts;
+  size_t pb = (size_t)nmeas * nz * nx * 8, tb = (size_t)nmeas * nz * np * 16;
+  DevBuf dp(pb), dt(tb);
+  NUDIM_HIP(hipMemcpyAsync(dp.p, proj, pb, hipMemcpyHostToDevice, ts.s));
+  projection_ft_dev(nx, nz, np, nmeas, dp.as<double>(), dt.as<cx<double>>(), ts.s);
+  NUDIM_HIP(hipMemcpyAsync(tomo, dt.p, tb, hipMemcpyDeviceToHost, ts.s));
+  NUDIM_HIP(hipStreamSynchronize(ts.s));
+  return NUDIM_OK;
+  API_END
+}
+
+extern "C" int nudim_project_2d1d(int nx, int ny, int nz, int np, double gam_min, double gam_max, double interval,
+                                  const double *density, double *proj) {
+  (void)nx; (void)ny; (void)nz; (void)np; (void)gam_min; (void)gam_max; (void)interval; (void)density; (void)proj;
+  return fail(NUDIM_EINVAL, "nudim_project_2d1d: not built yet");
+}

@@ -1,0 +1,82 @@
+// engine.hpp -- device-side operators of one reconstruction job.
+#pragma once
+
+#include "common.hpp"
+#include "nudim.h"
+#include "fft.hpp"
+#include "geom.hpp"
+#include "kernels.hpp"
+
+namespace nudim {
+
+// choose log2 of the plane chunk G of a line transform of length n over P planes
+int line_chunk_log2(int n, int P, int elem_bytes, int ept);
+
+template <typename T>
+struct LineFft {
+  FftDesc fd{};
+  DevBuf tw;  // cx<T>[n]
+  int n = 0;
+  void init(int len);
+  // one batched line transform (see k_fft_lines)
+  void run(const cx<T> *in, cx<T> *out, int nlines, const LineIO &li, const LineIO &lo, int sign, int P,
+           hipStream_t st, const cx<T> *mid = nullptr, long long mid_so = 0) const;
+};
+
+// Square N x N NFFT with plane batch P (all planes share the nodes).
+template <typename T>
+struct NfftOp {
+  int N = 0, n = 0, W = 0, M = 0, P = 0;
+  LineFft<T> fft;
+  DevBuf c0, c1, psi0, psi1, dec, cell_ptr, ent;
+  DevBuf X, grid;  // [N][n][P], [n][n][P]
+  size_t bytes() const;
+  void init(int N, int n, int W, const double *x, int M, int P);
+  void trafo(const cx<T> *fhat, cx<T> *f, hipStream_t st);
+  void adjoint(const cx<T> *f, cx<T> *fhat, hipStream_t st);
+};
+
+// Normal operator A^H W A as a 2N x 2N circulant embedding of its Toeplitz
+// kernel T(d) = sum_m w_m exp(2 pi i d.x_m).
+template <typename T>
+struct ToeplitzOp {
+  int N = 0, L = 0, P = 0;
+  LineFft<T> fft;
+  DevBuf khat;  // cx<T>[L][L], stored [column][row], scaled by 1/L^2
+  DevBuf X;     // [N][L][P]
+  size_t bytes() const;
+  void init(int N, const double *x, const double *w, int M, int P, hipStream_t st);
+  void apply(const cx<T> *p, cx<T> *q, hipStream_t st);
+};
+
+// per-plane reduction helpers
+struct Reducer {
+  DevBuf part, part2;  // [kRedBlocks][P] doubles
+  int P = 0;
+  void init(int P);
+  template <typename T>
+  void dot(const cx<T> *a, const double *w, long long rows, double *out, hipStream_t st);
+  template <typename T>
+  void redot(const cx<T> *a, const cx<T> *b, long long rows, double *out, hipStream_t st);
+  template <typename T>
+  void rfactor(const cx<T> *f, const cx<T> *meas, const int *kind, long long rows, double *d1, double *d2,
+               hipStream_t st);
+};
+
+// CGNR state for P planes
+template <typename T>
+struct Cgnr {
+  int N = 0, M = 0, P = 0, form = 1;
+  NfftOp<T> *op = nullptr;
+  ToeplitzOp<T> *toep = nullptr;
+  const double *w = nullptr;  // device weights [M] or null
+  DevBuf z, pv, q, r, v, t;   // z, p, q: [N^2][P]; r, v, t: [M][P]
+  DevBuf dot_z, coef, sc;     // [P] each
+  Reducer red;
+  size_t bytes() const;
+  void init(int N, int M, int P, int form, NfftOp<T> *op, ToeplitzOp<T> *toep, const double *w);
+  // solver_before_loop_complex + nsteps loop_one_step (fhat in/out; f0_zero skips A*0)
+  void solve(const cx<T> *y, cx<T> *fhat, int nsteps, bool f0_zero, hipStream_t st);
+};
+
+}  // namespace nudim

@@ -1,0 +1,398 @@
+// kernels.hpp -- NFFT window kernels, CGNR vector kernels, per-plane
+// reductions, TV / diffusion stencils and the density constraint.
+//
+// Layouts (P = planes of the job, innermost everywhere on the Fourier side):
+//   Fourier planes     f[y][x][p]      complex   (N x N x P)
+//   oversampled grid   g[r][c][p]      complex   (n x n x P)
+//   polar samples      s[m][p]         complex   (M x P)
+//   real volume        u[y][x][z]      real      (N x N x nz)
+#pragma once
+
+#include "common.hpp"
+
+namespace nudim {
+
+constexpr int kThreads = 256;
+constexpr int kRedBlocks = 240;  // partial-sum rows of the per-plane reductions
+
+// ---------------------------------------------------------------------------
+// NFFT window interpolation (forward transform) and gather adjoint
+// ---------------------------------------------------------------------------
+
+// out[m][p] = sum_{t0,t1} psi0[m][t0] psi1[m][t1] g[(c0+t0) mod n][(c1+t1) mod n][p]
+// One group of PC = 2^PCs lanes per node (lanes over planes); the window
+// weights are uniform over the group and the grid reads of one tap are PC
+// contiguous complex values.
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_interp(const cx<T> *__restrict__ g, cx<T> *__restrict__ out, const int *__restrict__ c0,
+         const int *__restrict__ c1, const double *__restrict__ psi0, const double *__restrict__ psi1,
+         int M, int n, int W, int P, int PCs) {
+  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  const int PC = 1 << PCs;
+  const int node = lb * (kThreads >> PCs) + (threadIdx.x >> PCs);
+  const int p = blockIdx.y * PC + (threadIdx.x & (PC - 1));
+  if (node >= M) return;
+  const int b0 = c0[node], b1 = c1[node];
+  const double *a0 = psi0 + (size_t)node * W, *a1 = psi1 + (size_t)node * W;
+  const size_t rs = (size_t)n * P;
+  cx<T> acc = mk((T)0, (T)0);
+  for (int t0 = 0; t0 < W; t0++) {
+    int r = b0 + t0;
+    r += r < 0 ? n : 0;
+    r -= r >= n ? n : 0;
+    const cx<T> *row = g + r * rs + p;
+    cx<T> racc = mk((T)0, (T)0);
+    for (int t1 = 0; t1 < W; t1++) {
+      int c = b1 + t1;
+      c += c < 0 ? n : 0;
+      c -= c >= n ? n : 0;
+      racc = racc + (T)a1[t1] * row[(size_t)c * P];
+    }
+    acc = acc + (T)a0[t0] * racc;
+  }
+  out[(size_t)node * P + p] = acc;
+}
+
+// g[cell][p] = sum over the cell's taps (node, t0, t1) of
+//              psi0[node][t0] psi1[node][t1] s[node][p]
+// in node order: a deterministic gather, no atomics.  Writes every cell.
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_gather(const cx<T> *__restrict__ s, cx<T> *__restrict__ g, const int *__restrict__ cell_ptr,
+         const uint32_t *__restrict__ ent, const double *__restrict__ psi0,
+         const double *__restrict__ psi1, int ncell, int W, int P, int PCs) {
+  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  const int PC = 1 << PCs;
+  const int cell = lb * (kThreads >> PCs) + (threadIdx.x >> PCs);
+  const int p = blockIdx.y * PC + (threadIdx.x & (PC - 1));
+  if (cell >= ncell) return;
+  const int e0 = cell_ptr[cell], e1 = cell_ptr[cell + 1];
+  cx<T> acc = mk((T)0, (T)0);
+  for (int e = e0; e < e1; e++) {
+    uint32_t u = ent[e];
+    int node = (int)(u & 0xFFFFFu), t0 = (int)((u >> 20) & 63u), t1 = (int)(u >> 26);
+    T w = (T)(psi0[(size_t)node * W + t0] * psi1[(size_t)node * W + t1]);
+    acc = acc + w * s[(size_t)node * P + p];
+  }
+  g[(size_t)cell * P + p] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// per-plane reductions: rows x P arrays, FP64 partial sums [kRedBlocks][P]
+// ---------------------------------------------------------------------------
+
+// block (bx, by): plane chunk by (PC planes), rows bx, bx + gridDim.x, ...
+// handled by lane groups; partial[bx][p] (and partial2) written.
+template <typename T, int MODE>
+__global__ void __launch_bounds__(kThreads)
+k_reduce(const cx<T> *__restrict__ a, const cx<T> *__restrict__ b, const double *__restrict__ w,
+         const int *__restrict__ kind, long long rows, int P, int PCs, double *__restrict__ part,
+         double *__restrict__ part2) {
+  // MODE 0: sum_k w_k |a|^2 (w may be null)
+  // MODE 1: R-factor sums over kind==1 rows: |a-b|^2 and |a+b|^2
+  // MODE 2: Re <a, b> = sum Re(conj(a) b)
+  __shared__ double sh[kThreads], sh2[kThreads];
+  const int PC = 1 << PCs, lanes = kThreads >> PCs;
+  const int pl = threadIdx.x & (PC - 1), lr = threadIdx.x >> PCs;
+  const int p = blockIdx.y * PC + pl;
+  double s = 0, s2 = 0;
+  for (long long k = (long long)blockIdx.x * lanes + lr; k < rows; k += (long long)gridDim.x * lanes) {
+    cx<T> v = a[k * P + p];
+    if constexpr (MODE == 0) {
+      double q = (double)v.x * v.x + (double)v.y * v.y;
+      s += w ? w[k] * q : q;
+    } else if constexpr (MODE == 1) {
+      if (kind[k] == 1) {
+        cx<T> m = b[k * P + p];
+        double dr = (double)v.x - (double)m.x, di = (double)v.y - (double)m.y;
+        double sr = (double)v.x + (double)m.x, si = (double)v.y + (double)m.y;
+        s += dr * dr + di * di;
+        s2 += sr * sr + si * si;
+      }
+    } else {
+      cx<T> m = b[k * P + p];
+      s += (double)v.x * m.x + (double)v.y * m.y;
+    }
+  }
+  sh[threadIdx.x] = s;
+  if constexpr (MODE == 1) sh2[threadIdx.x] = s2;
+  __syncthreads();
+  for (int h = lanes >> 1; h > 0; h >>= 1) {
+    if (lr < h) {
+      sh[threadIdx.x] += sh[threadIdx.x + h * PC];
+      if constexpr (MODE == 1) sh2[threadIdx.x] += sh2[threadIdx.x + h * PC];
+    }
+    __syncthreads();
+  }
+  if (lr == 0) {
+    part[(size_t)blockIdx.x * P + p] = sh[pl];
+    if constexpr (MODE == 1) part2[(size_t)blockIdx.x * P + p] = sh2[pl];
+  }
+}
+
+// sums the nb partial rows per plane in order: out[p] = sum_b part[b][p]
+__global__ void k_sum_partials(const double *__restrict__ part, int nb, int P, double *__restrict__ out);
+
+// CGNR scalar steps (NFFT3 solver_loop_one_step_complex, CGNR branch):
+//   mode 0: dot_z[p] = s[p]                                 (before_loop)
+//   mode 1: alpha[p] = dot_z[p] / s[p]                      (s = dot_v)
+//   mode 2: beta[p] = s[p] / dot_z[p]; dot_z[p] = s[p]      (s = new dot_z)
+__global__ void k_cg_scalars(double *__restrict__ dot_z, double *__restrict__ coef,
+                             const double *__restrict__ s, int P, int mode);
+
+// f += alpha_p * p_  (over rows x P)
+template <typename T>
+__global__ void k_axpy_planes(cx<T> *__restrict__ f, const cx<T> *__restrict__ x, const double *__restrict__ coef,
+                              double sgn, long long total, int P) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  int p = (int)(i % P);
+  T a = (T)(sgn * coef[p]);
+  cx<T> v = x[i];
+  f[i] = f[i] + a * v;
+}
+
+// p = beta_p * p + z
+template <typename T>
+__global__ void k_xpby_planes(cx<T> *__restrict__ pv, const cx<T> *__restrict__ z, const double *__restrict__ beta,
+                              long long total, int P) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  int p = (int)(i % P);
+  T b = (T)beta[p];
+  pv[i] = b * pv[i] + z[i];
+}
+
+// r -= alpha_p v ; t = w .* r   (rows = M)
+template <typename T>
+__global__ void k_resid_update(cx<T> *__restrict__ r, const cx<T> *__restrict__ v, const double *__restrict__ alpha,
+                               const double *__restrict__ w, cx<T> *__restrict__ t, long long total, int P) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  int p = (int)(i % P);
+  long long k = i / P;
+  T a = (T)(-alpha[p]);
+  cx<T> rv = r[i] + a * v[i];
+  r[i] = rv;
+  t[i] = w ? (T)w[k] * rv : rv;
+}
+
+// r = y - v ; t = w .* r   (before_loop)
+template <typename T>
+__global__ void k_resid_init(const cx<T> *__restrict__ y, const cx<T> *__restrict__ v, const double *__restrict__ w,
+                             cx<T> *__restrict__ r, cx<T> *__restrict__ t, long long total, int P) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  long long k = i / P;
+  cx<T> rv = v ? y[i] - v[i] : y[i];
+  if (r) r[i] = rv;
+  t[i] = w ? (T)w[k] * rv : rv;
+}
+
+// Fourier projector, reconstruction.c:574-582: y = known ? measured : f
+template <typename T>
+__global__ void k_projector(const cx<T> *__restrict__ f, const cx<T> *__restrict__ meas, const int *__restrict__ kind,
+                            cx<T> *__restrict__ y, long long total, int P) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  long long k = i / P;
+  y[i] = kind[k] == 1 ? meas[i] : f[i];
+}
+
+template <typename T>
+__global__ void k_copy_cx(cx<T> *__restrict__ d, const cx<T> *__restrict__ s, long long total) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < total) d[i] = s[i];
+}
+
+template <typename T>
+__global__ void k_zero_cx(cx<T> *__restrict__ d, long long total) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < total) d[i] = mk((T)0, (T)0);
+}
+
+// ---------------------------------------------------------------------------
+// discrete TV (denoise.c:67-249) on u[y][x][z]
+// ---------------------------------------------------------------------------
+
+// grad over rows ylo..yhi (inclusive; clipped), FP64 partial sums of grad^2
+// and of u^2 over u < 0 (the pre-update dA, denoise.c:128-133)
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_tv_grad(const T *__restrict__ u, T *__restrict__ grad, int ny, int nx, int nz, int ylo, int yhi,
+          double *__restrict__ part_g, double *__restrict__ part_a) {
+  __shared__ double sg[kThreads], sa[kThreads];
+  const long long rowlen = (long long)nx * nz;
+  const long long total = (long long)(yhi - ylo + 1) * rowlen;
+  const T eps = (T)1e-8;
+  double accg = 0, acca = 0;
+  for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < total; i += (long long)gridDim.x * kThreads) {
+    int y = ylo + (int)(i / rowlen);
+    long long rem = i - (long long)(y - ylo) * rowlen;
+    int x = (int)(rem / nz), z = (int)(rem - (long long)x * nz);
+    const long long sy = rowlen, sx = nz;
+    const long long o = (long long)y * sy + (long long)x * sx + z;
+    const long long dzm = z > 0 ? -1 : 0, dzp = z < nz - 1 ? 1 : 0;
+    const long long dym = y > 0 ? -sy : 0, dyp = y < ny - 1 ? sy : 0;
+    const long long dxm = x > 0 ? -sx : 0, dxp = x < nx - 1 ? sx : 0;
+    T c = u[o];
+    // reference (i,j,k) = (z,y,x)
+    T t1 = c - u[o + dzm];
+    T t2 = c - u[o + dym];
+    T t3 = c - u[o + dxm];
+    T uzp = u[o + dzp];
+    T t4 = uzp - c;
+    T t5 = uzp - u[o + dzp + dym];
+    T t6 = uzp - u[o + dzp + dxm];
+    T uyp = u[o + dyp];
+    T t7 = uyp - u[o + dyp + dzm];
+    T t8 = uyp - c;
+    T t9 = uyp - u[o + dyp + dxm];
+    T uxp = u[o + dxp];
+    T t10 = uxp - u[o + dxp + dzm];
+    T t11 = uxp - u[o + dxp + dym];
+    T t12 = uxp - c;
+    T gv = (t1 + t2 + t3) / sqrt(eps + t1 * t1 + t2 * t2 + t3 * t3) - t4 / sqrt(eps + t4 * t4 + t5 * t5 + t6 * t6) -
+           t8 / sqrt(eps + t7 * t7 + t8 * t8 + t9 * t9) - t12 / sqrt(eps + t10 * t10 + t11 * t11 + t12 * t12);
+    grad[o] = gv;
+    accg += (double)gv * (double)gv;
+    if (c < (T)0) acca += (double)c * (double)c;
+  }
+  sg[threadIdx.x] = accg;
+  sa[threadIdx.x] = acca;
+  __syncthreads();
+  for (int h = kThreads / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < h) {
+      sg[threadIdx.x] += sg[threadIdx.x + h];
+      sa[threadIdx.x] += sa[threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part_g[blockIdx.x] = sg[0];
+    part_a[blockIdx.x] = sa[0];
+  }
+}
+
+// u -= lam * sqrt(dA) * (grad / sqrt(sum grad^2))  (denoise.c:125-142)
+template <typename T>
+__global__ void k_tv_update(T *__restrict__ u, const T *__restrict__ grad, int nx, int nz, int ylo, int yhi,
+                            const double *__restrict__ sums, double lam) {
+  const long long rowlen = (long long)nx * nz;
+  const long long total = (long long)(yhi - ylo + 1) * rowlen;
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  long long o = (long long)ylo * rowlen + i;
+  T s1 = (T)sqrt(sums[0]);
+  T c = (T)(lam * sqrt(sums[1]));
+  T gn = grad[o] / s1;
+  u[o] = u[o] - c * gn;
+}
+
+__global__ void k_sum_scalars(const double *__restrict__ part_g, const double *__restrict__ part_a, int nb,
+                              double *__restrict__ sums);
+
+// ---------------------------------------------------------------------------
+// 2D anisotropic diffusion (denoise.c:320-378) as a hyperplane sweep.
+// Update (s, y, x) of sweep s reads the sweep-s values at (y-1,x), (y,x-1)
+// and the sweep-(s-1) values at (y+1,x), (y,x+1); all points with
+// 2s + y + x == tau are independent, so launch tau = 0, 1, ... in order.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_diffusion_tau(T *__restrict__ u, int ny, int nx, int nz, int niter, int tau, double lam,
+                                double ksq) {
+  // threads: (point on the hyperplane) x z ; z fastest
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  int z = (int)(i % nz);
+  long long j = i / nz;
+  // enumerate s in [smin, smax], y in [ymin, ymax] with x = tau - 2s - y
+  // j indexes (s, y) pairs with s = smin + j / ny, y = j % ny (invalid x skipped)
+  int smin = tau - (ny - 1) - (nx - 1);
+  smin = smin > 0 ? (smin + 1) / 2 : 0;
+  int s = smin + (int)(j / ny);
+  int y = (int)(j % ny);
+  if (s >= niter || 2 * s > tau) return;
+  int x = tau - 2 * s - y;
+  if (x < 0 || x >= nx) return;
+  const long long sy = (long long)nx * nz, sx = nz;
+  const long long o = (long long)y * sy + (long long)x * sx + z;
+  const long long dym = y > 0 ? -sy : 0, dyp = y < ny - 1 ? sy : 0;
+  const long long dxm = x > 0 ? -sx : 0, dxp = x < nx - 1 ? sx : 0;
+  T c = u[o];
+  T gN = u[o + dym] - c;
+  T gS = u[o + dyp] - c;
+  T gE = u[o + dxp] - c;
+  T gW = u[o + dxm] - c;
+  T k2 = (T)ksq;
+  T cN = (T)1 / ((T)1 + gN * gN / k2);
+  T cS = (T)1 / ((T)1 + gS * gS / k2);
+  T cE = (T)1 / ((T)1 + gE * gE / k2);
+  T cW = (T)1 / ((T)1 + gW * gW / k2);
+  u[o] = c + (T)lam * (cN * gN + cS * gS + cE * gE + cW * gW);
+}
+
+// ---------------------------------------------------------------------------
+// density constraint (reconstruction.c:672-700) on u[y][x][z]
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_constraint(T *__restrict__ u, const T *__restrict__ last, int np, int nz, int nx, int is_hio,
+                             int config_height, int nheight, int ncenter) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long total = (long long)np * np * nz;
+  if (i >= total) return;
+  long long yx = i / nz;
+  int yi = (int)(yx / np), xi = (int)(yx % np);
+  bool inside;
+  if (config_height)
+    inside = (yi >= (2 * ncenter - nheight) / 2 && yi <= (2 * ncenter + nheight) / 2);
+  else
+    inside = (yi >= (np - nx) / 2) && (yi < (np + nx) / 2) && (xi >= (np - nx) / 2) && (xi < (np + nx) / 2);
+  T v = u[i];
+  if (!inside || v < (T)0) u[i] = is_hio ? last[i] - (T)0.9 * v : (T)0;
+}
+
+// layout conversions between the reference volume [z][y][x] (double) and
+// the device volume [y][x][z] (T)
+template <typename T>
+__global__ void k_vol_from_ref(const double *__restrict__ ref, T *__restrict__ u, int ny, int nx, int nz) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long total = (long long)ny * nx * nz;
+  if (i >= total) return;
+  long long yx = i / nz;
+  int z = (int)(i % nz);
+  u[i] = (T)ref[(long long)z * ny * nx + yx];
+}
+
+template <typename T>
+__global__ void k_vol_to_ref(const T *__restrict__ u, double *__restrict__ ref, int ny, int nx, int nz) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long total = (long long)ny * nx * nz;
+  if (i >= total) return;
+  long long z = i / ((long long)ny * nx);
+  long long yx = i % ((long long)ny * nx);
+  ref[i] = (double)u[yx * nz + z];
+}
+
+// planes-major complex (double) [P][K] <-> plane-innermost (T) [K][P]
+template <typename T>
+__global__ void k_cx_from_planes(const double *__restrict__ src, cx<T> *__restrict__ dst, long long K, int P) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= K * P) return;
+  long long k = i / P;
+  int p = (int)(i % P);
+  const double *s = src + 2 * ((long long)p * K + k);
+  dst[i] = mk((T)s[0], (T)s[1]);
+}
+
+template <typename T>
+__global__ void k_cx_to_planes(const cx<T> *__restrict__ src, double *__restrict__ dst, long long K, int P) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= K * P) return;
+  long long p = i / K, k = i % K;
+  cx<T> v = src[k * P + p];
+  dst[2 * i] = (double)v.x;
+  dst[2 * i + 1] = (double)v.y;
+}
+
+}  // namespace nudim

@@ -1,0 +1,50 @@
+"""The C-ABI library loads and exports every entry point include/nudim.h declares."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    src = open(os.path.join(ROOT, "include", "nudim.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(nudim_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_header_declares_entry_points():
+    syms = declared_symbols()
+    for s in ["nudim_create", "nudim_iterate", "nudim_nfft_trafo", "nudim_nfft_adjoint", "nudim_cgnr",
+              "nudim_tv", "nudim_diffusion2d", "nudim_constraint", "nudim_zfft"]:
+        assert s in syms
+
+
+def test_library_exports_every_declared_symbol(hip):
+    L = ctypes.CDLL(hip.LIB_PATH)
+    missing = [s for s in declared_symbols() if not hasattr(L, s)]
+    assert not missing, missing
+
+
+def test_binding_covers_header(hip):
+    assert sorted(hip.EXPORTS) == declared_symbols()
+
+
+def test_errors_are_reported_not_raised_in_c(hip):
+    # invalid parameters fail with the reference's message, no device work needed
+    p = hip.make_params(64, 64, 64, -60, 60, 3, nhio=-1, ner=20)
+    with pytest.raises(hip.NudimError, match="valid nhio"):
+        hip.Reconstruction(p)
+    p = hip.make_params(64, 64, 64, -60, 60, 3, padangle=2.0)
+    with pytest.raises(hip.NudimError, match="padangle"):
+        hip.Reconstruction(p)
+    p = hip.make_params(64, 64, 64, -60, 60, 3, height=10)
+    with pytest.raises(hip.NudimError, match="height and center"):
+        hip.Reconstruction(p)
+
+
+def test_estimate_bytes_scales(hip):
+    small = hip.estimate_bytes(hip.make_params(128, 128, 128, -60, 60, 3))
+    big = hip.estimate_bytes(hip.make_params(384, 256, 384, -70, 70, 2))
+    assert big > 10 * small > 0
