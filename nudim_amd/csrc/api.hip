@@ -210,11 +210,11 @@ struct Job : JobBase {
   Cgnr<T> cg;
   DevBuf dw, dkind, dorder;
   DevBuf meas, y, Fh, F0;
-  DevBuf img_a, img_b, grad;
+  DevBuf img_a, img_b, img_c, grad, grad2;
   DevBuf d1, d2, tvpart_g, tvpart_a, tvsums;
   FftDesc fdz{};
   DevBuf twz;
-  T *img = nullptr, *img_last = nullptr;
+  T *img = nullptr, *img_last = nullptr, *spare = nullptr;
   static constexpr int kTvBlocks = 1024;
 
   Job(const nudim_params &p, const Geometry &g, hipStream_t s) : prm(p), geo(g), st(s) {
@@ -242,11 +242,18 @@ struct Job : JobBase {
     F0.alloc(NN * sizeof(cx<T>));
     img_a.alloc(V * sizeof(T));
     img_b.alloc(V * sizeof(T));
-    if (p.denoise && p.method == NUDIM_TV) grad.alloc(V * sizeof(T));
+    if (p.denoise && p.method == NUDIM_TV) {
+      img_c.alloc(V * sizeof(T));
+      grad.alloc(V * sizeof(T));
+      grad2.alloc(V * sizeof(T));
+      // zero outside the TV band for the life of the job
+      NUDIM_HIP(hipMemset(grad.p, 0, V * sizeof(T)));
+      NUDIM_HIP(hipMemset(grad2.p, 0, V * sizeof(T)));
+    }
     d1.alloc(P * 8);
     d2.alloc(P * 8);
-    tvpart_g.alloc(kTvBlocks * 8);
-    tvpart_a.alloc(kTvBlocks * 8);
+    tvpart_g.alloc((size_t)N * 64 * 8);
+    tvpart_a.alloc((size_t)N * 64 * 8);
     tvsums.alloc(2 * 8);
     fdz = make_fft_desc(P);
     std::vector<double> t;
@@ -257,12 +264,14 @@ struct Job : JobBase {
     NUDIM_HIP(hipMemcpy(twz.p, h.data(), P * sizeof(cx<T>), hipMemcpyHostToDevice));
     img = img_a.as<T>();
     img_last = img_b.as<T>();
+    spare = img_c.as<T>();
     NUDIM_HIP(hipMemsetAsync(img, 0, V * sizeof(T), st));
   }
 
   size_t bytes() const override {
     return op.bytes() + toep.bytes() + cg.bytes() + dw.bytes + dkind.bytes + dorder.bytes + meas.bytes + y.bytes +
-           Fh.bytes + F0.bytes + img_a.bytes + img_b.bytes + grad.bytes + d1.bytes + d2.bytes + tvpart_g.bytes +
+           Fh.bytes + F0.bytes + img_a.bytes + img_b.bytes + img_c.bytes + grad.bytes + grad2.bytes + d1.bytes +
+           d2.bytes + tvpart_g.bytes +
            tvpart_a.bytes + tvsums.bytes + twz.bytes;
   }
 
@@ -294,7 +303,7 @@ struct Job : JobBase {
   void zfft_inverse(const cx<T> *g, T *out) {
     int Gs = line_chunk_log2(P, 1 << 30, (int)sizeof(cx<T>), FftEpt<T>::value);
     long long nlines = (long long)N * N;
-    size_t lds = (size_t)P * (1 << Gs) * sizeof(cx<T>);
+    size_t lds = (size_t)P * ((1 << Gs) + 1) * sizeof(cx<T>);
     ProfScope ps(KC_ZFFT, st, 2.0 * V * sizeof(cx<T>));
     hipLaunchKernelGGL(k_zfft_inv<T>, dim3(cdiv(nlines, 1 << Gs)), dim3(kFftThreads), lds, st, g, out, fdz,
                        twz.as<cx<T>>(), Gs, nlines);
@@ -304,7 +313,7 @@ struct Job : JobBase {
   void zfft_forward(const T *in, cx<T> *fh, cx<T> *f0) {
     int Gs = line_chunk_log2(P, 1 << 30, (int)sizeof(cx<T>), FftEpt<T>::value);
     long long nlines = (long long)N * N;
-    size_t lds = (size_t)P * (1 << Gs) * sizeof(cx<T>);
+    size_t lds = (size_t)P * ((1 << Gs) + 1) * sizeof(cx<T>);
     ProfScope ps(KC_ZFFT, st, 2.0 * V * sizeof(cx<T>));
     hipLaunchKernelGGL(k_zfft_fwd<T>, dim3(cdiv(nlines, 1 << Gs)), dim3(kFftThreads), lds, st, in, fh, f0, fdz,
                        twz.as<cx<T>>(), Gs, nlines);
@@ -325,24 +334,10 @@ struct Job : JobBase {
       yhi = std::min(yhi, (2 * prm.ncenter + prm.nheight) / 2);
     }
     if (yhi < ylo) return;
-    long long total = (long long)(yhi - ylo + 1) * N * P;
-    double lam = (double)0.2f;
-    const double vb = (double)total * sizeof(T);  // algorithmic: read u once + write u once per sweep
-    for (int s = 0; s < 40; s++) {
-      {
-        ProfScope ps(KC_TV, st, vb);
-        hipLaunchKernelGGL(k_tv_grad<T>, dim3(kTvBlocks), dim3(kThreads), 0, st, u, grad.as<T>(), N, N, P, ylo, yhi,
-                           tvpart_g.as<double>(), tvpart_a.as<double>());
-      }
-      hipLaunchKernelGGL(k_sum_scalars, dim3(1), dim3(1), 0, st, tvpart_g.as<double>(), tvpart_a.as<double>(),
-                         kTvBlocks, tvsums.as<double>());
-      {
-        ProfScope ps(KC_TV, st, vb);
-        hipLaunchKernelGGL(k_tv_update<T>, dim3(cdiv(total, 256)), dim3(256), 0, st, u, grad.as<T>(), N, P, ylo,
-                           yhi, tvsums.as<double>(), lam);
-      }
-    }
-    NUDIM_HIP(hipGetLastError());
+    // total_variation_discrete(_height)(img, 0.2f, 40, ...), reconstruction.c:630-632
+    T *r = tv_sweeps<T>(u, spare, grad.as<T>(), grad2.as<T>(), N, N, P, ylo, yhi, 0.2f, 40, tvpart_g.as<double>(),
+                        tvpart_a.as<double>(), tvsums.as<double>(), N * 64, st);
+    if (r != img) std::swap(img, spare);
   }
 
   void diffusion(T *u) {
@@ -785,20 +780,17 @@ struct VolTmp {
     yhi = std::min(yhi, ny - 1);
     if (yhi < ylo) return;
     grad.alloc(V * sizeof(T));
-    const int nb = 1024;
+    DevBuf u2(V * sizeof(T)), g2(V * sizeof(T));
+    NUDIM_HIP(hipMemsetAsync(grad.p, 0, V * sizeof(T), ts.s));
+    NUDIM_HIP(hipMemsetAsync(g2.p, 0, V * sizeof(T), ts.s));
+    const int nb = ny * 64;
     part_g.alloc(nb * 8);
     part_a.alloc(nb * 8);
     sums.alloc(16);
-    long long total = (long long)(yhi - ylo + 1) * nx * nz;
-    for (int s = 0; s < niter; s++) {
-      hipLaunchKernelGGL(k_tv_grad<T>, dim3(nb), dim3(kThreads), 0, ts.s, u.as<T>(), grad.as<T>(), ny, nx, nz, ylo, yhi,
-                         part_g.as<double>(), part_a.as<double>());
-      hipLaunchKernelGGL(k_sum_scalars, dim3(1), dim3(1), 0, ts.s, part_g.as<double>(), part_a.as<double>(), nb,
-                         sums.as<double>());
-      hipLaunchKernelGGL(k_tv_update<T>, dim3(cdiv(total, 256)), dim3(256), 0, ts.s, u.as<T>(), grad.as<T>(), nx, nz,
-                         ylo, yhi, sums.as<double>(), (double)lam);
-    }
-    NUDIM_HIP(hipGetLastError());
+    T *r = tv_sweeps<T>(u.as<T>(), u2.as<T>(), grad.as<T>(), g2.as<T>(), ny, nx, nz, ylo, yhi, lam, niter,
+                        part_g.as<double>(), part_a.as<double>(), sums.as<double>(), nb, ts.s);
+    if (r != u.as<T>()) std::swap(u, u2);
+    NUDIM_HIP(hipStreamSynchronize(ts.s));
   }
 };
 

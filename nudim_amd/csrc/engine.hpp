@@ -9,6 +9,13 @@
 
 namespace nudim {
 
+// fixed-length register FFT (fft2.hip); false when n has no specialisation
+template <typename T>
+bool fft2_launch(int n, const cx<T> *in, cx<T> *out, int nlines, const LineIO &li, const LineIO &lo, int sign, int P,
+                 hipStream_t st, const cx<T> *tw, const cx<T> *mid, long long mid_so);
+// process-wide switch for the fixed-length kernels (on by default)
+extern bool g_use_fft2;
+
 // choose log2 of the plane chunk G of a line transform of length n over P planes
 int line_chunk_log2(int n, int P, int elem_bytes, int ept);
 
@@ -48,6 +55,14 @@ struct ToeplitzOp {
   void init(int N, const double *x, const double *w, int M, int P, hipStream_t st);
   void apply(const cx<T> *p, cx<T> *q, hipStream_t st);
 };
+
+// niter sweeps of discrete TV over rows ylo..yhi (clipped by the caller) on
+// a [ny][nx][nz] volume held in u0; u1, g0, g1 are volume-sized scratch, and
+// g0, g1 must hold zeros outside rows ylo..yhi (clear them once when they are
+// allocated).  Returns the buffer (u0 or u1) that holds the result.
+template <typename T>
+T *tv_sweeps(T *u0, T *u1, T *g0, T *g1, int ny, int nx, int nz, int ylo, int yhi, float lam, int niter,
+             double *part_g, double *part_a, double *sums, int nblocks, hipStream_t st);
 
 // per-plane reduction helpers
 struct Reducer {

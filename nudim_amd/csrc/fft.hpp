@@ -83,11 +83,25 @@ __device__ __forceinline__ void dft_small(cx<T> *v, int sign) {
   }
 }
 
-// one radix-R stage over G lines (Gs = log2 G) of length n in LDS
+// k = bf / m, q = bf % m for bf < 2^20 by a float reciprocal and one fix-up
+__device__ __forceinline__ void divmod_small(int bf, int m, float inv_m, int &k, int &q) {
+  k = (int)((float)bf * inv_m);
+  q = bf - k * m;
+  if (q < 0) {
+    k--;
+    q += m;
+  } else if (q >= m) {
+    k++;
+    q -= m;
+  }
+}
+
+// one radix-R stage over G lines (Gs = log2 G) of length n in LDS; the
+// twiddle table tw (w_n^j, j < n) is in LDS too
 template <typename T, int R, int EPT>
-__device__ __forceinline__ void fft_stage(cx<T> *lds, int n, int Gs, int l, const cx<T> *__restrict__ tw,
-                                          int sign) {
+__device__ __forceinline__ void fft_stage(cx<T> *lds, int n, int Gs, int l, const cx<T> *tw, int sign) {
   const int m = n / (l * R);
+  const float inv_m = 1.0f / (float)m;
   const int nb = (n / R) << Gs;
   constexpr int BPT = (EPT + R - 1) / R;
   const int G1 = (1 << Gs) - 1;
@@ -98,12 +112,16 @@ __device__ __forceinline__ void fft_stage(cx<T> *lds, int n, int Gs, int l, cons
     int b = tid + j * kFftThreads;
     if (b < nb) {
       int g = b & G1, bf = b >> Gs;
-      int k = bf / m, q = bf - k * m;
+      int k, q;
+      divmod_small(bf, m, inv_m, k, q);
+      const int base = ((k * R) * m + q) << Gs;
+      const int step = m << Gs;
+      const int tstep = k * m;
 #pragma unroll
       for (int t = 0; t < R; t++) {
-        cx<T> a = lds[(((k * R + t) * m + q) << Gs) + g];
-        if (t > 0 && k > 0) {
-          cx<T> w = tw[t * k * m];
+        cx<T> a = lds[base + t * step + g];
+        if (t > 0) {
+          cx<T> w = tw[t * tstep];
           if (sign > 0) w = conjx(w);
           a = a * w;
         }
@@ -118,16 +136,19 @@ __device__ __forceinline__ void fft_stage(cx<T> *lds, int n, int Gs, int l, cons
     int b = tid + j * kFftThreads;
     if (b < nb) {
       int g = b & G1, bf = b >> Gs;
-      int k = bf / m, q = bf - k * m;
+      int k, q;
+      divmod_small(bf, m, inv_m, k, q);
+      const int base = (k * m + q) << Gs;
+      const int step = (l * m) << Gs;
 #pragma unroll
-      for (int u = 0; u < R; u++) lds[(((k + l * u) * m + q) << Gs) + g] = v[j][u];
+      for (int u = 0; u < R; u++) lds[base + u * step + g] = v[j][u];
     }
   }
   __syncthreads();
 }
 
 template <typename T, int EPT>
-__device__ void fft_lds(cx<T> *lds, const FftDesc &fd, int Gs, const cx<T> *__restrict__ tw, int sign) {
+__device__ void fft_lds(cx<T> *lds, const FftDesc &fd, int Gs, const cx<T> *tw, int sign) {
   int l = 1;
   for (int s = 0; s < fd.nf; s++) {
     int r = fd.f[s];
@@ -140,6 +161,18 @@ __device__ void fft_lds(cx<T> *lds, const FftDesc &fd, int Gs, const cx<T> *__re
     }
     l *= r;
   }
+}
+
+// copies the n-entry twiddle table into LDS (caller synchronises)
+template <typename T>
+__device__ __forceinline__ void load_twiddles(cx<T> *stw, const cx<T> *__restrict__ tw, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) stw[i] = tw[i];
+}
+
+__device__ __forceinline__ int wrap_pos(int pos, int n) {
+  pos += pos < 0 ? n : 0;
+  pos -= pos >= n ? n : 0;
+  return pos;
 }
 
 // Elements per thread of the register stash: n*G <= 256*EPT.
@@ -156,10 +189,12 @@ struct LineIO {
   const double *scale_o;
 };
 
-// Batched line FFT over a plane-innermost array.  Block b handles line
-// o = b / nchunk and planes [chunk*G, chunk*G+G).  With `mid` the block
-// multiplies the transform by mid[o*mid_so + pos] and transforms back with
-// the opposite sign (fused Toeplitz column pass).
+// Batched line FFT over a plane-innermost array.  Block b (renumbered so
+// that neighbouring plane chunks run on the same XCD and share L2 lines)
+// handles line o = b / nchunk and planes [chunk*G, chunk*G+G).  With `mid`
+// the block multiplies the transform by mid[o*mid_so + pos] and transforms
+// back with the opposite sign (fused Toeplitz column pass).  LDS holds the
+// n x G lines followed by the n twiddles.
 template <typename T>
 __global__ void __launch_bounds__(kFftThreads)
 k_fft_lines(const cx<T> *in, cx<T> *out, FftDesc fd, LineIO li, LineIO lo,
@@ -169,44 +204,43 @@ k_fft_lines(const cx<T> *in, cx<T> *out, FftDesc fd, LineIO li, LineIO lo,
   cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
   constexpr int EPT = FftEpt<T>::value;
   const int n = fd.n, G = 1 << Gs, G1 = G - 1;
-  const int o = blockIdx.x / nchunk, chunk = blockIdx.x - o * nchunk;
+  cx<T> *stw = lds + (n << Gs);
+  const int lb = (int)xcd_block(blockIdx.x, gridDim.x);
+  const int o = lb / nchunk, chunk = lb - o * nchunk;
   const int tid = threadIdx.x;
+  load_twiddles(stw, tw, n);
   // zero the positions no input element lands on
   const int nz0 = (n - li.cnt) << Gs;
   for (int i = tid; i < nz0; i += kFftThreads) {
     int g = i & G1, k = li.cnt + (i >> Gs);
-    int pos = k + li.shift;
-    pos = ((pos % n) + n) % n;
-    lds[(pos << Gs) + g] = mk((T)0, (T)0);
+    lds[(wrap_pos(k + li.shift, n) << Gs) + g] = mk((T)0, (T)0);
   }
   const cx<T> *src = in + (long long)o * li.so + (long long)chunk * G;
   const T so_scale = li.scale_o ? (T)li.scale_o[o] : (T)1;
   const int nin = li.cnt << Gs;
+#pragma unroll 8
   for (int i = tid; i < nin; i += kFftThreads) {
     int g = i & G1, k = i >> Gs;
     cx<T> v = src[(long long)k * li.sk + g];
     if (li.scale_k) v = ((T)li.scale_k[k] * so_scale) * v;
     else if (li.scale_o) v = so_scale * v;
-    int pos = k + li.shift;
-    pos = ((pos % n) + n) % n;
-    lds[(pos << Gs) + g] = v;
+    lds[(wrap_pos(k + li.shift, n) << Gs) + g] = v;
   }
   __syncthreads();
-  fft_lds<T, EPT>(lds, fd, Gs, tw, sign);
+  fft_lds<T, EPT>(lds, fd, Gs, stw, sign);
   if (mid) {
     const cx<T> *mk_ = mid + (long long)o * mid_so;
     for (int i = tid; i < (n << Gs); i += kFftThreads) lds[i] = lds[i] * mk_[i >> Gs];
     __syncthreads();
-    fft_lds<T, EPT>(lds, fd, Gs, tw, -sign);
+    fft_lds<T, EPT>(lds, fd, Gs, stw, -sign);
   }
   cx<T> *dst = out + (long long)o * lo.so + (long long)chunk * G;
   const T oo_scale = lo.scale_o ? (T)lo.scale_o[o] : (T)1;
   const int nout = lo.cnt << Gs;
+#pragma unroll 8
   for (int i = tid; i < nout; i += kFftThreads) {
     int g = i & G1, k = i >> Gs;
-    int pos = k + lo.shift;
-    pos = ((pos % n) + n) % n;
-    cx<T> v = lds[(pos << Gs) + g];
+    cx<T> v = lds[(wrap_pos(k + lo.shift, n) << Gs) + g];
     if (lo.scale_k) v = ((T)lo.scale_k[k] * oo_scale) * v;
     else if (lo.scale_o) v = oo_scale * v;
     dst[(long long)k * lo.sk + g] = v;
@@ -229,8 +263,11 @@ k_zfft_fwd(const T *__restrict__ img, cx<T> *__restrict__ fh, cx<T> *__restrict_
   cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
   constexpr int EPT = FftEpt<T>::value;
   const int nz = fd.n, G = 1 << Gs, h = nz / 2;
+  cx<T> *stw = lds + (nz << Gs);
   const long long line0 = (long long)blockIdx.x * G;
   const int tid = threadIdx.x;
+  load_twiddles(stw, tw, nz);
+#pragma unroll 8
   for (int i = tid; i < nz * G; i += kFftThreads) {
     int g = i / nz, z = i - g * nz;
     long long line = line0 + g;
@@ -244,7 +281,8 @@ k_zfft_fwd(const T *__restrict__ img, cx<T> *__restrict__ fh, cx<T> *__restrict_
     lds[(z << Gs) + g] = mk(v, (T)0);
   }
   __syncthreads();
-  fft_lds<T, EPT>(lds, fd, Gs, tw, -1);
+  fft_lds<T, EPT>(lds, fd, Gs, stw, -1);
+#pragma unroll 8
   for (int i = tid; i < nz * G; i += kFftThreads) {
     int g = i / nz, p = i - g * nz;
     long long line = line0 + g;
@@ -264,8 +302,11 @@ k_zfft_inv(const cx<T> *__restrict__ gin, T *__restrict__ img, FftDesc fd, const
   cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
   constexpr int EPT = FftEpt<T>::value;
   const int nz = fd.n, G = 1 << Gs, h = nz / 2;
+  cx<T> *stw = lds + (nz << Gs);
   const long long line0 = (long long)blockIdx.x * G;
   const int tid = threadIdx.x;
+  load_twiddles(stw, tw, nz);
+#pragma unroll 8
   for (int i = tid; i < nz * G; i += kFftThreads) {
     int g = i / nz, k = i - g * nz;
     long long line = line0 + g;
@@ -278,7 +319,8 @@ k_zfft_inv(const cx<T> *__restrict__ gin, T *__restrict__ img, FftDesc fd, const
     lds[(k << Gs) + g] = v;
   }
   __syncthreads();
-  fft_lds<T, EPT>(lds, fd, Gs, tw, +1);
+  fft_lds<T, EPT>(lds, fd, Gs, stw, +1);
+#pragma unroll 8
   for (int i = tid; i < nz * G; i += kFftThreads) {
     int g = i / nz, z = i - g * nz;
     long long line = line0 + g;
@@ -290,7 +332,7 @@ k_zfft_inv(const cx<T> *__restrict__ gin, T *__restrict__ img, FftDesc fd, const
   }
 }
 
-// plain batched DFT of contiguous lines (parity entry point nudim_zfft)
+// plain batched DFT of contiguous lines
 template <typename T>
 __global__ void __launch_bounds__(kFftThreads)
 k_fft_contig(cx<T> *__restrict__ data, FftDesc fd, const cx<T> *__restrict__ tw, int Gs, long long nlines,
@@ -299,15 +341,17 @@ k_fft_contig(cx<T> *__restrict__ data, FftDesc fd, const cx<T> *__restrict__ tw,
   cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
   constexpr int EPT = FftEpt<T>::value;
   const int n = fd.n, G = 1 << Gs;
+  cx<T> *stw = lds + (n << Gs);
   const long long line0 = (long long)blockIdx.x * G;
   const int tid = threadIdx.x;
+  load_twiddles(stw, tw, n);
   for (int i = tid; i < n * G; i += kFftThreads) {
     int g = i / n, k = i - g * n;
     long long line = line0 + g;
     lds[(k << Gs) + g] = line < nlines ? data[line * n + k] : mk((T)0, (T)0);
   }
   __syncthreads();
-  fft_lds<T, EPT>(lds, fd, Gs, tw, sign);
+  fft_lds<T, EPT>(lds, fd, Gs, stw, sign);
   for (int i = tid; i < n * G; i += kFftThreads) {
     int g = i / n, k = i - g * n;
     long long line = line0 + g;

@@ -1,0 +1,255 @@
+// fft2.hpp -- register-resident four-step line FFTs for fixed lengths.
+//
+// n = N1 * N2.  With input positions pos = j2 + N2 j1 and output positions
+// k1 + N1 k2:
+//   X[k1 + N1 k2] = sum_j2 w_N2^{j2 k2} [ w_n^{j2 k1} sum_j1 x[j2 + N2 j1] w_N1^{j1 k1} ]
+// Step A: thread (g, j2) loads its N1 inputs straight from HBM, does an
+// N1-point DFT in registers and multiplies by w_n^{j2 k1}; one LDS exchange;
+// step B: thread (g, k1) does an N2-point DFT in registers and stores.
+// The inverse of a transform held as (thread k1, register k2) starts with
+// the N2-point DFT over k2 that the thread already holds, so a forward
+// transform, a pointwise product and the inverse (the Toeplitz column pass)
+// need two LDS exchanges in all.  All index arithmetic inside a line is
+// resolved at compile time.
+#pragma once
+
+#include "fft.hpp"
+
+namespace nudim {
+
+// ---- compile-time twiddles ---------------------------------------------------
+
+constexpr double kPi = 3.14159265358979323846264338327950288;
+
+constexpr double taylor_sin(double x) {
+  double x2 = x * x, t = x, s = x;
+  for (int i = 1; i < 14; i++) {
+    t *= -x2 / ((2.0 * i) * (2.0 * i + 1.0));
+    s += t;
+  }
+  return s;
+}
+constexpr double taylor_cos(double x) {
+  double x2 = x * x, t = 1.0, s = 1.0;
+  for (int i = 1; i < 14; i++) {
+    t *= -x2 / ((2.0 * i - 1.0) * (2.0 * i));
+    s += t;
+  }
+  return s;
+}
+
+struct CS {
+  double c, s;
+};
+
+// (cos, sin) of 2 pi e / R: octant reduction to |phi| <= pi/4 and an exact
+// quarter-turn rotation
+constexpr CS cis2pi(long e, long R) {
+  e %= R;
+  if (e < 0) e += R;
+  long o8 = (8 * e) / R;
+  double r = (double)(8 * e - o8 * R) / (double)R;  // in [0, 1)
+  double c = 0, s = 0;
+  long q = 0;
+  if (o8 % 2 == 0) {
+    double ph = r * (kPi / 4);
+    c = taylor_cos(ph);
+    s = taylor_sin(ph);
+    q = o8 / 2;
+  } else {
+    double ph = (1.0 - r) * (kPi / 4);  // angle = (o8+1) pi/4 - ph
+    c = taylor_cos(ph);
+    s = -taylor_sin(ph);
+    q = (o8 + 1) / 2;
+  }
+  switch (q % 4) {
+    case 0: return CS{c, s};
+    case 1: return CS{-s, c};
+    case 2: return CS{-c, -s};
+    default: return CS{s, -c};
+  }
+}
+
+// w_R^e = exp(S 2 pi i e / R)
+template <typename T, int S>
+__device__ __forceinline__ cx<T> wconst(long e, long R) {
+  CS v = cis2pi(e, R);
+  return mk((T)v.c, (T)(S * v.s));
+}
+
+constexpr int split_factor(int R) {
+  return (R % 8 == 0 && R != 8) ? 8 : (R % 4 == 0 && R != 4) ? 4 : (R % 2 == 0 && R != 2) ? 2
+       : (R % 3 == 0 && R != 3) ? 3 : (R % 5 == 0 && R != 5) ? 5 : R;
+}
+
+// in-register DFT of R points, natural order in and out, sign S
+template <typename T, int R, int S>
+__device__ __forceinline__ void dft_reg(cx<T> *v) {
+  if constexpr (R == 1) {
+    return;
+  } else if constexpr (R == 2 || R == 3 || R == 4 || R == 5 || R == 8) {
+    dft_small<T, R>(v, S);
+  } else {
+    constexpr int A = split_factor(R);
+    constexpr int B = R / A;
+    static_assert(A > 1 && B > 1 && A * B == R, "length must factor into 2, 3, 4, 5, 8");
+    cx<T> t[R];
+#pragma unroll
+    for (int j2 = 0; j2 < B; j2++) {
+      cx<T> a[A];
+#pragma unroll
+      for (int j1 = 0; j1 < A; j1++) a[j1] = v[j2 + B * j1];
+      dft_reg<T, A, S>(a);
+#pragma unroll
+      for (int k1 = 0; k1 < A; k1++) t[j2 + B * k1] = (j2 * k1 == 0) ? a[k1] : a[k1] * wconst<T, S>(j2 * k1, R);
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < A; k1++) {
+      cx<T> b[B];
+#pragma unroll
+      for (int j2 = 0; j2 < B; j2++) b[j2] = t[j2 + B * k1];
+      dft_reg<T, B, S>(b);
+#pragma unroll
+      for (int k2 = 0; k2 < B; k2++) v[k1 + A * k2] = b[k2];
+    }
+  }
+}
+
+// ---- the line kernel ------------------------------------------------------------
+
+// MODE 0: one transform with sign S; MODE 1: forward (S = -1), times
+// mid[o*mid_so + pos], inverse.  Threads: N1 * G (>= N2 * G).  LDS: the
+// exchange buffer [N2][N1*G + pad] followed by the n twiddles w_n^j.
+template <int N1, int N2>
+struct Fft2Shape {
+  static constexpr int n = N1 * N2;
+  static constexpr int big = N1 > N2 ? N1 : N2;
+};
+
+template <typename T, int N1, int N2, int S, int MODE>
+__global__ void __launch_bounds__(256)
+k_fft2(const cx<T> *in, cx<T> *out, LineIO li, LineIO lo, const cx<T> *__restrict__ tw, int Gs, int nchunk,
+       const cx<T> *__restrict__ mid, long long mid_so) {
+  constexpr int n = N1 * N2;
+  extern __shared__ unsigned char smem_raw[];
+  cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
+  const int G = 1 << Gs, G1 = G - 1;
+  // exchange row of one j2 (padded by G so that the step-A writes of a
+  // wavefront spread evenly over the LDS banks)
+  const int rowA = N1 * G + G;
+  cx<T> *stw = lds + N2 * rowA;         // n twiddles
+  const int lb = (int)xcd_block(blockIdx.x, gridDim.x);
+  const int o = lb / nchunk, chunk = lb - o * nchunk;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < n; i += blockDim.x) stw[i] = tw[i];
+  const int g = tid & G1, r = tid >> Gs;  // r = j2 in step A, k1 in step B
+  const bool actA = r < N2, actB = r < N1;
+  const cx<T> *src = in + (long long)o * li.so + (long long)chunk * G + g;
+  const T so_scale = li.scale_o ? (T)li.scale_o[o] : (T)1;
+
+  // ---- step A: load positions j2 + N2 j1, N1-point DFT ----
+  // Loads are unconditional (index clamped into range, out-of-range lanes
+  // zeroed afterwards) so that all N1 of them are in flight together.
+  cx<T> a[N1];
+  if (actA) {
+    int kk[N1];
+#pragma unroll
+    for (int j1 = 0; j1 < N1; j1++) kk[j1] = wrap_pos(r + N2 * j1 - li.shift, n);
+#pragma unroll
+    for (int j1 = 0; j1 < N1; j1++) a[j1] = src[(long long)min(kk[j1], li.cnt - 1) * li.sk];
+    if (li.scale_k) {
+      T sc[N1];
+#pragma unroll
+      for (int j1 = 0; j1 < N1; j1++) sc[j1] = (T)li.scale_k[min(kk[j1], li.cnt - 1)];
+#pragma unroll
+      for (int j1 = 0; j1 < N1; j1++) a[j1] = (sc[j1] * so_scale) * a[j1];
+    } else if (li.scale_o) {
+#pragma unroll
+      for (int j1 = 0; j1 < N1; j1++) a[j1] = so_scale * a[j1];
+    }
+#pragma unroll
+    for (int j1 = 0; j1 < N1; j1++)
+      if (kk[j1] >= li.cnt) a[j1] = mk((T)0, (T)0);
+    dft_reg<T, N1, (MODE == 1 ? -1 : S)>(a);
+  }
+  __syncthreads();  // twiddle table complete
+  if (actA) {
+#pragma unroll
+    for (int k1 = 0; k1 < N1; k1++) {
+      cx<T> w = stw[r * k1];
+      if ((MODE == 1 ? -1 : S) > 0) w = conjx(w);
+      lds[r * rowA + k1 * G + g] = a[k1] * w;
+    }
+  }
+  __syncthreads();
+
+  // ---- step B: N2-point DFT over j2 for fixed k1 ----
+  cx<T> b[N2];
+  if (actB) {
+#pragma unroll
+    for (int j2 = 0; j2 < N2; j2++) b[j2] = lds[j2 * rowA + r * G + g];
+    dft_reg<T, N2, (MODE == 1 ? -1 : S)>(b);
+  }
+  cx<T> *dst = out + (long long)o * lo.so + (long long)chunk * G + g;
+  const T oo_scale = lo.scale_o ? (T)lo.scale_o[o] : (T)1;
+  if constexpr (MODE == 0) {
+    if (actB) {
+      int kk[N2];
+#pragma unroll
+      for (int k2 = 0; k2 < N2; k2++) kk[k2] = wrap_pos(r + N1 * k2 - lo.shift, n);
+      if (lo.scale_k) {
+        T sc[N2];
+#pragma unroll
+        for (int k2 = 0; k2 < N2; k2++) sc[k2] = (T)lo.scale_k[min(kk[k2], lo.cnt - 1)];
+#pragma unroll
+        for (int k2 = 0; k2 < N2; k2++) b[k2] = (sc[k2] * oo_scale) * b[k2];
+      } else if (lo.scale_o) {
+#pragma unroll
+        for (int k2 = 0; k2 < N2; k2++) b[k2] = oo_scale * b[k2];
+      }
+#pragma unroll
+      for (int k2 = 0; k2 < N2; k2++)
+        if (kk[k2] < lo.cnt) dst[(long long)kk[k2] * lo.sk] = b[k2];
+    }
+  } else {
+    // pointwise product and inverse: N2-point inverse DFT over k2 (in
+    // registers), twiddle w_n^{-k1 m2}, exchange, N1-point inverse DFT
+    __syncthreads();  // every thread has read the exchange buffer
+    if (actB) {
+      const cx<T> *mrow = mid + (long long)o * mid_so;
+      cx<T> mv[N2];
+#pragma unroll
+      for (int k2 = 0; k2 < N2; k2++) mv[k2] = mrow[r + N1 * k2];
+#pragma unroll
+      for (int k2 = 0; k2 < N2; k2++) b[k2] = b[k2] * mv[k2];
+      dft_reg<T, N2, +1>(b);
+#pragma unroll
+      for (int m2 = 0; m2 < N2; m2++) lds[m2 * rowA + r * G + g] = b[m2] * conjx(stw[r * m2]);
+    }
+    __syncthreads();
+    if (actA) {
+      // thread j2 := m2 now gathers k1 = 0..N1-1
+#pragma unroll
+      for (int k1 = 0; k1 < N1; k1++) a[k1] = lds[r * rowA + k1 * G + g];
+      dft_reg<T, N1, +1>(a);
+      int kk[N1];
+#pragma unroll
+      for (int m1 = 0; m1 < N1; m1++) kk[m1] = wrap_pos(r + N2 * m1 - lo.shift, n);
+      if (lo.scale_k) {
+        T sc[N1];
+#pragma unroll
+        for (int m1 = 0; m1 < N1; m1++) sc[m1] = (T)lo.scale_k[min(kk[m1], lo.cnt - 1)];
+#pragma unroll
+        for (int m1 = 0; m1 < N1; m1++) a[m1] = (sc[m1] * oo_scale) * a[m1];
+      } else if (lo.scale_o) {
+#pragma unroll
+        for (int m1 = 0; m1 < N1; m1++) a[m1] = oo_scale * a[m1];
+      }
+#pragma unroll
+      for (int m1 = 0; m1 < N1; m1++)
+        if (kk[m1] < lo.cnt) dst[(long long)kk[m1] * lo.sk] = a[m1];
+    }
+  }
+}
+
+}  // namespace nudim

@@ -54,6 +54,92 @@ k_interp(const cx<T> *__restrict__ g, cx<T> *__restrict__ out, const int *__rest
   out[(size_t)node * P + p] = acc;
 }
 
+// k_interp with the window width W fixed at compile time: the node's 2W
+// weights and W column offsets are kept in registers and the W grid reads of
+// a row are issued together.  Same summation order as k_interp.
+template <typename T, int W>
+__global__ void __launch_bounds__(kThreads)
+k_interp_w(const cx<T> *__restrict__ g, cx<T> *__restrict__ out, const int *__restrict__ c0,
+           const int *__restrict__ c1, const double *__restrict__ psi0, const double *__restrict__ psi1, int M, int n,
+           int P, int PCs) {
+  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  const int PC = 1 << PCs;
+  const int node = lb * (kThreads >> PCs) + (threadIdx.x >> PCs);
+  const int p = blockIdx.y * PC + (threadIdx.x & (PC - 1));
+  if (node >= M) return;
+  const int b0 = c0[node], b1 = c1[node];
+  T a0[W], a1[W];
+  long long col[W];
+#pragma unroll
+  for (int t = 0; t < W; t++) {
+    a0[t] = (T)psi0[(size_t)node * W + t];
+    a1[t] = (T)psi1[(size_t)node * W + t];
+    int c = b1 + t;
+    c += c < 0 ? n : 0;
+    c -= c >= n ? n : 0;
+    col[t] = (long long)c * P;
+  }
+  const size_t rs = (size_t)n * P;
+  cx<T> acc = mk((T)0, (T)0);
+#pragma unroll
+  for (int t0 = 0; t0 < W; t0++) {
+    int r = b0 + t0;
+    r += r < 0 ? n : 0;
+    r -= r >= n ? n : 0;
+    const cx<T> *row = g + r * rs + p;
+    cx<T> v[W];
+#pragma unroll
+    for (int t1 = 0; t1 < W; t1++) v[t1] = row[col[t1]];
+    cx<T> racc = mk((T)0, (T)0);
+#pragma unroll
+    for (int t1 = 0; t1 < W; t1++) racc = racc + a1[t1] * v[t1];
+    acc = acc + a0[t0] * racc;
+  }
+  out[(size_t)node * P + p] = acc;
+}
+
+// k_gather with entries decoded eight at a time so that the weight and
+// sample reads of eight entries are in flight together.  Same summation
+// order as k_gather.
+template <typename T, int W>
+__global__ void __launch_bounds__(kThreads)
+k_gather_w(const cx<T> *__restrict__ s, cx<T> *__restrict__ g, const int *__restrict__ cell_ptr,
+           const uint32_t *__restrict__ ent, const double *__restrict__ psi0, const double *__restrict__ psi1,
+           int ncell, int P, int PCs) {
+  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  const int PC = 1 << PCs;
+  const int cell = lb * (kThreads >> PCs) + (threadIdx.x >> PCs);
+  const int p = blockIdx.y * PC + (threadIdx.x & (PC - 1));
+  if (cell >= ncell) return;
+  const int e0 = cell_ptr[cell], e1 = cell_ptr[cell + 1];
+  cx<T> acc = mk((T)0, (T)0);
+  constexpr int B = 8;
+  int e = e0;
+  for (; e + B <= e1; e += B) {
+    uint32_t u[B];
+#pragma unroll
+    for (int j = 0; j < B; j++) u[j] = ent[e + j];
+    double w0[B], w1[B];
+    cx<T> sv[B];
+#pragma unroll
+    for (int j = 0; j < B; j++) {
+      int node = (int)(u[j] & 0xFFFFFu), t0 = (int)((u[j] >> 20) & 63u), t1 = (int)(u[j] >> 26);
+      w0[j] = psi0[(size_t)node * W + t0];
+      w1[j] = psi1[(size_t)node * W + t1];
+      sv[j] = s[(size_t)node * P + p];
+    }
+#pragma unroll
+    for (int j = 0; j < B; j++) acc = acc + (T)(w0[j] * w1[j]) * sv[j];
+  }
+  for (; e < e1; e++) {
+    uint32_t u = ent[e];
+    int node = (int)(u & 0xFFFFFu), t0 = (int)((u >> 20) & 63u), t1 = (int)(u >> 26);
+    T w = (T)(psi0[(size_t)node * W + t0] * psi1[(size_t)node * W + t1]);
+    acc = acc + w * s[(size_t)node * P + p];
+  }
+  g[(size_t)cell * P + p] = acc;
+}
+
 // g[cell][p] = sum over the cell's taps (node, t0, t1) of
 //              psi0[node][t0] psi1[node][t1] s[node][p]
 // in node order: a deterministic gather, no atomics.  Writes every cell.
@@ -216,6 +302,13 @@ __global__ void k_zero_cx(cx<T> *__restrict__ d, long long total) {
 // discrete TV (denoise.c:67-249) on u[y][x][z]
 // ---------------------------------------------------------------------------
 
+// t / sqrt(ss): exact in FP64 (the reference's arithmetic), t * rsqrt(ss) in FP32
+template <typename T>
+__device__ __forceinline__ T tv_rs(T t, T ss) {
+  if constexpr (sizeof(T) == 4) return t * rsqrtf(ss);
+  else return t / sqrt(ss);
+}
+
 // grad over rows ylo..yhi (inclusive; clipped), FP64 partial sums of grad^2
 // and of u^2 over u < 0 (the pre-update dA, denoise.c:128-133)
 template <typename T>
@@ -227,10 +320,11 @@ k_tv_grad(const T *__restrict__ u, T *__restrict__ grad, int ny, int nx, int nz,
   const long long total = (long long)(yhi - ylo + 1) * rowlen;
   const T eps = (T)1e-8;
   double accg = 0, acca = 0;
-  for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < total; i += (long long)gridDim.x * kThreads) {
-    int y = ylo + (int)(i / rowlen);
-    long long rem = i - (long long)(y - ylo) * rowlen;
-    int x = (int)(rem / nz), z = (int)(rem - (long long)x * nz);
+  // grid: x = chunks of one row (x, z), y = band row; 32-bit index math
+  const int y = ylo + (int)blockIdx.y;
+  const unsigned rl = (unsigned)rowlen, unz = (unsigned)nz;
+  for (unsigned rem = blockIdx.x * kThreads + threadIdx.x; rem < rl; rem += gridDim.x * kThreads) {
+    int x = (int)(rem / unz), z = (int)(rem - (unsigned)x * unz);
     const long long sy = rowlen, sx = nz;
     const long long o = (long long)y * sy + (long long)x * sx + z;
     const long long dzm = z > 0 ? -1 : 0, dzp = z < nz - 1 ? 1 : 0;
@@ -253,8 +347,8 @@ k_tv_grad(const T *__restrict__ u, T *__restrict__ grad, int ny, int nx, int nz,
     T t10 = uxp - u[o + dxp + dzm];
     T t11 = uxp - u[o + dxp + dym];
     T t12 = uxp - c;
-    T gv = (t1 + t2 + t3) / sqrt(eps + t1 * t1 + t2 * t2 + t3 * t3) - t4 / sqrt(eps + t4 * t4 + t5 * t5 + t6 * t6) -
-           t8 / sqrt(eps + t7 * t7 + t8 * t8 + t9 * t9) - t12 / sqrt(eps + t10 * t10 + t11 * t11 + t12 * t12);
+    T gv = tv_rs(t1 + t2 + t3, eps + t1 * t1 + t2 * t2 + t3 * t3) - tv_rs(t4, eps + t4 * t4 + t5 * t5 + t6 * t6) -
+           tv_rs(t8, eps + t7 * t7 + t8 * t8 + t9 * t9) - tv_rs(t12, eps + t10 * t10 + t11 * t11 + t12 * t12);
     grad[o] = gv;
     accg += (double)gv * (double)gv;
     if (c < (T)0) acca += (double)c * (double)c;
@@ -270,8 +364,8 @@ k_tv_grad(const T *__restrict__ u, T *__restrict__ grad, int ny, int nx, int nz,
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    part_g[blockIdx.x] = sg[0];
-    part_a[blockIdx.x] = sa[0];
+    part_g[blockIdx.y * gridDim.x + blockIdx.x] = sg[0];
+    part_a[blockIdx.y * gridDim.x + blockIdx.x] = sa[0];
   }
 }
 
@@ -284,14 +378,100 @@ __global__ void k_tv_update(T *__restrict__ u, const T *__restrict__ grad, int n
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= total) return;
   long long o = (long long)ylo * rowlen + i;
-  T s1 = (T)sqrt(sums[0]);
+  const double s1d = sqrt(sums[0]);
   T c = (T)(lam * sqrt(sums[1]));
-  T gn = grad[o] / s1;
+  T gn;
+  if constexpr (sizeof(T) == 4) gn = grad[o] * (T)(1.0 / s1d);
+  else gn = grad[o] / s1d;
   u[o] = u[o] - c * gn;
 }
 
 __global__ void k_sum_scalars(const double *__restrict__ part_g, const double *__restrict__ part_a, int nb,
                               double *__restrict__ sums);
+
+// One fused TV sweep s -> s+1 (denoise.c:83-145 with the next sweep's
+// gradient):  reads u_s and grad_s, forms u_{s+1} = u_s - c_s grad_s/|grad_s|
+// (c_s = lam sqrt(dA_s)) at the 13 stencil points of each band voxel,
+// writes u_{s+1} and grad_{s+1}, and FP64 partial sums of grad_{s+1}^2 and of
+// u_{s+1}^2 over u_{s+1} < 0.  Rows outside the band are never updated.
+// FP64 keeps the reference's division and square root; FP32 multiplies by
+// the reciprocal norm and uses rsqrt.
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_tv_fused(const T *__restrict__ u, const T *__restrict__ g, T *__restrict__ un, T *__restrict__ gn, int ny, int nx,
+           int nz, int ylo, int yhi, const double *__restrict__ sums, double lam, double *__restrict__ part_g,
+           double *__restrict__ part_a) {
+  __shared__ double sg[kThreads], sa[kThreads];
+  const long long rowlen = (long long)nx * nz;
+  const long long total = (long long)(yhi - ylo + 1) * rowlen;
+  const T eps = (T)1e-8;
+  const double s1d = sqrt(sums[0]);
+  const T s1 = (T)s1d, inv_s1 = (T)(1.0 / s1d);
+  const T c = (T)(lam * sqrt(sums[1]));
+  // u_{s+1} at flat offset o.  The gradient buffers hold zeros outside the
+  // band (they are cleared when allocated and only band rows are written),
+  // so the update needs no row test and every stencil read is unconditional.
+  auto nxt = [&](long long o, int) -> T {
+    T v = u[o], gg = g[o];
+    if constexpr (sizeof(T) == 4) return v - c * (gg * inv_s1);
+    else return v - c * (gg / s1);
+  };
+  double accg = 0, acca = 0;
+  // grid: x = chunks of one row (x, z), y = band row; 32-bit index math
+  const int y = ylo + (int)blockIdx.y;
+  const unsigned rl = (unsigned)rowlen, unz = (unsigned)nz;
+  for (unsigned rem = blockIdx.x * kThreads + threadIdx.x; rem < rl; rem += gridDim.x * kThreads) {
+    int x = (int)(rem / unz), z = (int)(rem - (unsigned)x * unz);
+    const long long sy = rowlen, sx = nz;
+    const long long o = (long long)y * sy + (long long)x * sx + z;
+    const long long dzm = z > 0 ? -1 : 0, dzp = z < nz - 1 ? 1 : 0;
+    const long long dym = y > 0 ? -sy : 0, dyp = y < ny - 1 ? sy : 0;
+    const long long dxm = x > 0 ? -sx : 0, dxp = x < nx - 1 ? sx : 0;
+    const int ym = y > 0 ? y - 1 : 0, yp = y < ny - 1 ? y + 1 : y;
+    T cc = nxt(o, y);
+    T t1 = cc - nxt(o + dzm, y);
+    T t2 = cc - nxt(o + dym, ym);
+    T t3 = cc - nxt(o + dxm, y);
+    T uzp = nxt(o + dzp, y);
+    T t4 = uzp - cc;
+    T t5 = uzp - nxt(o + dzp + dym, ym);
+    T t6 = uzp - nxt(o + dzp + dxm, y);
+    T uyp = nxt(o + dyp, yp);
+    T t7 = uyp - nxt(o + dyp + dzm, yp);
+    T t8 = uyp - cc;
+    T t9 = uyp - nxt(o + dyp + dxm, yp);
+    T uxp = nxt(o + dxp, y);
+    T t10 = uxp - nxt(o + dxp + dzm, y);
+    T t11 = uxp - nxt(o + dxp + dym, ym);
+    T t12 = uxp - cc;
+    T gv = tv_rs(t1 + t2 + t3, eps + t1 * t1 + t2 * t2 + t3 * t3) - tv_rs(t4, eps + t4 * t4 + t5 * t5 + t6 * t6) -
+           tv_rs(t8, eps + t7 * t7 + t8 * t8 + t9 * t9) - tv_rs(t12, eps + t10 * t10 + t11 * t11 + t12 * t12);
+    un[o] = cc;
+    gn[o] = gv;
+    accg += (double)gv * (double)gv;
+    if (cc < (T)0) acca += (double)cc * (double)cc;
+  }
+  sg[threadIdx.x] = accg;
+  sa[threadIdx.x] = acca;
+  __syncthreads();
+  for (int h = kThreads / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < h) {
+      sg[threadIdx.x] += sg[threadIdx.x + h];
+      sa[threadIdx.x] += sa[threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part_g[blockIdx.y * gridDim.x + blockIdx.x] = sg[0];
+    part_a[blockIdx.y * gridDim.x + blockIdx.x] = sa[0];
+  }
+}
+
+template <typename T>
+__global__ void k_copy_real(T *__restrict__ d, const T *__restrict__ s, long long total) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < total) d[i] = s[i];
+}
 
 // ---------------------------------------------------------------------------
 // 2D anisotropic diffusion (denoise.c:320-378) as a hyperplane sweep.

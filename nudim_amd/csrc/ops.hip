@@ -8,17 +8,28 @@
 namespace nudim {
 
 thread_local KernelProfiler *g_prof = nullptr;
+bool g_use_fft2 = getenv("NUDIM_GENERIC_FFT") == nullptr;
 
 // ---------------------------------------------------------------------------
 // small non-template kernels
 // ---------------------------------------------------------------------------
 
+// block of 256 = 32 lanes x 8 planes; lane l sums rows l, l+32, ... and a
+// fixed tree combines the lanes (same order on every run)
 __global__ void k_sum_partials(const double *__restrict__ part, int nb, int P, double *__restrict__ out) {
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= P) return;
+  __shared__ double sh[256];
+  const int pl = threadIdx.x & 7, lane = threadIdx.x >> 3;
+  const int p = blockIdx.x * 8 + pl;
   double s = 0;
-  for (int b = 0; b < nb; b++) s += part[(size_t)b * P + p];
-  out[p] = s;
+  if (p < P)
+    for (int b = lane; b < nb; b += 32) s += part[(size_t)b * P + p];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 16; h > 0; h >>= 1) {
+    if (lane < h) sh[threadIdx.x] += sh[threadIdx.x + h * 8];
+    __syncthreads();
+  }
+  if (lane == 0 && p < P) out[p] = sh[pl];
 }
 
 __global__ void k_cg_scalars(double *__restrict__ dot_z, double *__restrict__ coef, const double *__restrict__ s,
@@ -36,16 +47,28 @@ __global__ void k_cg_scalars(double *__restrict__ dot_z, double *__restrict__ co
   }
 }
 
+// one block of 256 threads; fixed-order tree
 __global__ void k_sum_scalars(const double *__restrict__ part_g, const double *__restrict__ part_a, int nb,
                               double *__restrict__ sums) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    double a = 0, b = 0;
-    for (int i = 0; i < nb; i++) {
-      a += part_g[i];
-      b += part_a[i];
+  __shared__ double sa[256], sb[256];
+  double a = 0, b = 0;
+  for (int i = threadIdx.x; i < nb; i += 256) {
+    a += part_g[i];
+    b += part_a[i];
+  }
+  sa[threadIdx.x] = a;
+  sb[threadIdx.x] = b;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) {
+      sa[threadIdx.x] += sa[threadIdx.x + h];
+      sb[threadIdx.x] += sb[threadIdx.x + h];
     }
-    sums[0] = a;
-    sums[1] = b;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    sums[0] = sa[0];
+    sums[1] = sb[0];
   }
 }
 
@@ -94,7 +117,7 @@ static void upload(DevBuf &b, const std::vector<U> &v) {
 
 int line_chunk_log2(int n, int P, int elem_bytes, int ept) {
   int cap = (kFftThreads * ept) / n;  // register stash bound
-  int lds_cap = (64 * 1024) / (n * elem_bytes);
+  int lds_cap = (64 * 1024) / (n * elem_bytes) - 1;  // lines + one twiddle table
   cap = std::min(cap, lds_cap);
   cap = std::min(cap, 16);
   NUDIM_CHECK(cap >= 1, "line transform of length " + std::to_string(n) + " does not fit in LDS");
@@ -117,10 +140,14 @@ void LineFft<T>::run(const cx<T> *in, cx<T> *out, int nlines, const LineIO &li, 
                      int P, hipStream_t st, const cx<T> *mid, long long mid_so) const {
   int Gs = line_chunk_log2(n, P, (int)sizeof(cx<T>), FftEpt<T>::value);
   int nchunk = P >> Gs;
-  size_t lds = (size_t)n * (1 << Gs) * sizeof(cx<T>);
+  size_t lds = (size_t)n * ((1 << Gs) + 1) * sizeof(cx<T>);
   dim3 grid((unsigned)((long long)nlines * nchunk));
   double moved = (double)(li.cnt + lo.cnt) * nlines * P * sizeof(cx<T>) + (mid ? (double)n * nlines * sizeof(cx<T>) : 0.0);
   ProfScope ps(mid ? KC_TOEPLITZ_COL : KC_LINE_FFT, st, moved);
+  if (g_use_fft2 && fft2_launch<T>(n, in, out, nlines, li, lo, sign, P, st, tw.as<cx<T>>(), mid, mid_so)) {
+    NUDIM_HIP(hipGetLastError());
+    return;
+  }
   hipLaunchKernelGGL(k_fft_lines<T>, grid, dim3(kFftThreads), lds, st, in, out, fd, li, lo, tw.as<cx<T>>(), sign, Gs,
                      nchunk, mid, mid_so);
   NUDIM_HIP(hipGetLastError());
@@ -178,8 +205,25 @@ void NfftOp<T>::trafo(const cx<T> *fhat, cx<T> *f, hipStream_t st) {
   dim3 gr(cdiv(M, kThreads >> PCs), P >> PCs);
   // algorithmic bytes: grid read once, node values written, node coordinates
   ProfScope ps(KC_INTERP, st, ((double)n * n * sizeof(cx<T>) + (double)M * sizeof(cx<T>) + 2.0 * M * sizeof(T)) * P);
-  hipLaunchKernelGGL(k_interp<T>, gr, dim3(kThreads), 0, st, grid.as<cx<T>>(), f, c0.as<int>(), c1.as<int>(),
-                     psi0.as<double>(), psi1.as<double>(), M, n, W, P, PCs);
+  const cx<T> *gp = grid.as<cx<T>>();
+  const int *q0 = c0.as<int>(), *q1 = c1.as<int>();
+  const double *p0 = psi0.as<double>(), *p1 = psi1.as<double>();
+#define NUDIM_INTERP_W(WW)                                                                                      \
+  case WW:                                                                                                      \
+    hipLaunchKernelGGL((k_interp_w<T, WW>), gr, dim3(kThreads), 0, st, gp, f, q0, q1, p0, p1, M, n, P, PCs); \
+    break;
+  switch (W) {
+    NUDIM_INTERP_W(4)
+    NUDIM_INTERP_W(6)
+    NUDIM_INTERP_W(8)
+    NUDIM_INTERP_W(10)
+    NUDIM_INTERP_W(12)
+    NUDIM_INTERP_W(14)
+    NUDIM_INTERP_W(16)
+    default:
+      hipLaunchKernelGGL(k_interp<T>, gr, dim3(kThreads), 0, st, gp, f, q0, q1, p0, p1, M, n, W, P, PCs);
+  }
+#undef NUDIM_INTERP_W
   NUDIM_HIP(hipGetLastError());
 }
 
@@ -192,8 +236,26 @@ void NfftOp<T>::adjoint(const cx<T> *f, cx<T> *fhat, hipStream_t st) {
   {
   // algorithmic bytes: node values read, node ids of the inverse map, grid written
   ProfScope ps(KC_GATHER, st, ((double)M * sizeof(cx<T>) + 4.0 * M + (double)ncell * sizeof(cx<T>)) * P);
-  hipLaunchKernelGGL(k_gather<T>, gr, dim3(kThreads), 0, st, f, grid.as<cx<T>>(), cell_ptr.as<int>(),
-                     ent.as<uint32_t>(), psi0.as<double>(), psi1.as<double>(), ncell, W, P, PCs);
+  cx<T> *gp = grid.as<cx<T>>();
+  const int *cp = cell_ptr.as<int>();
+  const uint32_t *ep = ent.as<uint32_t>();
+  const double *p0 = psi0.as<double>(), *p1 = psi1.as<double>();
+#define NUDIM_GATHER_W(WW)                                                                                 \
+  case WW:                                                                                                 \
+    hipLaunchKernelGGL((k_gather_w<T, WW>), gr, dim3(kThreads), 0, st, f, gp, cp, ep, p0, p1, ncell, P, PCs); \
+    break;
+  switch (W) {
+    NUDIM_GATHER_W(4)
+    NUDIM_GATHER_W(6)
+    NUDIM_GATHER_W(8)
+    NUDIM_GATHER_W(10)
+    NUDIM_GATHER_W(12)
+    NUDIM_GATHER_W(14)
+    NUDIM_GATHER_W(16)
+    default:
+      hipLaunchKernelGGL(k_gather<T>, gr, dim3(kThreads), 0, st, f, gp, cp, ep, p0, p1, ncell, W, P, PCs);
+  }
+#undef NUDIM_GATHER_W
   }
   NUDIM_HIP(hipGetLastError());
   const double *d = dec.as<double>();
@@ -302,7 +364,7 @@ void Reducer::dot(const cx<T> *a, const double *w, long long rows, double *out, 
   dim3 gr(kRedBlocks, P >> PCs);
   hipLaunchKernelGGL((k_reduce<T, 0>), gr, dim3(kThreads), 0, st, a, (const cx<T> *)nullptr, w, (const int *)nullptr,
                      rows, P, PCs, part.as<double>(), (double *)nullptr);
-  hipLaunchKernelGGL(k_sum_partials, dim3(cdiv(P, 64)), dim3(64), 0, st, part.as<double>(), kRedBlocks, P, out);
+  hipLaunchKernelGGL(k_sum_partials, dim3(cdiv(P, 8)), dim3(256), 0, st, part.as<double>(), kRedBlocks, P, out);
   NUDIM_HIP(hipGetLastError());
 }
 
@@ -312,7 +374,7 @@ void Reducer::redot(const cx<T> *a, const cx<T> *b, long long rows, double *out,
   dim3 gr(kRedBlocks, P >> PCs);
   hipLaunchKernelGGL((k_reduce<T, 2>), gr, dim3(kThreads), 0, st, a, b, (const double *)nullptr,
                      (const int *)nullptr, rows, P, PCs, part.as<double>(), (double *)nullptr);
-  hipLaunchKernelGGL(k_sum_partials, dim3(cdiv(P, 64)), dim3(64), 0, st, part.as<double>(), kRedBlocks, P, out);
+  hipLaunchKernelGGL(k_sum_partials, dim3(cdiv(P, 8)), dim3(256), 0, st, part.as<double>(), kRedBlocks, P, out);
   NUDIM_HIP(hipGetLastError());
 }
 
@@ -323,8 +385,8 @@ void Reducer::rfactor(const cx<T> *f, const cx<T> *meas, const int *kind, long l
   dim3 gr(kRedBlocks, P >> PCs);
   hipLaunchKernelGGL((k_reduce<T, 1>), gr, dim3(kThreads), 0, st, f, meas, (const double *)nullptr, kind, rows, P,
                      PCs, part.as<double>(), part2.as<double>());
-  hipLaunchKernelGGL(k_sum_partials, dim3(cdiv(P, 64)), dim3(64), 0, st, part.as<double>(), kRedBlocks, P, d1);
-  hipLaunchKernelGGL(k_sum_partials, dim3(cdiv(P, 64)), dim3(64), 0, st, part2.as<double>(), kRedBlocks, P, d2);
+  hipLaunchKernelGGL(k_sum_partials, dim3(cdiv(P, 8)), dim3(256), 0, st, part.as<double>(), kRedBlocks, P, d1);
+  hipLaunchKernelGGL(k_sum_partials, dim3(cdiv(P, 8)), dim3(256), 0, st, part2.as<double>(), kRedBlocks, P, d2);
   NUDIM_HIP(hipGetLastError());
 }
 
@@ -393,6 +455,56 @@ void Cgnr<T>::solve(const cx<T> *y, cx<T> *fhat, int nsteps, bool f0_zero, hipSt
     NUDIM_HIP(hipGetLastError());
   }
 }
+
+// ---------------------------------------------------------------------------
+// TV sweeps (denoise.c:67-249): gradient pre-pass, niter-1 fused sweeps, and
+// a final update, each followed by the fixed-order sum of the two norms
+// ---------------------------------------------------------------------------
+
+template <typename T>
+T *tv_sweeps(T *u0, T *u1, T *g0, T *g1, int ny, int nx, int nz, int ylo, int yhi, float lam, int niter,
+             double *part_g, double *part_a, double *sums, int nb, hipStream_t st) {
+  if (niter <= 0 || yhi < ylo) return u0;
+  const long long V = (long long)ny * nx * nz;
+  const long long band = (long long)(yhi - ylo + 1) * nx * nz;
+  const double lamd = (double)lam;
+  const double vb = (double)band * sizeof(T);  // algorithmic: read u once + write u once per sweep
+  // grid: (chunks of a row, band rows); one partial sum per block
+  const int nrows = yhi - ylo + 1;
+  const int gx = (int)std::min<long long>(cdiv((long long)nx * nz, kThreads), 64);
+  const int np_ = nrows * gx;
+  NUDIM_CHECK(np_ <= nb, "TV partial-sum buffer too small");
+  const dim3 grid(gx, nrows);
+  // rows outside the band keep their values in both buffers
+  hipLaunchKernelGGL(k_copy_real<T>, dim3(cdiv(V, 256)), dim3(256), 0, st, u1, u0, V);
+  {
+    ProfScope ps(KC_TV, st, vb);
+    hipLaunchKernelGGL(k_tv_grad<T>, grid, dim3(kThreads), 0, st, u0, g0, ny, nx, nz, ylo, yhi, part_g, part_a);
+  }
+  hipLaunchKernelGGL(k_sum_scalars, dim3(1), dim3(256), 0, st, part_g, part_a, np_, sums);
+  T *uc = u0, *un = u1, *gc = g0, *gn = g1;
+  for (int s = 0; s + 1 < niter; s++) {
+    {
+      ProfScope ps(KC_TV, st, 2.0 * vb);
+      hipLaunchKernelGGL(k_tv_fused<T>, grid, dim3(kThreads), 0, st, uc, gc, un, gn, ny, nx, nz, ylo, yhi, sums, lamd,
+                         part_g, part_a);
+    }
+    hipLaunchKernelGGL(k_sum_scalars, dim3(1), dim3(256), 0, st, part_g, part_a, np_, sums);
+    std::swap(uc, un);
+    std::swap(gc, gn);
+  }
+  {
+    ProfScope ps(KC_TV, st, vb);
+    hipLaunchKernelGGL(k_tv_update<T>, dim3(cdiv(band, 256)), dim3(256), 0, st, uc, gc, nx, nz, ylo, yhi, sums, lamd);
+  }
+  NUDIM_HIP(hipGetLastError());
+  return uc;
+}
+
+template float *tv_sweeps<float>(float *, float *, float *, float *, int, int, int, int, int, float, int, double *,
+                                 double *, double *, int, hipStream_t);
+template double *tv_sweeps<double>(double *, double *, double *, double *, int, int, int, int, int, float, int,
+                                   double *, double *, double *, int, hipStream_t);
 
 template struct LineFft<float>;
 template struct LineFft<double>;

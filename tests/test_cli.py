@@ -1,0 +1,108 @@
+"""The drop-in `reconstruction` binary: command line, files, output formats."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from tests.util import blob_phantom, rel
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "nudim_amd", "bin", "reconstruction")
+
+
+def run(args, cwd, env=None):
+    e = dict(os.environ)
+    if env:
+        e.update(env)
+    return subprocess.run([BIN] + args, cwd=cwd, capture_output=True, text=True, env=e, timeout=600)
+
+
+@pytest.fixture(scope="module")
+def binary(hip):
+    if not os.path.exists(BIN):
+        from nudim_amd import build
+        build.build_host()
+    return BIN
+
+
+# (arguments, message of src/reconstruction.c:245-293)
+BAD = [
+    (["--nx=8", "--nz=8", "-p", "8"], "You need to provide prefix name of tomogram files with --prefix."),
+    (["--prefix=prj", "--nz=8"], "You need to provide valid projection size with --nx, --nz."),
+    (["--prefix=prj", "--nx=8", "--nz=8"], "You need to provide valid projection dimension in XY plane with --pad."),
+    (["--prefix=prj", "--nx=8", "--nz=8", "-p", "8"], "You need to provide valid nhio with --nhio."),
+    (["--prefix=prj", "--nx=8", "--nz=8", "-p", "8", "--nhio=0"], "You need to provide valid ner with --ner."),
+    (["--prefix=prj", "--nx=8", "--nz=8", "-p", "8", "--nhio=0", "--ner=1"],
+     "You need to provide valid angular sampling interval with --interval."),
+    (["--prefix=prj", "--nx=8", "--nz=8", "-p", "8", "--nhio=0", "--ner=1", "-t", "3"],
+     "You need to provide valid padded angular sampling interval with --padangle."),
+    (["--prefix=prj", "--nx=8", "--nz=8", "-p", "8", "--nhio=0", "--ner=1", "-t", "3", "--padangle=2"],
+     "You need to provide valid padded angular sampling interval with --padangle."),
+    (["--prefix=prj", "--nx=8", "--nz=8", "-p", "8", "--nhio=0", "--ner=1", "-t", "3", "--padangle=1"],
+     "You need to provide valid scope of tomogram angles (-90,90] with --max, --min."),
+    (["--prefix=prj", "--nx=8", "--nz=8", "-p", "8", "--nhio=0", "--ner=1", "-t", "3", "--padangle=1",
+      "--max=60", "--min=-60", "--denoise"], "You must provide valid denoising method or delete --denoise."),
+    (["--prefix=prj", "--nx=8", "--nz=8", "-p", "8", "--nhio=0", "--ner=1", "-t", "3", "--padangle=1",
+      "--max=60", "--min=-60", "--height=4"],
+     "You must provide valid both sample height and center with --height and --center."),
+    (["--prefix=prj", "--nx=8x"], "Invalid X dimension!"),
+    (["--prefix=prj", "-t", "3a"], "Invalid angular sampling interval!"),
+]
+
+
+@pytest.mark.parametrize("args,msg", BAD)
+def test_validation_messages(binary, tmp_path, args, msg):
+    r = run(args, tmp_path)
+    assert r.returncode == 1
+    assert msg in r.stderr
+
+
+def test_help_and_unknown_option(binary, tmp_path):
+    r = run(["-h"], tmp_path)
+    assert r.returncode == 0 and "--padangle" in r.stdout
+    r = run(["--bogus"], tmp_path)
+    assert r.returncode == 1
+
+
+def test_missing_projection_file(binary, tmp_path):
+    (tmp_path / "files").mkdir()
+    r = run(["--prefix=prj", "--nx=16", "--nz=16", "-p", "16", "--nhio=0", "--ner=1", "-t", "30",
+             "--padangle=10", "--max=60", "--min=-60"], tmp_path)
+    assert r.returncode != 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_cli_end_to_end(binary, orc, tmp_path, prec):
+    np_ = nz = 32
+    rng = np.random.default_rng(21)
+    dens = blob_phantom((nz, np_, np_), 10, rng, sig=(1.5, 3.0), slab=16)
+    proj = orc.project_2d1d(dens, np_, -60, 60, 5)
+    files = tmp_path / "files"
+    files.mkdir()
+    for j in range(proj.shape[0]):
+        # projection_sim writes "%f\n" (src/projection_sim.c:401-402)
+        np.savetxt(files / ("prj-%d.txt" % j), proj[j].ravel(), fmt="%f")
+    args = ["--prefix=prj", "--pad=32", "--nx=24", "--nz=32", "--max=60", "--min=-60", "-t", "5",
+            "--padangle=1", "--nhio=1", "--ner=2", "--weight", "--print"]
+    r = run(args, tmp_path, env={"NUDIM_PRECISION": prec})
+    assert r.returncode == 0, r.stderr
+    lines = [l for l in r.stdout.splitlines() if l.startswith("Iteration:")]
+    assert len(lines) == 3 and lines[0].startswith("Iteration: 0 Measured Residual: ")
+    # the oracle on the same (text-rounded) files; with --nx=24 the reader
+    # takes the first 24*32 values of each file (src/reconstruction.c:345-356)
+    proj_in = np.stack([np.loadtxt(files / ("prj-%d.txt" % j))[: 24 * 32].reshape(32, 24)
+                        for j in range(proj.shape[0])])
+    op = orc.params(24, nz, np_, -60, 60, 5, padangle=1.0, weight=1, nhio=1, ner=2)
+    img0, img, rt = orc.reconstruct(op, proj_in)
+    r0 = np.loadtxt(tmp_path / "recovered-0.txt").reshape(nz, np_, np_)
+    rec = np.loadtxt(tmp_path / "recovered.txt").reshape(nz, 24, 24)
+    crop = img[:, 4:28, 4:28]
+    # recovered-0.txt is "%le": 7 significant digits
+    tol0 = 1e-6 if prec == "fp64" else 2e-5
+    assert rel(r0, img0) < tol0
+    # recovered.txt is "%f": 6 decimals
+    assert np.max(np.abs(rec - crop)) < (1e-6 if prec == "fp64" else 1e-3 * np.abs(crop).max())
+    printed = np.array([float(l.split()[-1]) for l in lines])
+    assert np.max(np.abs(printed - rt)) < (1e-6 if prec == "fp64" else 1e-4)
