@@ -114,11 +114,12 @@ k_gather_w(const cx<T> *__restrict__ s, cx<T> *__restrict__ g, const int *__rest
   const int e0 = cell_ptr[cell], e1 = cell_ptr[cell + 1];
   cx<T> acc = mk((T)0, (T)0);
   constexpr int B = 8;
-  int e = e0;
-  for (; e + B <= e1; e += B) {
+  // batches of B entries; entries past e1 re-read the last one (clamped) and
+  // are not accumulated, so every read of a batch is unconditional
+  for (int e = e0; e < e1; e += B) {
     uint32_t u[B];
 #pragma unroll
-    for (int j = 0; j < B; j++) u[j] = ent[e + j];
+    for (int j = 0; j < B; j++) u[j] = ent[min(e + j, e1 - 1)];
     double w0[B], w1[B];
     cx<T> sv[B];
 #pragma unroll
@@ -129,13 +130,8 @@ k_gather_w(const cx<T> *__restrict__ s, cx<T> *__restrict__ g, const int *__rest
       sv[j] = s[(size_t)node * P + p];
     }
 #pragma unroll
-    for (int j = 0; j < B; j++) acc = acc + (T)(w0[j] * w1[j]) * sv[j];
-  }
-  for (; e < e1; e++) {
-    uint32_t u = ent[e];
-    int node = (int)(u & 0xFFFFFu), t0 = (int)((u >> 20) & 63u), t1 = (int)(u >> 26);
-    T w = (T)(psi0[(size_t)node * W + t0] * psi1[(size_t)node * W + t1]);
-    acc = acc + w * s[(size_t)node * P + p];
+    for (int j = 0; j < B; j++)
+      if (e + j < e1) acc = acc + (T)(w0[j] * w1[j]) * sv[j];
   }
   g[(size_t)cell * P + p] = acc;
 }
@@ -237,6 +233,49 @@ __global__ void k_axpy_planes(cx<T> *__restrict__ f, const cx<T> *__restrict__ x
   T a = (T)(sgn * coef[p]);
   cx<T> v = x[i];
   f[i] = f[i] + a * v;
+}
+
+// Toeplitz CGNR step, fused: alpha_p = zz[p] / pq[p];  f += alpha p;
+// z -= alpha q;  FP64 partial sums of |z|^2 per (row block, plane).
+// Block (bx, by): plane chunk by of PC = 2^PCs planes, rows bx, bx+gridDim.x..
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_cg_update(cx<T> *__restrict__ f, const cx<T> *__restrict__ pv, cx<T> *__restrict__ z, const cx<T> *__restrict__ q,
+            const double *__restrict__ zz, const double *__restrict__ pq, long long rows, int P, int PCs,
+            double *__restrict__ part) {
+  __shared__ double sh[kThreads];
+  const int PC = 1 << PCs, lanes = kThreads >> PCs;
+  const int pl = threadIdx.x & (PC - 1), lr = threadIdx.x >> PCs;
+  const int p = blockIdx.y * PC + pl;
+  const double alpha = zz[p] / pq[p];
+  const T a = (T)alpha, na = (T)(-alpha);
+  double s = 0;
+  for (long long k = (long long)blockIdx.x * lanes + lr; k < rows; k += (long long)gridDim.x * lanes) {
+    const long long i = k * P + p;
+    cx<T> pp = pv[i], qq = q[i], ff = f[i], zv = z[i];
+    f[i] = ff + a * pp;
+    cx<T> zn = zv + na * qq;
+    z[i] = zn;
+    s += (double)zn.x * zn.x + (double)zn.y * zn.y;
+  }
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = lanes >> 1; h > 0; h >>= 1) {
+    if (lr < h) sh[threadIdx.x] += sh[threadIdx.x + h * PC];
+    __syncthreads();
+  }
+  if (lr == 0) part[(size_t)blockIdx.x * P + p] = sh[pl];
+}
+
+// p = beta p + z with beta_p = zz_new[p] / zz_old[p]
+template <typename T>
+__global__ void k_xpby_ratio(cx<T> *__restrict__ pv, const cx<T> *__restrict__ z, const double *__restrict__ zz_new,
+                             const double *__restrict__ zz_old, long long total, int P) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  int p = (int)(i % P);
+  T b = (T)(zz_new[p] / zz_old[p]);
+  pv[i] = b * pv[i] + z[i];
 }
 
 // p = beta_p * p + z

@@ -183,11 +183,17 @@ void NfftOp<T>::init(int N_, int n_, int W_, const double *x, int M_, int P_) {
   grid.alloc((size_t)n * n * P * sizeof(cx<T>));
 }
 
-static int plane_group_log2(int P) {
+// lanes per node / cell: the largest power of two <= cap dividing P
+static int plane_group_log2(int P, int cap = 64) {
   int s = 0;
-  while ((2 << s) <= 64 && P % (2 << s) == 0) s++;
+  while ((2 << s) <= cap && P % (2 << s) == 0) s++;
   return s;
 }
+
+// planes per cell group of the gather adjoint (a power of two up to 64):
+// a whole wavefront on one cell's planes keeps every entry load coalesced
+// across planes, which measured faster than several cells per wavefront
+static const int g_gather_lanes = getenv("NUDIM_GATHER_LANES") ? atoi(getenv("NUDIM_GATHER_LANES")) : 64;
 
 template <typename T>
 void NfftOp<T>::trafo(const cx<T> *fhat, cx<T> *f, hipStream_t st) {
@@ -230,7 +236,7 @@ void NfftOp<T>::trafo(const cx<T> *fhat, cx<T> *f, hipStream_t st) {
 template <typename T>
 void NfftOp<T>::adjoint(const cx<T> *f, cx<T> *fhat, hipStream_t st) {
   const long long NP = (long long)N * P, nP = (long long)n * P;
-  int PCs = plane_group_log2(P);
+  int PCs = plane_group_log2(P, g_gather_lanes);
   int ncell = n * n;
   dim3 gr(cdiv(ncell, kThreads >> PCs), P >> PCs);
   {
@@ -431,24 +437,34 @@ void Cgnr<T>::solve(const cx<T> *y, cx<T> *fhat, int nsteps, bool f0_zero, hipSt
   hipLaunchKernelGGL(k_cg_scalars, dim3(cdiv(P, 64)), dim3(64), 0, st, dz, cf, s, P, 0);
   hipLaunchKernelGGL(k_copy_cx<T>, dim3(bN), dim3(256), 0, st, pp, zp, NN);
   NUDIM_HIP(hipGetLastError());
-  for (int it = 0; it < nsteps; it++) {
-    if (form == NUDIM_CG_TOEPLITZ) {
-      // q = A^H W A p ; dot_v = <p, q> = sum w |A p|^2
-      cx<T> *qp = q.as<cx<T>>();
+  if (form == NUDIM_CG_TOEPLITZ) {
+    // q = A^H W A p;  <p, q> = sum w |A p|^2;  alpha = |z|^2 / <p, q>;
+    // f += alpha p;  z -= alpha q;  beta = |z_new|^2 / |z|^2;  p = z + beta p
+    cx<T> *qp = q.as<cx<T>>();
+    double *zz = dz, *zz_new = cf, *pq = s;
+    const int PCs = plane_group_log2(P);
+    const dim3 rg(kRedBlocks, P >> PCs);
+    for (int it = 0; it < nsteps; it++) {
       toep->apply(pp, qp, st);
-      red.redot<T>(pp, qp, (long long)N * N, s, st);
-      hipLaunchKernelGGL(k_cg_scalars, dim3(cdiv(P, 64)), dim3(64), 0, st, dz, cf, s, P, 1);
-      hipLaunchKernelGGL(k_axpy_planes<T>, dim3(bN), dim3(256), 0, st, fhat, pp, cf, 1.0, NN, P);
-      hipLaunchKernelGGL(k_axpy_planes<T>, dim3(bN), dim3(256), 0, st, zp, qp, cf, -1.0, NN, P);
-    } else {
-      // v = A p ; dot_v = sum w |v|^2
-      op->trafo(pp, vp, st);
-      red.dot<T>(vp, w, M, s, st);
-      hipLaunchKernelGGL(k_cg_scalars, dim3(cdiv(P, 64)), dim3(64), 0, st, dz, cf, s, P, 1);
-      hipLaunchKernelGGL(k_axpy_planes<T>, dim3(bN), dim3(256), 0, st, fhat, pp, cf, 1.0, NN, P);
-      hipLaunchKernelGGL(k_resid_update<T>, dim3(bM), dim3(256), 0, st, rp, vp, cf, w, tp, MM, P);
-      op->adjoint(tp, zp, st);
+      red.redot<T>(pp, qp, (long long)N * N, pq, st);
+      hipLaunchKernelGGL(k_cg_update<T>, rg, dim3(kThreads), 0, st, fhat, pp, zp, qp, zz, pq, (long long)N * N, P,
+                         PCs, red.part.as<double>());
+      hipLaunchKernelGGL(k_sum_partials, dim3(cdiv(P, 8)), dim3(256), 0, st, red.part.as<double>(), kRedBlocks, P,
+                         zz_new);
+      hipLaunchKernelGGL(k_xpby_ratio<T>, dim3(bN), dim3(256), 0, st, pp, zp, zz_new, zz, NN, P);
+      NUDIM_HIP(hipGetLastError());
+      std::swap(zz, zz_new);
     }
+    return;
+  }
+  for (int it = 0; it < nsteps; it++) {
+    // v = A p ; dot_v = sum w |v|^2
+    op->trafo(pp, vp, st);
+    red.dot<T>(vp, w, M, s, st);
+    hipLaunchKernelGGL(k_cg_scalars, dim3(cdiv(P, 64)), dim3(64), 0, st, dz, cf, s, P, 1);
+    hipLaunchKernelGGL(k_axpy_planes<T>, dim3(bN), dim3(256), 0, st, fhat, pp, cf, 1.0, NN, P);
+    hipLaunchKernelGGL(k_resid_update<T>, dim3(bM), dim3(256), 0, st, rp, vp, cf, w, tp, MM, P);
+    op->adjoint(tp, zp, st);
     red.dot<T>(zp, nullptr, (long long)N * N, s, st);
     hipLaunchKernelGGL(k_cg_scalars, dim3(cdiv(P, 64)), dim3(64), 0, st, dz, cf, s, P, 2);
     hipLaunchKernelGGL(k_xpby_planes<T>, dim3(bN), dim3(256), 0, st, pp, zp, cf, NN, P);
