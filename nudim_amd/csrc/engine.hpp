@@ -15,6 +15,13 @@ bool fft2_launch(int n, const cx<T> *in, cx<T> *out, int nlines, const LineIO &l
                  hipStream_t st, const cx<T> *tw, const cx<T> *mid, long long mid_so);
 // process-wide switch for the fixed-length kernels (on by default)
 extern bool g_use_fft2;
+// Toeplitz passes with the CGNR vector work (fft2.hip) for n = 2N in
+// toep_supported(n); kind is TK_ROWF, TK_COL or TK_ROWI
+bool toep_supported(int n);
+template <typename T>
+void toep_launch(int kind, int n, int nlines, const cx<T> *in, cx<T> *out, long long in_so, long long in_sk,
+                 long long out_so, long long out_sk, int P, hipStream_t st, const cx<T> *tw, const T *mid,
+                 const ToepCg<T> &cg);
 
 // choose log2 of the plane chunk G of a line transform of length n over P planes
 int line_chunk_log2(int n, int P, int elem_bytes, int ept);
@@ -36,6 +43,9 @@ struct NfftOp {
   int N = 0, n = 0, W = 0, M = 0, P = 0;
   LineFft<T> fft;
   DevBuf c0, c1, psi0, psi1, dec, cell_ptr, ent;
+  // tiled gather map (used when n % kGatherTile == 0 and P % 64 == 0)
+  bool tiled = false;
+  DevBuf tile_chunk, chunk_node, tnode, chunk_cell, tslot, tweight;
   DevBuf X, grid;  // [N][n][P], [n][n][P]
   size_t bytes() const;
   void init(int N, int n, int W, const double *x, int M, int P);
@@ -50,10 +60,15 @@ struct ToeplitzOp {
   int N = 0, L = 0, P = 0;
   LineFft<T> fft;
   DevBuf khat;  // cx<T>[L][L], stored [column][row], scaled by 1/L^2
+  DevBuf khat_re;  // T[L][L]: its real part (the kernel is Hermitian, so khat is real)
   DevBuf X;     // [N][L][P]
   size_t bytes() const;
   void init(int N, const double *x, const double *w, int M, int P, hipStream_t st);
   void apply(const cx<T> *p, cx<T> *q, hipStream_t st);
+  // fused CGNR step on the fixed-length passes (when fused()): p from z and
+  // the old p (see ToepCg), q = T p, part[N][P] = row sums of Re <p, q>
+  bool fused() const { return g_use_fft2 && toep_supported(L); }
+  void apply_cg(const ToepCg<T> &cg, cx<T> *q, hipStream_t st);
 };
 
 // niter sweeps of discrete TV over rows ylo..yhi (clipped by the caller) on
@@ -87,6 +102,7 @@ struct Cgnr {
   const double *w = nullptr;  // device weights [M] or null
   DevBuf z, pv, q, r, v, t;   // z, p, q: [N^2][P]; r, v, t: [M][P]
   DevBuf dot_z, coef, sc;     // [P] each
+  DevBuf zz2, alpha, rowpart; // fused Toeplitz steps: [P], [P], [N][P]
   Reducer red;
   size_t bytes() const;
   void init(int N, int M, int P, int form, NfftOp<T> *op, ToeplitzOp<T> *toep, const double *w);

@@ -189,6 +189,24 @@ struct LineIO {
   const double *scale_o;
 };
 
+// Passes of the Toeplitz normal operator on the fixed-length kernels:
+// row forward (with the CGNR prologue), column (forward, product, inverse),
+// row inverse (with the <p, q> partial sums).
+enum { TK_ROWF = 0, TK_COL = 1, TK_ROWI = 2 };
+
+// CGNR vectors carried by the row passes, [N][N][P] with planes innermost.
+//   row forward: p = z (zz_old null) or p = (zz/zz_old) p + z, after
+//                f += alpha p with the old p (alpha non-null); then FFT of p
+//   row inverse: q = the cropped inverse; part[row][plane] = sum Re conj(p) q
+template <typename T>
+struct ToepCg {
+  const cx<T> *z;
+  cx<T> *p;
+  cx<T> *f;
+  const double *zz, *zz_old, *alpha;
+  double *part;
+};
+
 // Batched line FFT over a plane-innermost array.  Block b (renumbered so
 // that neighbouring plane chunks run on the same XCD and share L2 lines)
 // handles line o = b / nchunk and planes [chunk*G, chunk*G+G).  With `mid`

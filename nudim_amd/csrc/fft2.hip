@@ -32,7 +32,67 @@ bool launch_shape(const cx<T> *in, cx<T> *out, int nlines, const LineIO &li, con
   return true;
 }
 
+template <typename T, int N1, int N2>
+void launch_toep(int kind, int nlines, const cx<T> *in, cx<T> *out, long long in_so, long long in_sk,
+                 long long out_so, long long out_sk, int P, hipStream_t st, const cx<T> *tw, const T *mid,
+                 const ToepCg<T> &cg) {
+  constexpr int n = N1 * N2;
+  int cap = std::max(1, std::min(256, g_fft2_threads) / N1);
+  int Gs = 0;
+  while ((2 << Gs) <= cap && P % (2 << Gs) == 0) Gs++;
+  const int G = 1 << Gs;
+  const size_t lds = (size_t)N2 * (N1 * G + G) * (toep_split(kind) ? 1 : 2) * sizeof(T) + (size_t)n * sizeof(cx<T>);
+  const int nchunk = P >> Gs;
+  dim3 grid((unsigned)((long long)nlines * nchunk)), block(N1 * G);
+  switch (kind) {
+    case TK_ROWF:
+      hipLaunchKernelGGL((k_toep<T, N1, N2, TK_ROWF>), grid, block, lds, st, in, out, in_so, in_sk, out_so, out_sk,
+                         tw, Gs, nchunk, mid, cg);
+      break;
+    case TK_COL:
+      hipLaunchKernelGGL((k_toep<T, N1, N2, TK_COL>), grid, block, lds, st, in, out, in_so, in_sk, out_so, out_sk,
+                         tw, Gs, nchunk, mid, cg);
+      break;
+    default:
+      hipLaunchKernelGGL((k_toep<T, N1, N2, TK_ROWI>), grid, block, lds, st, in, out, in_so, in_sk, out_so, out_sk,
+                         tw, Gs, nchunk, mid, cg);
+  }
+}
+
 }  // namespace
+
+bool toep_supported(int n) {
+  switch (n) {
+    case 128: case 256: case 384: case 512: case 768: case 1024: case 1536: return true;
+    default: return false;
+  }
+}
+
+template <typename T>
+void toep_launch(int kind, int n, int nlines, const cx<T> *in, cx<T> *out, long long in_so, long long in_sk,
+                 long long out_so, long long out_sk, int P, hipStream_t st, const cx<T> *tw, const T *mid,
+                 const ToepCg<T> &cg) {
+#define NUDIM_TOEP(NN, A, B) \
+  case NN: launch_toep<T, A, B>(kind, nlines, in, out, in_so, in_sk, out_so, out_sk, P, st, tw, mid, cg); return;
+  switch (n) {
+    NUDIM_TOEP(128, 16, 8)
+    NUDIM_TOEP(256, 16, 16)
+    NUDIM_TOEP(384, 24, 16)
+    NUDIM_TOEP(512, 32, 16)
+    NUDIM_TOEP(768, 32, 24)
+    NUDIM_TOEP(1024, 32, 32)
+    NUDIM_TOEP(1536, 48, 32)
+    default: throw Error(NUDIM_EINVAL, "no fixed-length Toeplitz pass for length " + std::to_string(n));
+  }
+#undef NUDIM_TOEP
+}
+
+template void toep_launch<float>(int, int, int, const cx<float> *, cx<float> *, long long, long long, long long,
+                                 long long, int, hipStream_t, const cx<float> *, const float *,
+                                 const ToepCg<float> &);
+template void toep_launch<double>(int, int, int, const cx<double> *, cx<double> *, long long, long long, long long,
+                                  long long, int, hipStream_t, const cx<double> *, const double *,
+                                  const ToepCg<double> &);
 
 // Returns false when no fixed-length kernel exists for n (the caller then
 // uses the generic Stockham kernel).

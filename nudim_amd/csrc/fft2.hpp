@@ -252,4 +252,229 @@ k_fft2(const cx<T> *in, cx<T> *out, LineIO li, LineIO lo, const cx<T> *__restric
   }
 }
 
+// ---- Toeplitz passes ---------------------------------------------------------------
+//
+// The circulant embedding of an N x N plane into n x n, n = 2N, holds the
+// plane in the first N positions of every line and keeps only the first N
+// outputs.  With positions j2 + N2 j1 this makes inputs j1 >= N1/2 zero in
+// step A and leaves only outputs m < R/2 wanted in the last DFT, so those
+// DFTs run at half size.  The row passes also carry the CGNR vector work.
+
+// R-point DFT (sign S) of v whose inputs v[R/2..R) are zero (not read):
+// even outputs = DFT_{R/2}(v), odd outputs = DFT_{R/2}(v[j] w_R^{Sj}).
+template <typename T, int R, int S>
+__device__ __forceinline__ void dft_half_in(cx<T> *v) {
+  constexpr int H = R / 2;
+  cx<T> e[H], o[H];
+#pragma unroll
+  for (int j = 0; j < H; j++) {
+    e[j] = v[j];
+    o[j] = (j == 0) ? v[0] : v[j] * wconst<T, S>(j, R);
+  }
+  dft_reg<T, H, S>(e);
+  dft_reg<T, H, S>(o);
+#pragma unroll
+  for (int k = 0; k < H; k++) {
+    v[2 * k] = e[k];
+    v[2 * k + 1] = o[k];
+  }
+}
+
+// First R/2 outputs of the R-point DFT (sign S) of v:
+// Y[m] = DFT_{R/2}(v[2j])[m] + w_R^{Sm} DFT_{R/2}(v[2j+1])[m].
+template <typename T, int R, int S>
+__device__ __forceinline__ void dft_half_out(cx<T> *v) {
+  constexpr int H = R / 2;
+  cx<T> e[H], o[H];
+#pragma unroll
+  for (int j = 0; j < H; j++) {
+    e[j] = v[2 * j];
+    o[j] = v[2 * j + 1];
+  }
+  dft_reg<T, H, S>(e);
+  dft_reg<T, H, S>(o);
+#pragma unroll
+  for (int m = 0; m < H; m++) v[m] = e[m] + ((m == 0) ? o[0] : o[m] * wconst<T, S>(m, R));
+}
+
+// Exchange of the four-step transform through LDS.  SPLIT: through a real
+// buffer, real parts then imaginary parts -- half the LDS of a complex
+// buffer (more blocks per CU) for two more barriers.  On entry the buffer
+// must be free; on exit it is still being read.
+template <typename T, int NW, int NR, bool SPLIT>
+__device__ __forceinline__ void xchg(T *lx, const cx<T> *wv, bool wact, int wbase, int wstride, cx<T> *rv, bool ract,
+                                     int rbase, int rstride) {
+  if constexpr (!SPLIT) {
+    cx<T> *lc = reinterpret_cast<cx<T> *>(lx);
+    if (wact) {
+#pragma unroll
+      for (int k = 0; k < NW; k++) lc[wbase + k * wstride] = wv[k];
+    }
+    __syncthreads();
+    if (ract) {
+#pragma unroll
+      for (int j = 0; j < NR; j++) rv[j] = lc[rbase + j * rstride];
+    }
+    return;
+  }
+  if (wact) {
+#pragma unroll
+    for (int k = 0; k < NW; k++) lx[wbase + k * wstride] = wv[k].x;
+  }
+  __syncthreads();
+  if (ract) {
+#pragma unroll
+    for (int j = 0; j < NR; j++) rv[j].x = lx[rbase + j * rstride];
+  }
+  __syncthreads();
+  if (wact) {
+#pragma unroll
+    for (int k = 0; k < NW; k++) lx[wbase + k * wstride] = wv[k].y;
+  }
+  __syncthreads();
+  if (ract) {
+#pragma unroll
+    for (int j = 0; j < NR; j++) rv[j].y = lx[rbase + j * rstride];
+  }
+}
+
+// Line o of a pass, plane chunk*G + g; element k of a line at
+// o*so + k*sk (+ plane).  ROWF: o = row, in = p (N positions) -> out = X
+// (n positions).  COL: o = column of X, rows 0..N-1 -> forward, times the
+// real mid[o*n + k], inverse, rows 0..N-1 back in place.  ROWI: o = row of X
+// (n positions) -> out = q (N positions).  LDS: the exchange buffer
+// [N2][N1*G + G] (real with toep_split(KIND), else complex) followed by the
+// n twiddles.
+constexpr bool toep_split(int kind) { return kind != TK_ROWF; }
+
+template <typename T, int N1, int N2, int KIND>
+__global__ void __launch_bounds__(256)
+k_toep(const cx<T> *in, cx<T> *out, long long in_so, long long in_sk, long long out_so, long long out_sk,
+       const cx<T> *__restrict__ tw, int Gs, int nchunk, const T *__restrict__ mid, ToepCg<T> cg) {
+  constexpr int n = N1 * N2, H1 = N1 / 2, H2 = N2 / 2;
+  constexpr bool SPLIT = toep_split(KIND);
+  static_assert(N1 % 2 == 0 && N2 % 2 == 0 && N1 >= N2, "even factors, N1 >= N2");
+  extern __shared__ unsigned char smem_raw[];
+  T *lx = reinterpret_cast<T *>(smem_raw);
+  const int G = 1 << Gs, G1 = G - 1;
+  const int rowA = N1 * G + G;
+  cx<T> *stw = reinterpret_cast<cx<T> *>(lx + (SPLIT ? 1 : 2) * N2 * rowA);
+  const int lb = (int)xcd_block(blockIdx.x, gridDim.x);
+  const int o = lb / nchunk, chunk = lb - o * nchunk;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < n; i += blockDim.x) stw[i] = tw[i];
+  const int g = tid & G1, r = tid >> Gs;
+  const bool actA = r < N2;
+  const int plane = chunk * G + g;
+  constexpr int SA = (KIND == TK_ROWI) ? +1 : -1;
+
+  // ---- step A ----
+  cx<T> a[N1];
+  if (actA) {
+    if constexpr (KIND == TK_ROWI) {
+      const cx<T> *src = in + (long long)o * in_so + plane;
+#pragma unroll
+      for (int j1 = 0; j1 < N1; j1++) a[j1] = src[(long long)(r + N2 * j1) * in_sk];
+      dft_reg<T, N1, +1>(a);
+    } else {
+      // positions r + N2 j1 < N for j1 < N1/2; the rest are zero
+      const long long base = (long long)o * in_so + plane;
+      if constexpr (KIND == TK_ROWF) {
+        cx<T> zv[H1];
+#pragma unroll
+        for (int j1 = 0; j1 < H1; j1++) zv[j1] = cg.z[base + (long long)(r + N2 * j1) * in_sk];
+        if (cg.zz_old) {
+          cx<T> pv[H1];
+#pragma unroll
+          for (int j1 = 0; j1 < H1; j1++) pv[j1] = cg.p[base + (long long)(r + N2 * j1) * in_sk];
+          if (cg.alpha) {
+            const T al = (T)cg.alpha[plane];
+            cx<T> fv[H1];
+#pragma unroll
+            for (int j1 = 0; j1 < H1; j1++) fv[j1] = cg.f[base + (long long)(r + N2 * j1) * in_sk];
+#pragma unroll
+            for (int j1 = 0; j1 < H1; j1++) cg.f[base + (long long)(r + N2 * j1) * in_sk] = fv[j1] + al * pv[j1];
+          }
+          const T b = (T)(cg.zz[plane] / cg.zz_old[plane]);
+#pragma unroll
+          for (int j1 = 0; j1 < H1; j1++) a[j1] = b * pv[j1] + zv[j1];
+        } else {
+#pragma unroll
+          for (int j1 = 0; j1 < H1; j1++) a[j1] = zv[j1];
+        }
+#pragma unroll
+        for (int j1 = 0; j1 < H1; j1++) cg.p[base + (long long)(r + N2 * j1) * in_sk] = a[j1];
+      } else {
+        const cx<T> *src = in + base;
+#pragma unroll
+        for (int j1 = 0; j1 < H1; j1++) a[j1] = src[(long long)(r + N2 * j1) * in_sk];
+      }
+      dft_half_in<T, N1, -1>(a);
+    }
+  }
+  __syncthreads();  // twiddle table complete
+  if (actA) {
+#pragma unroll
+    for (int k1 = 0; k1 < N1; k1++) {
+      cx<T> w = stw[r * k1];
+      if (SA > 0) w = conjx(w);
+      a[k1] = a[k1] * w;
+    }
+  }
+  // ---- step B (thread r = k1 < N1: every thread) ----
+  cx<T> b[N2];
+  xchg<T, N1, N2, SPLIT>(lx, a, actA, r * rowA + g, G, b, true, r * G + g, rowA);
+  if constexpr (KIND == TK_ROWF) {
+    dft_reg<T, N2, -1>(b);
+    cx<T> *dst = out + (long long)o * out_so + plane;
+#pragma unroll
+    for (int k2 = 0; k2 < N2; k2++) dst[(long long)(r + N1 * k2) * out_sk] = b[k2];
+  } else if constexpr (KIND == TK_ROWI) {
+    // outputs r + N1 k2 < N for k2 < N2/2
+    dft_half_out<T, N2, +1>(b);
+    cx<T> *dst = out + (long long)o * out_so + plane;
+    const cx<T> *pp = cg.p + (long long)o * out_so + plane;
+    cx<T> pv[H2];
+#pragma unroll
+    for (int k2 = 0; k2 < H2; k2++) pv[k2] = pp[(long long)(r + N1 * k2) * out_sk];
+    double s = 0;
+#pragma unroll
+    for (int k2 = 0; k2 < H2; k2++) {
+      dst[(long long)(r + N1 * k2) * out_sk] = b[k2];
+      s += (double)pv[k2].x * b[k2].x + (double)pv[k2].y * b[k2].y;
+    }
+    // fixed-order sum over the N1 threads of each plane
+    __syncthreads();  // exchange buffer free
+    double *sh = reinterpret_cast<double *>(smem_raw);
+    sh[r * G + g] = s;
+    __syncthreads();
+    if (r == 0) {
+      double t = 0;
+      for (int rr = 0; rr < N1; rr++) t += sh[rr * G + g];
+      cg.part[(long long)o * ((long long)nchunk << Gs) + plane] = t;
+    }
+  } else {
+    // forward N2-point DFT, real product, inverse N2-point DFT, twiddle, exchange
+    dft_reg<T, N2, -1>(b);
+    const T *mrow = mid + (long long)o * n;
+    T mv[N2];
+#pragma unroll
+    for (int k2 = 0; k2 < N2; k2++) mv[k2] = mrow[r + N1 * k2];
+#pragma unroll
+    for (int k2 = 0; k2 < N2; k2++) b[k2] = mv[k2] * b[k2];
+    dft_reg<T, N2, +1>(b);
+#pragma unroll
+    for (int m2 = 0; m2 < N2; m2++) b[m2] = b[m2] * conjx(stw[r * m2]);
+    __syncthreads();  // every thread has read the exchange buffer
+    // thread r = m2 gathers k1 = 0..N1-1; outputs r + N2 m1 < N for m1 < N1/2
+    xchg<T, N2, N1, SPLIT>(lx, b, true, r * G + g, rowA, a, actA, r * rowA + g, G);
+    if (actA) {
+      dft_half_out<T, N1, +1>(a);
+      cx<T> *dst = out + (long long)o * out_so + plane;
+#pragma unroll
+      for (int m1 = 0; m1 < H1; m1++) dst[(long long)(r + N2 * m1) * out_sk] = a[m1];
+    }
+  }
+}
+
 }  // namespace nudim

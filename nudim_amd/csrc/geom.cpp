@@ -1,8 +1,11 @@
 // geom.cpp -- host-side set-up tables (see geom.hpp).
 #include "geom.hpp"
 
+#include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdlib>
+#include <stdexcept>
 
 namespace nudim {
 
@@ -167,6 +170,70 @@ void build_nfft_tables(int N, int n, int W, const double *x, int M, bool with_ga
         t->ent[fill[c]++] = (uint32_t)m | ((uint32_t)a << 20) | ((uint32_t)b << 26);
       }
     }
+}
+
+void build_gather_tiles(const NfftTables &t, int ts, int k, GatherTiles *g) {
+  const int n = t.n, W = t.W, M = t.M;
+  if (n % ts != 0) throw std::runtime_error("gather tiles: grid size not a multiple of the tile size");
+  if (k < 1 || k > 65536) throw std::runtime_error("gather tiles: chunk size out of range");
+  const int nt = n / ts, ntile = nt * nt, tt = ts * ts;
+  g->ts = ts; g->k = k; g->ntile = ntile;
+  // tile -> nodes meeting it, ascending (nodes are visited in order, and a
+  // node is appended to a tile at most once)
+  std::vector<std::vector<int>> tn(ntile);
+  std::vector<int> mark(ntile, -1);
+  for (int m = 0; m < M; m++)
+    for (int a = 0; a < W; a++) {
+      int ty = pmod(t.c0[m] + a, n) / ts;
+      for (int b = 0; b < W; b++) {
+        int tl = ty * nt + pmod(t.c1[m] + b, n) / ts;
+        if (mark[tl] != m) { mark[tl] = m; tn[tl].push_back(m); }
+      }
+    }
+  g->tile_chunk.assign(ntile + 1, 0);
+  g->chunk_node.assign(1, 0);
+  g->node.clear();
+  for (int tl = 0; tl < ntile; tl++) {
+    const std::vector<int> &v = tn[tl];
+    int nch = ((int)v.size() + k - 1) / k;
+    for (int c = 0; c < nch; c++) {
+      int a = c * k, b = std::min((int)v.size(), a + k);
+      g->node.insert(g->node.end(), v.begin() + a, v.begin() + b);
+      g->chunk_node.push_back((int)g->node.size());
+    }
+    g->tile_chunk[tl + 1] = g->tile_chunk[tl] + nch;
+  }
+  const int nchunk = g->tile_chunk[ntile];
+  g->chunk_cell.assign((size_t)nchunk * (tt + 1), 0);
+  g->slot.clear();
+  g->weight.clear();
+  g->max_chunk_entries = 0;
+  // per chunk and cell: the cell's taps (already in node order in the
+  // per-cell map) whose node lies in the chunk
+  for (int tl = 0; tl < ntile; tl++) {
+    const int ty = tl / nt, tx = tl % nt;
+    const int cb = g->tile_chunk[tl], ce = g->tile_chunk[tl + 1];
+    for (int c = cb; c < ce; c++) {
+      const int *nb = g->node.data() + g->chunk_node[c], *ne = g->node.data() + g->chunk_node[c + 1];
+      const size_t e_start = g->slot.size();
+      int *cc = g->chunk_cell.data() + (size_t)c * (tt + 1);
+      for (int cl = 0; cl < tt; cl++) {
+        cc[cl] = (int)g->slot.size();
+        size_t cell = (size_t)(ty * ts + cl / ts) * n + (size_t)(tx * ts + cl % ts);
+        for (int e = t.cell_ptr[cell]; e < t.cell_ptr[cell + 1]; e++) {
+          uint32_t u = t.ent[e];
+          int m = (int)(u & 0xFFFFFu), t0 = (int)((u >> 20) & 63u), t1 = (int)(u >> 26);
+          if (m < *nb || m > *(ne - 1)) continue;
+          const int *it = std::lower_bound(nb, ne, m);
+          g->slot.push_back((uint16_t)(it - nb));
+          g->weight.push_back(t.psi0[(size_t)m * W + t0] * t.psi1[(size_t)m * W + t1]);
+        }
+      }
+      cc[tt] = (int)g->slot.size();
+      g->max_chunk_entries = std::max(g->max_chunk_entries, (int)(g->slot.size() - e_start));
+    }
+  }
+  if (g->slot.size() > (size_t)INT32_MAX) throw std::runtime_error("gather tiles: too many entries");
 }
 
 int next_pow2(int v) {

@@ -49,6 +49,24 @@ struct NfftTables {
 void build_nfft_tables(int N, int n, int W, const double *x, int M, bool with_gather,
                        NfftTables *t);
 
+// Tiled form of the gather inverse map.  The n x n grid is cut into tiles of
+// ts x ts cells (n % ts == 0), tile (ty, tx) = ty * (n/ts) + tx.  The nodes
+// whose W x W footprint meets a tile, in increasing node order, are cut into
+// chunks of at most k nodes; a chunk's node values are what one block stages
+// in LDS.  Within a chunk the entries are grouped by cell (row-major within
+// the tile) and, per cell, kept in node order, so that summing chunk after
+// chunk visits each cell's taps in the same order as the per-cell map.
+struct GatherTiles {
+  int ts = 0, k = 0, ntile = 0, max_chunk_entries = 0;
+  std::vector<int> tile_chunk;      // [ntile+1]: chunk ids of each tile
+  std::vector<int> chunk_node;      // [nchunk+1]: range of each chunk in node
+  std::vector<int> node;            // node ids
+  std::vector<int> chunk_cell;      // [nchunk][ts*ts+1]: absolute entry ranges per cell
+  std::vector<uint16_t> slot;       // entry -> node slot within its chunk
+  std::vector<double> weight;       // entry -> psi0[node][t0] * psi1[node][t1]
+};
+void build_gather_tiles(const NfftTables &t, int ts, int k, GatherTiles *g);
+
 // smallest even 2^a 3^b 5^c >= v
 int nice_fft_size(int v);
 int next_pow2(int v);

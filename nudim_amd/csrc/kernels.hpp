@@ -13,6 +13,7 @@
 namespace nudim {
 
 constexpr int kThreads = 256;
+constexpr int kGatherTile = 8;  // cells per tile edge of the tiled gather
 constexpr int kRedBlocks = 240;  // partial-sum rows of the per-plane reductions
 
 // ---------------------------------------------------------------------------
@@ -134,6 +135,74 @@ k_gather_w(const cx<T> *__restrict__ s, cx<T> *__restrict__ g, const int *__rest
       if (e + j < e1) acc = acc + (T)(w0[j] * w1[j]) * sv[j];
   }
   g[(size_t)cell * P + p] = acc;
+}
+
+__device__ __forceinline__ float lane_bcast(float v, int j) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+__device__ __forceinline__ double lane_bcast(double v, int j) {
+  long long b = __double_as_longlong(v);
+  int lo = __builtin_amdgcn_readlane((int)b, j), hi = __builtin_amdgcn_readlane((int)(b >> 32), j);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Tiled gather (same sums, same order as k_gather_w): one block per TS x TS
+// tile of cells and 64 planes.  Lane = plane; wave w owns the tile's cells
+// w*CPW .. w*CPW+CPW-1 (row-major in the tile).  For each node chunk of the
+// tile (at most 64 nodes) the block stages the chunk's node values in LDS,
+// [slot][64 planes], so each node value is read from memory once per tile
+// instead of once per tap; then each wave walks its cells' entries, 64 at a
+// time in registers (slot, weight) and broadcast lane by lane.
+template <typename T, int TS>
+__global__ void __launch_bounds__(kThreads)
+k_gather_tile(const cx<T> *__restrict__ s, cx<T> *__restrict__ g, const int *__restrict__ tile_chunk,
+              const int *__restrict__ chunk_node, const int *__restrict__ node, const int *__restrict__ chunk_cell,
+              const uint16_t *__restrict__ slot, const T *__restrict__ wt, int n, int P) {
+  static_assert(kThreads == 256, "four wavefronts per block");
+  constexpr int TT = TS * TS, CPW = TT / 4;
+  extern __shared__ unsigned char smem_raw[];
+  cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
+  const int nt = n / TS;
+  const int tile = (int)xcd_block(blockIdx.x, gridDim.x);
+  const int ty = tile / nt, tx = tile - ty * nt;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int pbase = blockIdx.y * 64;
+  const cx<T> *sp = s + pbase + lane;
+  cx<T> acc[CPW];
+#pragma unroll
+  for (int k = 0; k < CPW; k++) acc[k] = mk((T)0, (T)0);
+  const int cb = tile_chunk[tile], ce = tile_chunk[tile + 1];
+  for (int c = cb; c < ce; c++) {
+    const int a = chunk_node[c], nn = chunk_node[c + 1] - a;  // 1 <= nn <= 64
+    const int myn = node[a + min(lane, nn - 1)];
+#pragma unroll 4
+    for (int i = wv; i < nn; i += 4) lds[i * 64 + lane] = sp[(size_t)__builtin_amdgcn_readlane(myn, i) * P];
+    __syncthreads();
+    const int *cc = chunk_cell + (size_t)c * (TT + 1) + wv * CPW;
+    const int E0 = cc[0], E1 = cc[CPW];
+    for (int base = E0; base < E1; base += 64) {
+      const int ei = min(base + lane, E1 - 1);
+      const int sl = (int)slot[ei];
+      const T w = wt[ei];
+#pragma unroll
+      for (int k = 0; k < CPW; k++) {
+        const int ka = max(cc[k], base), kb = min(cc[k + 1], base + 64);
+        for (int e = ka; e < kb; e++) {
+          const int j = e - base;
+          const int si = __builtin_amdgcn_readlane(sl, j);
+          acc[k] = acc[k] + lane_bcast(w, j) * lds[si * 64 + lane];
+        }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int k = 0; k < CPW; k++) {
+    const int cl = wv * CPW + k;
+    const size_t cell = (size_t)(ty * TS + cl / TS) * n + (size_t)(tx * TS + cl % TS);
+    g[cell * P + pbase + lane] = acc[k];
+  }
 }
 
 // g[cell][p] = sum over the cell's taps (node, t0, t1) of
@@ -265,6 +334,39 @@ k_cg_update(cx<T> *__restrict__ f, const cx<T> *__restrict__ pv, cx<T> *__restri
     __syncthreads();
   }
   if (lr == 0) part[(size_t)blockIdx.x * P + p] = sh[pl];
+}
+
+// Toeplitz CGNR step without the f update (which the next row pass does):
+// alpha_p = zz[p] / pq[p] (kept in alpha_out);  z -= alpha q;  FP64 partial
+// sums of |z|^2 per (row block, plane), blocks as in k_cg_update.
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_cg_zupdate(cx<T> *__restrict__ z, const cx<T> *__restrict__ q, const double *__restrict__ zz,
+             const double *__restrict__ pq, double *__restrict__ alpha_out, long long rows, int P, int PCs,
+             double *__restrict__ part) {
+  __shared__ double sh[kThreads];
+  const int PC = 1 << PCs, lanes = kThreads >> PCs;
+  const int pl = threadIdx.x & (PC - 1), lr = threadIdx.x >> PCs;
+  const int p = blockIdx.y * PC + pl;
+  const double alpha = zz[p] / pq[p];
+  const T na = (T)(-alpha);
+  double s = 0;
+  for (long long k = (long long)blockIdx.x * lanes + lr; k < rows; k += (long long)gridDim.x * lanes) {
+    const long long i = k * P + p;
+    cx<T> zn = z[i] + na * q[i];
+    z[i] = zn;
+    s += (double)zn.x * zn.x + (double)zn.y * zn.y;
+  }
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = lanes >> 1; h > 0; h >>= 1) {
+    if (lr < h) sh[threadIdx.x] += sh[threadIdx.x + h * PC];
+    __syncthreads();
+  }
+  if (lr == 0) {
+    part[(size_t)blockIdx.x * P + p] = sh[pl];
+    if (blockIdx.x == 0) alpha_out[p] = alpha;
+  }
 }
 
 // p = beta p + z with beta_p = zz_new[p] / zz_old[p]

@@ -160,6 +160,7 @@ void LineFft<T>::run(const cx<T> *in, cx<T> *out, int nlines, const LineIO &li, 
 template <typename T>
 size_t NfftOp<T>::bytes() const {
   return fft.tw.bytes + c0.bytes + c1.bytes + psi0.bytes + psi1.bytes + dec.bytes + cell_ptr.bytes + ent.bytes +
+         tile_chunk.bytes + chunk_node.bytes + tnode.bytes + chunk_cell.bytes + tslot.bytes + tweight.bytes +
          X.bytes + grid.bytes;
 }
 
@@ -177,8 +178,22 @@ void NfftOp<T>::init(int N_, int n_, int W_, const double *x, int M_, int P_) {
   upload(psi0, t.psi0);
   upload(psi1, t.psi1);
   upload(dec, t.dec);
-  upload(cell_ptr, t.cell_ptr);
-  upload(ent, t.ent);
+  tiled = (n % kGatherTile == 0) && (P % 64 == 0);
+  if (tiled) {
+    // node values of a chunk staged as [slot][64 planes]: 32 KB of LDS
+    GatherTiles gt;
+    build_gather_tiles(t, kGatherTile, (int)(32768 / (64 * sizeof(cx<T>))), &gt);
+    upload(tile_chunk, gt.tile_chunk);
+    upload(chunk_node, gt.chunk_node);
+    upload(tnode, gt.node);
+    upload(chunk_cell, gt.chunk_cell);
+    upload(tslot, gt.slot);
+    std::vector<T> wv(gt.weight.begin(), gt.weight.end());
+    upload(tweight, wv);
+  } else {
+    upload(cell_ptr, t.cell_ptr);
+    upload(ent, t.ent);
+  }
   X.alloc((size_t)N * n * P * sizeof(cx<T>));
   grid.alloc((size_t)n * n * P * sizeof(cx<T>));
 }
@@ -239,7 +254,13 @@ void NfftOp<T>::adjoint(const cx<T> *f, cx<T> *fhat, hipStream_t st) {
   int PCs = plane_group_log2(P, g_gather_lanes);
   int ncell = n * n;
   dim3 gr(cdiv(ncell, kThreads >> PCs), P >> PCs);
-  {
+  if (tiled) {
+    const int nt = n / kGatherTile;
+    ProfScope ps(KC_GATHER, st, ((double)M * sizeof(cx<T>) + 4.0 * M + (double)ncell * sizeof(cx<T>)) * P);
+    hipLaunchKernelGGL((k_gather_tile<T, kGatherTile>), dim3(nt * nt, P / 64), dim3(kThreads),
+                       32768, st, f, grid.as<cx<T>>(), tile_chunk.as<int>(), chunk_node.as<int>(),
+                       tnode.as<int>(), chunk_cell.as<int>(), tslot.as<uint16_t>(), tweight.as<T>(), n, P);
+  } else {
   // algorithmic bytes: node values read, node ids of the inverse map, grid written
   ProfScope ps(KC_GATHER, st, ((double)M * sizeof(cx<T>) + 4.0 * M + (double)ncell * sizeof(cx<T>)) * P);
   cx<T> *gp = grid.as<cx<T>>();
@@ -291,17 +312,19 @@ __global__ void k_toeplitz_embed(const cx<double> *__restrict__ adj, cx<double> 
 }
 
 template <typename T>
-__global__ void k_toeplitz_finish(const cx<double> *__restrict__ spec, cx<T> *__restrict__ khat, int L, double scale) {
+__global__ void k_toeplitz_finish(const cx<double> *__restrict__ spec, cx<T> *__restrict__ khat, T *__restrict__ khat_re,
+                                  int L, double scale) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (long long)L * L) return;
   int r = (int)(i / L), c = (int)(i % L);
   cx<double> v = spec[i];
   khat[(long long)c * L + r] = mk((T)(v.x * scale), (T)(v.y * scale));
+  khat_re[(long long)c * L + r] = (T)(v.x * scale);
 }
 
 template <typename T>
 size_t ToeplitzOp<T>::bytes() const {
-  return fft.tw.bytes + khat.bytes + X.bytes;
+  return fft.tw.bytes + khat.bytes + khat_re.bytes + X.bytes;
 }
 
 template <typename T>
@@ -334,8 +357,9 @@ void ToeplitzOp<T>::init(int N_, const double *x, const double *w, int M, int P_
   LineIO ci{1, L, L, 0, nullptr, nullptr};
   f64.run(tmp.as<cx<double>>(), emb.as<cx<double>>(), L, ci, ci, -1, 1, st);
   khat.alloc((size_t)L * L * sizeof(cx<T>));
+  khat_re.alloc((size_t)L * L * sizeof(T));
   hipLaunchKernelGGL(k_toeplitz_finish<T>, dim3(cdiv((size_t)L * L, 256)), dim3(256), 0, st, emb.as<cx<double>>(),
-                     khat.as<cx<T>>(), L, 1.0 / ((double)L * (double)L));
+                     khat.as<cx<T>>(), khat_re.as<T>(), L, 1.0 / ((double)L * (double)L));
   NUDIM_HIP(hipGetLastError());
   NUDIM_HIP(hipStreamSynchronize(st));
   X.alloc((size_t)N * L * P * sizeof(cx<T>));
@@ -352,6 +376,26 @@ void ToeplitzOp<T>::apply(const cx<T> *p, cx<T> *q, hipStream_t st) {
   LineIO bi{LP, P, L, 0, nullptr, nullptr};
   LineIO bo{NP, P, N, 0, nullptr, nullptr};
   fft.run(X.as<cx<T>>(), q, N, bi, bo, +1, P, st);
+}
+
+template <typename T>
+void ToeplitzOp<T>::apply_cg(const ToepCg<T> &cg, cx<T> *q, hipStream_t st) {
+  const long long NP = (long long)N * P, LP = (long long)L * P;
+  cx<T> *x = X.as<cx<T>>();
+  const cx<T> *tw = fft.tw.template as<cx<T>>();
+  {
+    ProfScope ps(KC_LINE_FFT, st, (3.0 * N + 2.0 * L) * N * P * sizeof(cx<T>));
+    toep_launch<T>(TK_ROWF, L, N, nullptr, x, NP, P, LP, P, P, st, tw, nullptr, cg);
+  }
+  {
+    ProfScope ps(KC_TOEPLITZ_COL, st, 2.0 * N * L * P * sizeof(cx<T>) + (double)L * L * sizeof(T));
+    toep_launch<T>(TK_COL, L, L, x, x, P, LP, P, LP, P, st, tw, khat_re.template as<T>(), cg);
+  }
+  {
+    ProfScope ps(KC_LINE_FFT, st, (2.0 * N + L) * N * P * sizeof(cx<T>));
+    toep_launch<T>(TK_ROWI, L, N, x, q, LP, P, NP, P, P, st, tw, nullptr, cg);
+  }
+  NUDIM_HIP(hipGetLastError());
 }
 
 // ---------------------------------------------------------------------------
@@ -403,7 +447,7 @@ void Reducer::rfactor(const cx<T> *f, const cx<T> *meas, const int *kind, long l
 template <typename T>
 size_t Cgnr<T>::bytes() const {
   return z.bytes + pv.bytes + q.bytes + r.bytes + v.bytes + t.bytes + dot_z.bytes + coef.bytes + sc.bytes +
-         red.part.bytes + red.part2.bytes;
+         zz2.bytes + alpha.bytes + rowpart.bytes + red.part.bytes + red.part2.bytes;
 }
 
 template <typename T>
@@ -412,7 +456,14 @@ void Cgnr<T>::init(int N_, int M_, int P_, int form_, NfftOp<T> *op_, ToeplitzOp
   size_t NN = (size_t)N * N * P * sizeof(cx<T>), MM = (size_t)M * P * sizeof(cx<T>);
   z.alloc(NN);
   pv.alloc(NN);
-  if (form == NUDIM_CG_TOEPLITZ) q.alloc(NN);
+  if (form == NUDIM_CG_TOEPLITZ) {
+    q.alloc(NN);
+    if (toep->fused()) {
+      zz2.alloc(P * sizeof(double));
+      alpha.alloc(P * sizeof(double));
+      rowpart.alloc((size_t)N * P * sizeof(double));
+    }
+  }
   r.alloc(MM);
   v.alloc(MM);
   t.alloc(MM);
@@ -435,6 +486,33 @@ void Cgnr<T>::solve(const cx<T> *y, cx<T> *fhat, int nsteps, bool f0_zero, hipSt
   op->adjoint(tp, zp, st);
   red.dot<T>(zp, nullptr, (long long)N * N, s, st);
   hipLaunchKernelGGL(k_cg_scalars, dim3(cdiv(P, 64)), dim3(64), 0, st, dz, cf, s, P, 0);
+  NUDIM_HIP(hipGetLastError());
+  if (form == NUDIM_CG_TOEPLITZ && toep->fused()) {
+    // step k: row pass forms p_k = z_k + beta_k p_{k-1} (p_0 = z_0) and
+    // applies the previous step's f += alpha p_{k-1}; q = T p_k with the
+    // row sums of <p_k, q>; alpha_k = |z_k|^2 / <p_k, q>; z -= alpha_k q;
+    // |z_{k+1}|^2.  The last step's f update follows the loop.
+    cx<T> *qp = q.as<cx<T>>();
+    double *zz = dz, *zz_old = zz2.as<double>(), *pq = s, *al = alpha.as<double>();
+    const int PCs = plane_group_log2(P);
+    const dim3 rg(kRedBlocks, P >> PCs);
+    for (int it = 0; it < nsteps; it++) {
+      ToepCg<T> cg{zp, pp, fhat, zz, it ? zz_old : nullptr, it ? al : nullptr, rowpart.as<double>()};
+      toep->apply_cg(cg, qp, st);
+      hipLaunchKernelGGL(k_sum_partials, dim3(cdiv(P, 8)), dim3(256), 0, st, rowpart.as<double>(), N, P, pq);
+      hipLaunchKernelGGL(k_cg_zupdate<T>, rg, dim3(kThreads), 0, st, zp, qp, zz, pq, al, (long long)N * N, P, PCs,
+                         red.part.as<double>());
+      hipLaunchKernelGGL(k_sum_partials, dim3(cdiv(P, 8)), dim3(256), 0, st, red.part.as<double>(), kRedBlocks, P,
+                         zz_old);
+      NUDIM_HIP(hipGetLastError());
+      std::swap(zz, zz_old);
+    }
+    if (nsteps > 0) {
+      hipLaunchKernelGGL(k_axpy_planes<T>, dim3(bN), dim3(256), 0, st, fhat, pp, al, 1.0, NN, P);
+      NUDIM_HIP(hipGetLastError());
+    }
+    return;
+  }
   hipLaunchKernelGGL(k_copy_cx<T>, dim3(bN), dim3(256), 0, st, pp, zp, NN);
   NUDIM_HIP(hipGetLastError());
   if (form == NUDIM_CG_TOEPLITZ) {

@@ -60,6 +60,27 @@ def test_nfft_fp32(hip, orc, polar32, n, W, tol):
         assert rel(a[p], orc.ndft_adjoint(polar32, g[p], N)) < tol
 
 
+@pytest.mark.parametrize("fp64,n,W,tol", [(True, 128, 14, 1e-12), (False, 128, 8, 2e-6), (False, 96, 6, 1e-4)])
+def test_nfft_adjoint_tiled(hip, orc, fp64, n, W, tol):
+    """64 planes on a grid whose edge is a multiple of 8 take the LDS-tiled
+    gather (the dense centre tiles need several node chunks); 63 planes take
+    the per-cell gather.  Both sum each cell's taps in node order, so the
+    shared planes agree bit for bit, and both match the NDFT."""
+    N = 64
+    pmin, pmax = orc.pad_range(-60, 3)
+    x = orc.polar_nodes(pmin, pmax, 1.0, N)
+    M = x.shape[0]
+    rng = np.random.default_rng(7)
+    g = rng.standard_normal((64, M)) + 1j * rng.standard_normal((64, M))
+    prec = hip.FP64_PLAIN if fp64 else hip.FP32
+    a64 = hip.nfft_adjoint(x, g, N, n=n, W=W, precision=prec)
+    a63 = hip.nfft_adjoint(x, g[:63], N, n=n, W=W, precision=prec)
+    assert np.array_equal(a64[:63], a63)
+    # tol: FP64 rounding, FP32 rounding, window error of sigma 1.5 and W 6 (5.3e-5 in FP64)
+    for p in (0, 37, 63):
+        assert rel(a64[p], orc.ndft_adjoint(x, g[p], N)) < tol
+
+
 @pytest.mark.parametrize("weighted", [True, False])
 def test_cgnr(hip, orc, polar32, weighted):
     rng = np.random.default_rng(3)
@@ -240,12 +261,22 @@ def test_end_to_end_small(hip, orc, denoise, method, height):
         assert rel(v4_32, img_ref) < 4 * rel(f32, img_ref) + 1e-4
 
 
-@pytest.mark.parametrize("denoise,method", [(0, 1), (1, 0)])
-def test_end_to_end_er_fp32(hip, orc, denoise, method):
-    """ER passes only (continuous): FP32 form within 1e-4 relative L2."""
-    np_ = nz = 32
+@pytest.mark.parametrize("denoise,method,np_", [(0, 1, 32), (1, 0, 32), (0, 1, 64), (1, 0, 64)])
+def test_end_to_end_er_fp32(hip, orc, denoise, method, np_):
+    """ER passes only (continuous): FP32 form within 1e-4 relative L2.  At
+    np = nz = 64 the Toeplitz CGNR runs on the fused fixed-length passes
+    (2N = 128) and the adjoint on the tiled gather (64 planes).
+
+    With DIFFUSION at 64^3 the denoiser amplifies the ~2e-6 per-pass FP32
+    rounding with a heavy tail: the oracle fed relative perturbations of 2e-6
+    before each pass ends 1.9e-5 .. 5.5e-5 away from itself over six seeds,
+    and the FP32 form (either FFT path) ends 2.3e-5 .. 2.1e-4 away over four
+    phantoms, so that case is held to 5e-4 (a wrong operator is off by 1e-2
+    or more)."""
+    nz = np_
     proj = small_case(orc, np_, nz, seed=12)
-    kw = dict(padangle=1.0, weight=1, denoise=denoise, method=method, nhio=0, ner=3, height=16, center=16)
+    kw = dict(padangle=1.0, weight=1, denoise=denoise, method=method, nhio=0, ner=3, height=np_ // 2,
+              center=np_ // 2)
     op = orc.params(np_, nz, np_, -60, 60, 5, **kw)
     img0_ref, img_ref, r_ref = orc.reconstruct(op, proj)
     with hip.Reconstruction(hip.make_params(np_, nz, np_, -60, 60, 5, precision=hip.FP32, **kw)) as job:
@@ -253,5 +284,5 @@ def test_end_to_end_er_fp32(hip, orc, denoise, method):
         job.initial()
         r = job.iterate(3)
         v = job.volume()
-    assert rel(v, img_ref) < 1e-4
+    assert rel(v, img_ref) < (5e-4 if (denoise and np_ == 64) else 1e-4)
     assert np.max(np.abs(r - r_ref)) < 1e-5
