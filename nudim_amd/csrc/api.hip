@@ -252,8 +252,8 @@ struct Job : JobBase {
     }
     d1.alloc(P * 8);
     d2.alloc(P * 8);
-    tvpart_g.alloc((size_t)N * 64 * 8);
-    tvpart_a.alloc((size_t)N * 64 * 8);
+    tvpart_g.alloc((size_t)tv_blocks(N, N, P) * 8);
+    tvpart_a.alloc((size_t)tv_blocks(N, N, P) * 8);
     tvsums.alloc(2 * 8);
     fdz = make_fft_desc(P);
     std::vector<double> t;
@@ -336,7 +336,7 @@ struct Job : JobBase {
     if (yhi < ylo) return;
     // total_variation_discrete(_height)(img, 0.2f, 40, ...), reconstruction.c:630-632
     T *r = tv_sweeps<T>(u, spare, grad.as<T>(), grad2.as<T>(), N, N, P, ylo, yhi, 0.2f, 40, tvpart_g.as<double>(),
-                        tvpart_a.as<double>(), tvsums.as<double>(), N * 64, st);
+                        tvpart_a.as<double>(), tvsums.as<double>(), tv_blocks(N, N, P), st);
     if (r != img) std::swap(img, spare);
   }
 
@@ -605,11 +605,11 @@ extern "C" int nudim_estimate_bytes(const nudim_params *p, size_t *bytes) {
   size_t NN = N * N * P, MM = M * P;
   size_t b = 0;
   b += (size_t)N * n * P * cb + (size_t)n * n * P * cb;           // NFFT X, grid
-  b += (size_t)M * W * W * 4 + (size_t)n * n * 4 + M * W * 16;      // gather map, window tables
-  if (form == NUDIM_CG_TOEPLITZ) b += (size_t)4 * N * N * cb + (size_t)N * 2 * N * P * cb + NN * cb;
+  b += (size_t)M * W * W * (4 + rb) + (size_t)n * n * 4 + M * W * 16;  // gather maps, window tables
+  if (form == NUDIM_CG_TOEPLITZ) b += (size_t)4 * N * N * (cb + rb) + (size_t)N * 2 * N * P * cb + NN * cb;
   b += 2 * NN * cb + 3 * MM * cb;                                   // CG z, p; r, v, t
   b += 2 * MM * cb + 2 * NN * cb;                                   // meas, y, Fh, F0
-  b += 2 * NN * rb + ((p->denoise && p->method == NUDIM_TV) ? NN * rb : 0);
+  b += 2 * NN * rb + ((p->denoise && p->method == NUDIM_TV) ? 3 * NN * rb : 0);  // volumes, TV gradients
   *bytes = b;
   return NUDIM_OK;
   API_END
@@ -754,7 +754,7 @@ extern "C" int nudim_zfft(int precision, int len, int howmany, double *data, int
 template <typename T>
 struct VolTmp {
   TmpStream ts;
-  DevBuf u, grad, part_g, part_a, sums;
+  DevBuf u, part_g, part_a, sums;
   int nz, ny, nx;
   long long V;
   VolTmp(const double *h, int nz_, int ny_, int nx_) : nz(nz_), ny(ny_), nx(nx_) {
@@ -779,15 +779,15 @@ struct VolTmp {
     ylo = std::max(ylo, 0);
     yhi = std::min(yhi, ny - 1);
     if (yhi < ylo) return;
-    grad.alloc(V * sizeof(T));
-    DevBuf u2(V * sizeof(T)), g2(V * sizeof(T));
-    NUDIM_HIP(hipMemsetAsync(grad.p, 0, V * sizeof(T), ts.s));
-    NUDIM_HIP(hipMemsetAsync(g2.p, 0, V * sizeof(T), ts.s));
-    const int nb = ny * 64;
+    DevBuf u2(V * sizeof(T));
+    const int nb = tv_blocks(ny, nx, nz);
     part_g.alloc(nb * 8);
     part_a.alloc(nb * 8);
     sums.alloc(16);
-    T *r = tv_sweeps<T>(u.as<T>(), u2.as<T>(), grad.as<T>(), g2.as<T>(), ny, nx, nz, ylo, yhi, lam, niter,
+    DevBuf g0(V * sizeof(T)), g1(V * sizeof(T));
+    NUDIM_HIP(hipMemsetAsync(g0.p, 0, V * sizeof(T), ts.s));
+    NUDIM_HIP(hipMemsetAsync(g1.p, 0, V * sizeof(T), ts.s));
+    T *r = tv_sweeps<T>(u.as<T>(), u2.as<T>(), g0.as<T>(), g1.as<T>(), ny, nx, nz, ylo, yhi, lam, niter,
                         part_g.as<double>(), part_a.as<double>(), sums.as<double>(), nb, ts.s);
     if (r != u.as<T>()) std::swap(u, u2);
     NUDIM_HIP(hipStreamSynchronize(ts.s));

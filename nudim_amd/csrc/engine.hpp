@@ -16,12 +16,13 @@ bool fft2_launch(int n, const cx<T> *in, cx<T> *out, int nlines, const LineIO &l
 // process-wide switch for the fixed-length kernels (on by default)
 extern bool g_use_fft2;
 // Toeplitz passes with the CGNR vector work (fft2.hip) for n = 2N in
-// toep_supported(n); kind is TK_ROWF, TK_COL or TK_ROWI
+// toep_supported(n); kind is TK_ROWF, TK_COL or TK_ROWI.  P planes from the
+// given (offset) pointers; pstride = row stride of cg.part.
 bool toep_supported(int n);
 template <typename T>
 void toep_launch(int kind, int n, int nlines, const cx<T> *in, cx<T> *out, long long in_so, long long in_sk,
                  long long out_so, long long out_sk, int P, hipStream_t st, const cx<T> *tw, const T *mid,
-                 const ToepCg<T> &cg);
+                 const ToepCg<T> &cg, int pstride);
 
 // choose log2 of the plane chunk G of a line transform of length n over P planes
 int line_chunk_log2(int n, int P, int elem_bytes, int ept);
@@ -74,10 +75,12 @@ struct ToeplitzOp {
 // niter sweeps of discrete TV over rows ylo..yhi (clipped by the caller) on
 // a [ny][nx][nz] volume held in u0; u1, g0, g1 are volume-sized scratch, and
 // g0, g1 must hold zeros outside rows ylo..yhi (clear them once when they are
-// allocated).  Returns the buffer (u0 or u1) that holds the result.
+// allocated).  part_g and part_a hold nb >= tv_blocks(ny, nx, nz) partial
+// sums.  Returns the buffer (u0 or u1) that holds the result.
+int tv_blocks(int ny, int nx, int nz);
 template <typename T>
 T *tv_sweeps(T *u0, T *u1, T *g0, T *g1, int ny, int nx, int nz, int ylo, int yhi, float lam, int niter,
-             double *part_g, double *part_a, double *sums, int nblocks, hipStream_t st);
+             double *part_g, double *part_a, double *sums, int nb, hipStream_t st);
 
 // per-plane reduction helpers
 struct Reducer {

@@ -14,6 +14,7 @@ template <typename T, int N1, int N2>
 bool launch_shape(const cx<T> *in, cx<T> *out, int nlines, const LineIO &li, const LineIO &lo, int sign, int P,
                   hipStream_t st, const cx<T> *tw, const cx<T> *mid, long long mid_so) {
   constexpr int n = N1 * N2;
+  if (mid) return false;  // the pointwise-product form runs on the Toeplitz passes
   int cap = std::max(1, std::min(256, g_fft2_threads) / N1);
   int Gs = 0;
   while ((2 << Gs) <= cap && P % (2 << Gs) == 0) Gs++;
@@ -22,20 +23,28 @@ bool launch_shape(const cx<T> *in, cx<T> *out, int nlines, const LineIO &li, con
   if (lds > 160 * 1024) return false;
   const int nchunk = P >> Gs;
   dim3 grid((unsigned)((long long)nlines * nchunk)), block(N1 * G);
-  if (mid) {
-    hipLaunchKernelGGL((k_fft2<T, N1, N2, -1, 1>), grid, block, lds, st, in, out, li, lo, tw, Gs, nchunk, mid, mid_so);
-  } else if (sign < 0) {
-    hipLaunchKernelGGL((k_fft2<T, N1, N2, -1, 0>), grid, block, lds, st, in, out, li, lo, tw, Gs, nchunk, mid, mid_so);
+  // n/2 elements centred on position 0: the NFFT's N = n/2 modes
+  const bool hin = li.cnt * 2 == n && li.shift == -n / 4;
+  const bool hout = !hin && lo.cnt * 2 == n && lo.shift == -n / 4;
+#define NUDIM_FFT2(SG, HI, HO) \
+  hipLaunchKernelGGL((k_fft2<T, N1, N2, SG, HI, HO>), grid, block, lds, st, in, out, li, lo, tw, Gs, nchunk)
+  if (sign < 0) {
+    if (hin) NUDIM_FFT2(-1, true, false);
+    else if (hout) NUDIM_FFT2(-1, false, true);
+    else NUDIM_FFT2(-1, false, false);
   } else {
-    hipLaunchKernelGGL((k_fft2<T, N1, N2, +1, 0>), grid, block, lds, st, in, out, li, lo, tw, Gs, nchunk, mid, mid_so);
+    if (hin) NUDIM_FFT2(+1, true, false);
+    else if (hout) NUDIM_FFT2(+1, false, true);
+    else NUDIM_FFT2(+1, false, false);
   }
+#undef NUDIM_FFT2
   return true;
 }
 
 template <typename T, int N1, int N2>
 void launch_toep(int kind, int nlines, const cx<T> *in, cx<T> *out, long long in_so, long long in_sk,
                  long long out_so, long long out_sk, int P, hipStream_t st, const cx<T> *tw, const T *mid,
-                 const ToepCg<T> &cg) {
+                 const ToepCg<T> &cg, int pstride) {
   constexpr int n = N1 * N2;
   int cap = std::max(1, std::min(256, g_fft2_threads) / N1);
   int Gs = 0;
@@ -47,15 +56,15 @@ void launch_toep(int kind, int nlines, const cx<T> *in, cx<T> *out, long long in
   switch (kind) {
     case TK_ROWF:
       hipLaunchKernelGGL((k_toep<T, N1, N2, TK_ROWF>), grid, block, lds, st, in, out, in_so, in_sk, out_so, out_sk,
-                         tw, Gs, nchunk, mid, cg);
+                         tw, Gs, nchunk, mid, cg, pstride);
       break;
     case TK_COL:
       hipLaunchKernelGGL((k_toep<T, N1, N2, TK_COL>), grid, block, lds, st, in, out, in_so, in_sk, out_so, out_sk,
-                         tw, Gs, nchunk, mid, cg);
+                         tw, Gs, nchunk, mid, cg, pstride);
       break;
     default:
       hipLaunchKernelGGL((k_toep<T, N1, N2, TK_ROWI>), grid, block, lds, st, in, out, in_so, in_sk, out_so, out_sk,
-                         tw, Gs, nchunk, mid, cg);
+                         tw, Gs, nchunk, mid, cg, pstride);
   }
 }
 
@@ -71,9 +80,10 @@ bool toep_supported(int n) {
 template <typename T>
 void toep_launch(int kind, int n, int nlines, const cx<T> *in, cx<T> *out, long long in_so, long long in_sk,
                  long long out_so, long long out_sk, int P, hipStream_t st, const cx<T> *tw, const T *mid,
-                 const ToepCg<T> &cg) {
+                 const ToepCg<T> &cg, int pstride) {
 #define NUDIM_TOEP(NN, A, B) \
-  case NN: launch_toep<T, A, B>(kind, nlines, in, out, in_so, in_sk, out_so, out_sk, P, st, tw, mid, cg); return;
+  case NN: launch_toep<T, A, B>(kind, nlines, in, out, in_so, in_sk, out_so, out_sk, P, st, tw, mid, cg, pstride); \
+    return;
   switch (n) {
     NUDIM_TOEP(128, 16, 8)
     NUDIM_TOEP(256, 16, 16)
@@ -89,10 +99,10 @@ void toep_launch(int kind, int n, int nlines, const cx<T> *in, cx<T> *out, long 
 
 template void toep_launch<float>(int, int, int, const cx<float> *, cx<float> *, long long, long long, long long,
                                  long long, int, hipStream_t, const cx<float> *, const float *,
-                                 const ToepCg<float> &);
+                                 const ToepCg<float> &, int);
 template void toep_launch<double>(int, int, int, const cx<double> *, cx<double> *, long long, long long, long long,
                                   long long, int, hipStream_t, const cx<double> *, const double *,
-                                  const ToepCg<double> &);
+                                  const ToepCg<double> &, int);
 
 // Returns false when no fixed-length kernel exists for n (the caller then
 // uses the generic Stockham kernel).

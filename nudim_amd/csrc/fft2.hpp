@@ -115,150 +115,7 @@ __device__ __forceinline__ void dft_reg(cx<T> *v) {
   }
 }
 
-// ---- the line kernel ------------------------------------------------------------
-
-// MODE 0: one transform with sign S; MODE 1: forward (S = -1), times
-// mid[o*mid_so + pos], inverse.  Threads: N1 * G (>= N2 * G).  LDS: the
-// exchange buffer [N2][N1*G + pad] followed by the n twiddles w_n^j.
-template <int N1, int N2>
-struct Fft2Shape {
-  static constexpr int n = N1 * N2;
-  static constexpr int big = N1 > N2 ? N1 : N2;
-};
-
-template <typename T, int N1, int N2, int S, int MODE>
-__global__ void __launch_bounds__(256)
-k_fft2(const cx<T> *in, cx<T> *out, LineIO li, LineIO lo, const cx<T> *__restrict__ tw, int Gs, int nchunk,
-       const cx<T> *__restrict__ mid, long long mid_so) {
-  constexpr int n = N1 * N2;
-  extern __shared__ unsigned char smem_raw[];
-  cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
-  const int G = 1 << Gs, G1 = G - 1;
-  // exchange row of one j2 (padded by G so that the step-A writes of a
-  // wavefront spread evenly over the LDS banks)
-  const int rowA = N1 * G + G;
-  cx<T> *stw = lds + N2 * rowA;         // n twiddles
-  const int lb = (int)xcd_block(blockIdx.x, gridDim.x);
-  const int o = lb / nchunk, chunk = lb - o * nchunk;
-  const int tid = threadIdx.x;
-  for (int i = tid; i < n; i += blockDim.x) stw[i] = tw[i];
-  const int g = tid & G1, r = tid >> Gs;  // r = j2 in step A, k1 in step B
-  const bool actA = r < N2, actB = r < N1;
-  const cx<T> *src = in + (long long)o * li.so + (long long)chunk * G + g;
-  const T so_scale = li.scale_o ? (T)li.scale_o[o] : (T)1;
-
-  // ---- step A: load positions j2 + N2 j1, N1-point DFT ----
-  // Loads are unconditional (index clamped into range, out-of-range lanes
-  // zeroed afterwards) so that all N1 of them are in flight together.
-  cx<T> a[N1];
-  if (actA) {
-    int kk[N1];
-#pragma unroll
-    for (int j1 = 0; j1 < N1; j1++) kk[j1] = wrap_pos(r + N2 * j1 - li.shift, n);
-#pragma unroll
-    for (int j1 = 0; j1 < N1; j1++) a[j1] = src[(long long)min(kk[j1], li.cnt - 1) * li.sk];
-    if (li.scale_k) {
-      T sc[N1];
-#pragma unroll
-      for (int j1 = 0; j1 < N1; j1++) sc[j1] = (T)li.scale_k[min(kk[j1], li.cnt - 1)];
-#pragma unroll
-      for (int j1 = 0; j1 < N1; j1++) a[j1] = (sc[j1] * so_scale) * a[j1];
-    } else if (li.scale_o) {
-#pragma unroll
-      for (int j1 = 0; j1 < N1; j1++) a[j1] = so_scale * a[j1];
-    }
-#pragma unroll
-    for (int j1 = 0; j1 < N1; j1++)
-      if (kk[j1] >= li.cnt) a[j1] = mk((T)0, (T)0);
-    dft_reg<T, N1, (MODE == 1 ? -1 : S)>(a);
-  }
-  __syncthreads();  // twiddle table complete
-  if (actA) {
-#pragma unroll
-    for (int k1 = 0; k1 < N1; k1++) {
-      cx<T> w = stw[r * k1];
-      if ((MODE == 1 ? -1 : S) > 0) w = conjx(w);
-      lds[r * rowA + k1 * G + g] = a[k1] * w;
-    }
-  }
-  __syncthreads();
-
-  // ---- step B: N2-point DFT over j2 for fixed k1 ----
-  cx<T> b[N2];
-  if (actB) {
-#pragma unroll
-    for (int j2 = 0; j2 < N2; j2++) b[j2] = lds[j2 * rowA + r * G + g];
-    dft_reg<T, N2, (MODE == 1 ? -1 : S)>(b);
-  }
-  cx<T> *dst = out + (long long)o * lo.so + (long long)chunk * G + g;
-  const T oo_scale = lo.scale_o ? (T)lo.scale_o[o] : (T)1;
-  if constexpr (MODE == 0) {
-    if (actB) {
-      int kk[N2];
-#pragma unroll
-      for (int k2 = 0; k2 < N2; k2++) kk[k2] = wrap_pos(r + N1 * k2 - lo.shift, n);
-      if (lo.scale_k) {
-        T sc[N2];
-#pragma unroll
-        for (int k2 = 0; k2 < N2; k2++) sc[k2] = (T)lo.scale_k[min(kk[k2], lo.cnt - 1)];
-#pragma unroll
-        for (int k2 = 0; k2 < N2; k2++) b[k2] = (sc[k2] * oo_scale) * b[k2];
-      } else if (lo.scale_o) {
-#pragma unroll
-        for (int k2 = 0; k2 < N2; k2++) b[k2] = oo_scale * b[k2];
-      }
-#pragma unroll
-      for (int k2 = 0; k2 < N2; k2++)
-        if (kk[k2] < lo.cnt) dst[(long long)kk[k2] * lo.sk] = b[k2];
-    }
-  } else {
-    // pointwise product and inverse: N2-point inverse DFT over k2 (in
-    // registers), twiddle w_n^{-k1 m2}, exchange, N1-point inverse DFT
-    __syncthreads();  // every thread has read the exchange buffer
-    if (actB) {
-      const cx<T> *mrow = mid + (long long)o * mid_so;
-      cx<T> mv[N2];
-#pragma unroll
-      for (int k2 = 0; k2 < N2; k2++) mv[k2] = mrow[r + N1 * k2];
-#pragma unroll
-      for (int k2 = 0; k2 < N2; k2++) b[k2] = b[k2] * mv[k2];
-      dft_reg<T, N2, +1>(b);
-#pragma unroll
-      for (int m2 = 0; m2 < N2; m2++) lds[m2 * rowA + r * G + g] = b[m2] * conjx(stw[r * m2]);
-    }
-    __syncthreads();
-    if (actA) {
-      // thread j2 := m2 now gathers k1 = 0..N1-1
-#pragma unroll
-      for (int k1 = 0; k1 < N1; k1++) a[k1] = lds[r * rowA + k1 * G + g];
-      dft_reg<T, N1, +1>(a);
-      int kk[N1];
-#pragma unroll
-      for (int m1 = 0; m1 < N1; m1++) kk[m1] = wrap_pos(r + N2 * m1 - lo.shift, n);
-      if (lo.scale_k) {
-        T sc[N1];
-#pragma unroll
-        for (int m1 = 0; m1 < N1; m1++) sc[m1] = (T)lo.scale_k[min(kk[m1], lo.cnt - 1)];
-#pragma unroll
-        for (int m1 = 0; m1 < N1; m1++) a[m1] = (sc[m1] * oo_scale) * a[m1];
-      } else if (lo.scale_o) {
-#pragma unroll
-        for (int m1 = 0; m1 < N1; m1++) a[m1] = oo_scale * a[m1];
-      }
-#pragma unroll
-      for (int m1 = 0; m1 < N1; m1++)
-        if (kk[m1] < lo.cnt) dst[(long long)kk[m1] * lo.sk] = a[m1];
-    }
-  }
-}
-
-// ---- Toeplitz passes ---------------------------------------------------------------
-//
-// The circulant embedding of an N x N plane into n x n, n = 2N, holds the
-// plane in the first N positions of every line and keeps only the first N
-// outputs.  With positions j2 + N2 j1 this makes inputs j1 >= N1/2 zero in
-// step A and leaves only outputs m < R/2 wanted in the last DFT, so those
-// DFTs run at half size.  The row passes also carry the CGNR vector work.
+// ---- pruned DFTs and the LDS exchange ---------------------------------------------
 
 // R-point DFT (sign S) of v whose inputs v[R/2..R) are zero (not read):
 // even outputs = DFT_{R/2}(v), odd outputs = DFT_{R/2}(v[j] w_R^{Sj}).
@@ -338,6 +195,156 @@ __device__ __forceinline__ void xchg(T *lx, const cx<T> *wv, bool wact, int wbas
   }
 }
 
+// v * exp(-S 2 pi i q / 4): a quarter-turn rotation (no flops)
+template <typename T, int S>
+__device__ __forceinline__ cx<T> mul_quarter(cx<T> v, int q) {
+  q &= 3;
+  if (S > 0) q = (4 - q) & 3;
+  switch (q) {
+    case 0: return v;
+    case 1: return mk(-v.y, v.x);
+    case 2: return mk(-v.x, -v.y);
+    default: return mk(v.y, -v.x);
+  }
+}
+
+// ---- the line kernel ------------------------------------------------------------
+
+// One transform of sign S per line.  Threads: N1 * G (>= N2 * G).  LDS: the
+// exchange buffer [N2][N1*G + pad] followed by the n twiddles w_n^j.
+//
+// HIN: the input holds n/2 elements centred on position 0 (cnt = n/2,
+// shift = -n/4), i.e. positions j1 in [3N1/4, N1) and [0, N1/4) of step A.
+// With b[j'] = a[(j' - N1/4) mod N1], X[k] = exp(-S 2 pi i k/4) DFT(b)[k]
+// and b has its upper half zero: a half-size DFT and a quarter turn.
+// HOUT: only the n/2 outputs centred on position 0 are stored, i.e. k2 in
+// [3N2/4, N2) and [0, N2/4) of step B: Y[m' - N2/4] is output m' < N2/2 of
+// the DFT of v[j] exp(-S 2 pi i j/4).
+template <typename T, int N1, int N2, int S, bool HIN, bool HOUT>
+__global__ void __launch_bounds__(256)
+k_fft2(const cx<T> *in, cx<T> *out, LineIO li, LineIO lo, const cx<T> *__restrict__ tw, int Gs, int nchunk) {
+  constexpr int n = N1 * N2, Q1 = N1 / 4, Q2 = N2 / 4;
+  static_assert(!HIN || N1 % 4 == 0, "centred input needs N1 % 4 == 0");
+  static_assert(!HOUT || N2 % 4 == 0, "centred output needs N2 % 4 == 0");
+  extern __shared__ unsigned char smem_raw[];
+  cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
+  const int G = 1 << Gs, G1 = G - 1;
+  // exchange row of one j2 (padded by G so that the step-A writes of a
+  // wavefront spread evenly over the LDS banks)
+  const int rowA = N1 * G + G;
+  cx<T> *stw = lds + N2 * rowA;         // n twiddles
+  const int lb = (int)xcd_block(blockIdx.x, gridDim.x);
+  const int o = lb / nchunk, chunk = lb - o * nchunk;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < n; i += blockDim.x) stw[i] = tw[i];
+  const int g = tid & G1, r = tid >> Gs;  // r = j2 in step A, k1 in step B
+  const bool actA = r < N2, actB = r < N1;
+  const cx<T> *src = in + (long long)o * li.so + (long long)chunk * G + g;
+  const T so_scale = li.scale_o ? (T)li.scale_o[o] : (T)1;
+
+  // ---- step A: load positions j2 + N2 j1, N1-point DFT ----
+  cx<T> a[N1];
+  if (actA) {
+    if constexpr (HIN) {
+      // every loaded position is in range: all N1/2 loads in flight together
+      int kk[N1 / 2];
+#pragma unroll
+      for (int jp = 0; jp < N1 / 2; jp++)
+        kk[jp] = (jp < Q1) ? r + N2 * (jp + 3 * Q1) + n / 4 - n : r + N2 * (jp - Q1) + n / 4;
+#pragma unroll
+      for (int jp = 0; jp < N1 / 2; jp++) a[jp] = src[(long long)kk[jp] * li.sk];
+      if (li.scale_k) {
+        T sc[N1 / 2];
+#pragma unroll
+        for (int jp = 0; jp < N1 / 2; jp++) sc[jp] = (T)li.scale_k[kk[jp]];
+#pragma unroll
+        for (int jp = 0; jp < N1 / 2; jp++) a[jp] = (sc[jp] * so_scale) * a[jp];
+      } else if (li.scale_o) {
+#pragma unroll
+        for (int jp = 0; jp < N1 / 2; jp++) a[jp] = so_scale * a[jp];
+      }
+      dft_half_in<T, N1, S>(a);
+#pragma unroll
+      for (int k = 0; k < N1; k++) a[k] = mul_quarter<T, S>(a[k], k);
+    } else {
+      // Loads are unconditional (index clamped into range, out-of-range lanes
+      // zeroed afterwards) so that all N1 of them are in flight together.
+      int kk[N1];
+#pragma unroll
+      for (int j1 = 0; j1 < N1; j1++) kk[j1] = wrap_pos(r + N2 * j1 - li.shift, n);
+#pragma unroll
+      for (int j1 = 0; j1 < N1; j1++) a[j1] = src[(long long)min(kk[j1], li.cnt - 1) * li.sk];
+      if (li.scale_k) {
+        T sc[N1];
+#pragma unroll
+        for (int j1 = 0; j1 < N1; j1++) sc[j1] = (T)li.scale_k[min(kk[j1], li.cnt - 1)];
+#pragma unroll
+        for (int j1 = 0; j1 < N1; j1++) a[j1] = (sc[j1] * so_scale) * a[j1];
+      } else if (li.scale_o) {
+#pragma unroll
+        for (int j1 = 0; j1 < N1; j1++) a[j1] = so_scale * a[j1];
+      }
+#pragma unroll
+      for (int j1 = 0; j1 < N1; j1++)
+        if (kk[j1] >= li.cnt) a[j1] = mk((T)0, (T)0);
+      dft_reg<T, N1, S>(a);
+    }
+  }
+  __syncthreads();  // twiddle table complete
+  if (actA) {
+#pragma unroll
+    for (int k1 = 0; k1 < N1; k1++) {
+      cx<T> w = stw[r * k1];
+      if (S > 0) w = conjx(w);
+      lds[r * rowA + k1 * G + g] = a[k1] * w;
+    }
+  }
+  __syncthreads();
+
+  // ---- step B: N2-point DFT over j2 for fixed k1 ----
+  if (!actB) return;
+  cx<T> b[N2];
+#pragma unroll
+  for (int j2 = 0; j2 < N2; j2++) b[j2] = lds[j2 * rowA + r * G + g];
+  cx<T> *dst = out + (long long)o * lo.so + (long long)chunk * G + g;
+  const T oo_scale = lo.scale_o ? (T)lo.scale_o[o] : (T)1;
+  constexpr int NO = HOUT ? N2 / 2 : N2;
+  int kk[NO];
+  if constexpr (HOUT) {
+#pragma unroll
+    for (int j = 0; j < N2; j++) b[j] = mul_quarter<T, S>(b[j], j);
+    dft_half_out<T, N2, S>(b);
+#pragma unroll
+    for (int mp = 0; mp < NO; mp++)
+      kk[mp] = (mp < Q2) ? r + N1 * (mp + 3 * Q2) + n / 4 - n : r + N1 * (mp - Q2) + n / 4;
+  } else {
+    dft_reg<T, N2, S>(b);
+#pragma unroll
+    for (int k2 = 0; k2 < NO; k2++) kk[k2] = wrap_pos(r + N1 * k2 - lo.shift, n);
+  }
+  if (lo.scale_k) {
+    T sc[NO];
+#pragma unroll
+    for (int k2 = 0; k2 < NO; k2++) sc[k2] = (T)lo.scale_k[min(kk[k2], lo.cnt - 1)];
+#pragma unroll
+    for (int k2 = 0; k2 < NO; k2++) b[k2] = (sc[k2] * oo_scale) * b[k2];
+  } else if (lo.scale_o) {
+#pragma unroll
+    for (int k2 = 0; k2 < NO; k2++) b[k2] = oo_scale * b[k2];
+  }
+#pragma unroll
+  for (int k2 = 0; k2 < NO; k2++)
+    if (HOUT || kk[k2] < lo.cnt) dst[(long long)kk[k2] * lo.sk] = b[k2];
+}
+
+// ---- Toeplitz passes ---------------------------------------------------------------
+//
+// The circulant embedding of an N x N plane into n x n, n = 2N, holds the
+// plane in the first N positions of every line and keeps only the first N
+// outputs.  With positions j2 + N2 j1 this makes inputs j1 >= N1/2 zero in
+// step A and leaves only outputs m < R/2 wanted in the last DFT, so those
+// DFTs run at half size.  The row passes also carry the CGNR vector work.
+
 // Line o of a pass, plane chunk*G + g; element k of a line at
 // o*so + k*sk (+ plane).  ROWF: o = row, in = p (N positions) -> out = X
 // (n positions).  COL: o = column of X, rows 0..N-1 -> forward, times the
@@ -350,7 +357,7 @@ constexpr bool toep_split(int kind) { return kind != TK_ROWF; }
 template <typename T, int N1, int N2, int KIND>
 __global__ void __launch_bounds__(256)
 k_toep(const cx<T> *in, cx<T> *out, long long in_so, long long in_sk, long long out_so, long long out_sk,
-       const cx<T> *__restrict__ tw, int Gs, int nchunk, const T *__restrict__ mid, ToepCg<T> cg) {
+       const cx<T> *__restrict__ tw, int Gs, int nchunk, const T *__restrict__ mid, ToepCg<T> cg, int pstride) {
   constexpr int n = N1 * N2, H1 = N1 / 2, H2 = N2 / 2;
   constexpr bool SPLIT = toep_split(KIND);
   static_assert(N1 % 2 == 0 && N2 % 2 == 0 && N1 >= N2, "even factors, N1 >= N2");
@@ -451,7 +458,7 @@ k_toep(const cx<T> *in, cx<T> *out, long long in_so, long long in_sk, long long 
     if (r == 0) {
       double t = 0;
       for (int rr = 0; rr < N1; rr++) t += sh[rr * G + g];
-      cg.part[(long long)o * ((long long)nchunk << Gs) + plane] = t;
+      cg.part[(long long)o * pstride + plane] = t;
     }
   } else {
     // forward N2-point DFT, real product, inverse N2-point DFT, twiddle, exchange

@@ -433,6 +433,13 @@ __global__ void k_copy_cx(cx<T> *__restrict__ d, const cx<T> *__restrict__ s, lo
   if (i < total) d[i] = s[i];
 }
 
+// a -= b
+template <typename T>
+__global__ void k_sub_cx(cx<T> *__restrict__ a, const cx<T> *__restrict__ b, long long total) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < total) a[i] = a[i] - b[i];
+}
+
 template <typename T>
 __global__ void k_zero_cx(cx<T> *__restrict__ d, long long total) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -461,10 +468,14 @@ k_tv_grad(const T *__restrict__ u, T *__restrict__ grad, int ny, int nx, int nz,
   const long long total = (long long)(yhi - ylo + 1) * rowlen;
   const T eps = (T)1e-8;
   double accg = 0, acca = 0;
-  // grid: x = chunks of one row (x, z), y = band row; 32-bit index math
-  const int y = ylo + (int)blockIdx.y;
+  // blocks: (chunk of one row (x, z), band row), renumbered so that each XCD
+  // takes a contiguous run of rows (the y neighbours then share its L2);
+  // 32-bit index math
+  const unsigned lb = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+  const unsigned bxk = lb % gridDim.x;
+  const int y = ylo + (int)(lb / gridDim.x);
   const unsigned rl = (unsigned)rowlen, unz = (unsigned)nz;
-  for (unsigned rem = blockIdx.x * kThreads + threadIdx.x; rem < rl; rem += gridDim.x * kThreads) {
+  for (unsigned rem = bxk * kThreads + threadIdx.x; rem < rl; rem += gridDim.x * kThreads) {
     int x = (int)(rem / unz), z = (int)(rem - (unsigned)x * unz);
     const long long sy = rowlen, sx = nz;
     const long long o = (long long)y * sy + (long long)x * sx + z;
@@ -505,8 +516,8 @@ k_tv_grad(const T *__restrict__ u, T *__restrict__ grad, int ny, int nx, int nz,
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    part_g[blockIdx.y * gridDim.x + blockIdx.x] = sg[0];
-    part_a[blockIdx.y * gridDim.x + blockIdx.x] = sa[0];
+    part_g[lb] = sg[0];
+    part_a[lb] = sa[0];
   }
 }
 
@@ -558,10 +569,12 @@ k_tv_fused(const T *__restrict__ u, const T *__restrict__ g, T *__restrict__ un,
     else return v - c * (gg / s1);
   };
   double accg = 0, acca = 0;
-  // grid: x = chunks of one row (x, z), y = band row; 32-bit index math
-  const int y = ylo + (int)blockIdx.y;
+  // blocks as in k_tv_grad
+  const unsigned lb = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+  const unsigned bxk = lb % gridDim.x;
+  const int y = ylo + (int)(lb / gridDim.x);
   const unsigned rl = (unsigned)rowlen, unz = (unsigned)nz;
-  for (unsigned rem = blockIdx.x * kThreads + threadIdx.x; rem < rl; rem += gridDim.x * kThreads) {
+  for (unsigned rem = bxk * kThreads + threadIdx.x; rem < rl; rem += gridDim.x * kThreads) {
     int x = (int)(rem / unz), z = (int)(rem - (unsigned)x * unz);
     const long long sy = rowlen, sx = nz;
     const long long o = (long long)y * sy + (long long)x * sx + z;
@@ -603,8 +616,8 @@ k_tv_fused(const T *__restrict__ u, const T *__restrict__ g, T *__restrict__ un,
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    part_g[blockIdx.y * gridDim.x + blockIdx.x] = sg[0];
-    part_a[blockIdx.y * gridDim.x + blockIdx.x] = sa[0];
+    part_g[lb] = sg[0];
+    part_a[lb] = sa[0];
   }
 }
 

@@ -385,15 +385,15 @@ void ToeplitzOp<T>::apply_cg(const ToepCg<T> &cg, cx<T> *q, hipStream_t st) {
   const cx<T> *tw = fft.tw.template as<cx<T>>();
   {
     ProfScope ps(KC_LINE_FFT, st, (3.0 * N + 2.0 * L) * N * P * sizeof(cx<T>));
-    toep_launch<T>(TK_ROWF, L, N, nullptr, x, NP, P, LP, P, P, st, tw, nullptr, cg);
+    toep_launch<T>(TK_ROWF, L, N, nullptr, x, NP, P, LP, P, P, st, tw, nullptr, cg, P);
   }
   {
     ProfScope ps(KC_TOEPLITZ_COL, st, 2.0 * N * L * P * sizeof(cx<T>) + (double)L * L * sizeof(T));
-    toep_launch<T>(TK_COL, L, L, x, x, P, LP, P, LP, P, st, tw, khat_re.template as<T>(), cg);
+    toep_launch<T>(TK_COL, L, L, x, x, P, LP, P, LP, P, st, tw, khat_re.template as<T>(), cg, P);
   }
   {
     ProfScope ps(KC_LINE_FFT, st, (2.0 * N + L) * N * P * sizeof(cx<T>));
-    toep_launch<T>(TK_ROWI, L, N, x, q, LP, P, NP, P, P, st, tw, nullptr, cg);
+    toep_launch<T>(TK_ROWI, L, N, x, q, LP, P, NP, P, P, st, tw, nullptr, cg, P);
   }
   NUDIM_HIP(hipGetLastError());
 }
@@ -479,15 +479,30 @@ void Cgnr<T>::solve(const cx<T> *y, cx<T> *fhat, int nsteps, bool f0_zero, hipSt
   const unsigned bN = cdiv(NN, 256), bM = cdiv(MM, 256);
   cx<T> *zp = z.as<cx<T>>(), *pp = pv.as<cx<T>>(), *rp = r.as<cx<T>>(), *vp = v.as<cx<T>>(), *tp = t.as<cx<T>>();
   double *dz = dot_z.as<double>(), *cf = coef.as<double>(), *s = sc.as<double>();
-  // before_loop: r = y - A f ; t = W r ; z = A^H t ; dot_z ; p = z
-  if (!f0_zero) op->trafo(fhat, vp, st);
-  hipLaunchKernelGGL(k_resid_init<T>, dim3(bM), dim3(256), 0, st, y, f0_zero ? (const cx<T> *)nullptr : vp, w, rp,
-                     tp, MM, P);
-  op->adjoint(tp, zp, st);
+  const bool fused = form == NUDIM_CG_TOEPLITZ && toep->fused();
+  if (fused) {
+    // z = A^H W (y - A f) = A^H (W y) - T f: one adjoint and one normal-
+    // operator pass (row pass with p := f, q = T f; p and the row sums are
+    // overwritten by the first step)
+    hipLaunchKernelGGL(k_resid_init<T>, dim3(bM), dim3(256), 0, st, y, (const cx<T> *)nullptr, w, rp, tp, MM, P);
+    op->adjoint(tp, zp, st);
+    if (!f0_zero) {
+      cx<T> *qp = q.as<cx<T>>();
+      ToepCg<T> c0{fhat, pp, nullptr, dz, nullptr, nullptr, rowpart.as<double>()};
+      toep->apply_cg(c0, qp, st);
+      hipLaunchKernelGGL(k_sub_cx<T>, dim3(bN), dim3(256), 0, st, zp, qp, NN);
+    }
+  } else {
+    // before_loop: r = y - A f ; t = W r ; z = A^H t ; dot_z ; p = z
+    if (!f0_zero) op->trafo(fhat, vp, st);
+    hipLaunchKernelGGL(k_resid_init<T>, dim3(bM), dim3(256), 0, st, y, f0_zero ? (const cx<T> *)nullptr : vp, w, rp,
+                       tp, MM, P);
+    op->adjoint(tp, zp, st);
+  }
   red.dot<T>(zp, nullptr, (long long)N * N, s, st);
   hipLaunchKernelGGL(k_cg_scalars, dim3(cdiv(P, 64)), dim3(64), 0, st, dz, cf, s, P, 0);
   NUDIM_HIP(hipGetLastError());
-  if (form == NUDIM_CG_TOEPLITZ && toep->fused()) {
+  if (fused) {
     // step k: row pass forms p_k = z_k + beta_k p_{k-1} (p_0 = z_0) and
     // applies the previous step's f += alpha p_{k-1}; q = T p_k with the
     // row sums of <p_k, q>; alpha_k = |z_k|^2 / <p_k, q>; z -= alpha_k q;
@@ -555,17 +570,20 @@ void Cgnr<T>::solve(const cx<T> *y, cx<T> *fhat, int nsteps, bool f0_zero, hipSt
 // a final update, each followed by the fixed-order sum of the two norms
 // ---------------------------------------------------------------------------
 
+int tv_blocks(int ny, int nx, int nz) { return ny * 64; }
+
 template <typename T>
 T *tv_sweeps(T *u0, T *u1, T *g0, T *g1, int ny, int nx, int nz, int ylo, int yhi, float lam, int niter,
              double *part_g, double *part_a, double *sums, int nb, hipStream_t st) {
   if (niter <= 0 || yhi < ylo) return u0;
-  const long long V = (long long)ny * nx * nz;
-  const long long band = (long long)(yhi - ylo + 1) * nx * nz;
+  const long long rowlen = (long long)nx * nz;
+  const long long V = (long long)ny * rowlen;
+  const long long band = (long long)(yhi - ylo + 1) * rowlen;
   const double lamd = (double)lam;
   const double vb = (double)band * sizeof(T);  // algorithmic: read u once + write u once per sweep
   // grid: (chunks of a row, band rows); one partial sum per block
   const int nrows = yhi - ylo + 1;
-  const int gx = (int)std::min<long long>(cdiv((long long)nx * nz, kThreads), 64);
+  const int gx = (int)std::min<long long>(cdiv(rowlen, kThreads), 64);
   const int np_ = nrows * gx;
   NUDIM_CHECK(np_ <= nb, "TV partial-sum buffer too small");
   const dim3 grid(gx, nrows);
@@ -596,9 +614,9 @@ T *tv_sweeps(T *u0, T *u1, T *g0, T *g1, int ny, int nx, int nz, int ylo, int yh
 }
 
 template float *tv_sweeps<float>(float *, float *, float *, float *, int, int, int, int, int, float, int, double *,
-                                 double *, double *, int, hipStream_t);
+                                   double *, double *, int, hipStream_t);
 template double *tv_sweeps<double>(double *, double *, double *, double *, int, int, int, int, int, float, int,
-                                   double *, double *, double *, int, hipStream_t);
+                                     double *, double *, double *, int, hipStream_t);
 
 template struct LineFft<float>;
 template struct LineFft<double>;

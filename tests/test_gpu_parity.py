@@ -81,6 +81,27 @@ def test_nfft_adjoint_tiled(hip, orc, fp64, n, W, tol):
         assert rel(a64[p], orc.ndft_adjoint(x, g[p], N)) < tol
 
 
+@pytest.mark.parametrize("N", [64, 192])
+def test_nfft_centred_half_lengths(hip, orc, N):
+    """n = 2N on the fixed-length FFT kernels (16 x 8, 24 x 16): the row and
+    column passes of trafo read, and those of the adjoint write, only the N
+    modes centred on zero (half-size DFTs).  Reference: the oracle's NFFT at
+    its own sizes (n = 2 nextpow2(N), 14 taps)."""
+    pmin, pmax = orc.pad_range(-60, 3)
+    x = orc.polar_nodes(pmin, pmax, 2.0, N)
+    M, P = x.shape[0], 4
+    rng = np.random.default_rng(N + 1)
+    fh = rng.standard_normal((P, N, N)) + 1j * rng.standard_normal((P, N, N))
+    g = rng.standard_normal((P, M)) + 1j * rng.standard_normal((P, M))
+    plan = orc.NFFT(x, N)
+    for prec, W, tol in ((hip.FP64_PLAIN, 14, 1e-11), (hip.FP32, 8, 3e-6)):
+        f = hip.nfft_trafo(x, fh, n=2 * N, W=W, precision=prec)
+        a = hip.nfft_adjoint(x, g, N, n=2 * N, W=W, precision=prec)
+        for p in (0, P - 1):
+            assert rel(f[p], plan.trafo(fh[p])) < tol
+            assert rel(a[p], plan.adjoint(g[p])) < tol
+
+
 @pytest.mark.parametrize("weighted", [True, False])
 def test_cgnr(hip, orc, polar32, weighted):
     rng = np.random.default_rng(3)
