@@ -31,8 +31,13 @@ import nudim_amd as nd  # noqa: E402
 from nudim_amd import phantom  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-KC_NAMES = {1: "k_fft_lines<float> (Toeplitz column pass)", 2: "k_interp<float>", 3: "k_gather<float>",
-            4: "k_fft_lines<float> (row passes)", 5: "k_tv_grad/k_tv_update<float>", 6: "k_zfft<float>"}
+# profiler classes of the library (nudim_profile) and the kernels they time
+KC_NAMES = {1: "k_toep<float,32,24,1> (Toeplitz column pass)", 2: "k_interp_w<float,8>",
+            3: "k_gather_tile<float,8>", 4: "k_fft2<float> (NFFT grid passes)", 5: "k_tv_fused<float>",
+            6: "k_zfft<float>", 7: "k_toep<float,32,24,0> (Toeplitz row forward + CG update)",
+            8: "k_toep<float,32,24,2> (Toeplitz row inverse + <p,q>)"}
+KC_CANDIDATES = (7, 1, 8, 3, 5, 4, 2)
+PROFILES = os.path.join(ROOT, "profiles")
 
 
 def log(*a):
@@ -72,9 +77,21 @@ def timed(job, steps, kclass=0):
     return dt, r, prof
 
 
-def cpu_baseline(prm, proj, seconds_budget=30.0):
-    """FP64 C restatement (oracle) on the host: Fourier step of a few planes,
-    full-volume TV and z FFTs, scaled to one pass of the whole volume."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
+def cpu_baseline(prm, proj):
+    """FP64 C restatement (the oracle) on the host: Fourier step of a bounded
+    sample of planes, TV sweeps and z FFTs, scaled to one pass of the whole
+    volume (about 10-20 s of host work)."""
     from oracle import oracle
     threads = min(os.cpu_count() or 1, 16)
     oracle.set_threads(threads)
@@ -88,7 +105,7 @@ def cpu_baseline(prm, proj, seconds_budget=30.0):
     t_setup = time.perf_counter() - t0
     rng = np.random.default_rng(0)
     img = np.abs(rng.standard_normal((nz, np_, np_))) * 0.1
-    nplanes = threads
+    nplanes = min(nz, 4 * threads)
     t_planes = oracle.time_fourier_planes(op, img, slices, kind, 0, nplanes)
     t_fourier = t_planes * nz / nplanes
     data = img.reshape(nz, -1).astype(np.complex128)
@@ -96,17 +113,57 @@ def cpu_baseline(prm, proj, seconds_budget=30.0):
     oracle.fft_strided(data, -1)
     t_zfft = time.perf_counter() - t1
     t1 = time.perf_counter()
-    sweeps = 4
+    sweeps = 8
     oracle.tv_height(img, prm.get("height", np_), prm.get("center", np_ // 2), niter=sweeps)
     t_tv = (time.perf_counter() - t1) * 40 / sweeps
     t_iter = t_fourier + 2 * t_zfft + t_tv
+    sample = t_planes + t_zfft + t_tv * sweeps / 40
     log("  cpu: setup %.1fs, %d planes %.2fs -> Fourier step %.1fs, z-FFT %.2fs, TV(40) %.1fs -> %.1f s/pass"
         % (t_setup, nplanes, t_planes, t_fourier, t_zfft, t_tv, t_iter))
     return {"value": 1.0 / t_iter, "unit": "volume-iter/s", "cores": threads, "kind": "port",
-            "sample": "FP64 C restatement (-O3, OpenMP over planes, %d threads): per-plane projector + "
-                      "10-step NFFT CGNR of %d of %d planes (scaled x%d), 4 TV sweeps (scaled x10), "
-                      "one z FFT (x2); sample %.1f s" % (threads, nplanes, nz, nz // nplanes,
-                                                          t_planes + t_zfft + t_tv / 10)}
+            "sample": "FP64 C restatement of reconstruction.c (-O3, OpenMP over planes, %d threads on %s): "
+                      "projector + 10-step NFFT CGNR of %d of %d planes (scaled x%.0f), %d TV sweeps (scaled x%.0f), "
+                      "one z FFT (x2); %.1f s of host work" % (threads, cpu_model(), nplanes, nz, nz / nplanes,
+                                                               sweeps, 40 / sweeps, sample)}
+
+
+def traffic_from_profiles(kclass):
+    """HBM bytes per launch of the kernel from the committed counter summary
+    (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs, scaled by a copy kernel
+    of known size); None when no summary covers the kernel."""
+    path = os.path.join(PROFILES, "pmc_traffic_c3.txt")
+    try:
+        with open(path) as fh:
+            for line in fh:
+                f = line.split()
+                if len(f) >= 2 and f[0] == "class%d" % kclass:
+                    return float(f[1])
+    except OSError:
+        pass
+    return None
+
+
+def probe(prm, proj, copy_mib):
+    """One FP32 pass and one device-to-device hipMemcpy of copy_mib MiB (the
+    runtime's blit kernel, plain loads and stores: the scale for the
+    FETCH_SIZE / WRITE_SIZE counters): the program the counter runs profile."""
+    import ctypes
+    import torch
+    job = make_job(prm, proj, nd.FP32)
+    job.iterate(1)
+    job.synchronize()
+    nbytes = copy_mib << 20
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    rc = hip.hipMemcpy(ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(a.data_ptr()), nbytes, 3)  # device to device
+    torch.cuda.synchronize()
+    if rc != 0:
+        raise RuntimeError("hipMemcpy failed: %d" % rc)
+    log("probe: one FP32 pass; hipMemcpy device to device of %d MiB" % copy_mib)
+    job.close()
 
 
 def main():
@@ -116,10 +173,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--config", default="c3")
-    ap.add_argument("--kclass", type=int, default=nd.KC_TOEPLITZ_COL)
+    ap.add_argument("--kclass", type=int, default=0, help="profile one kernel class only (0: all candidates)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--probe", type=int, default=0, metavar="MIB",
+                    help="run one FP32 pass and a MIB-MiB device copy, then exit (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -128,6 +187,9 @@ def main():
     t0 = time.perf_counter()
     prm, proj = phantom.config_workload(args.config)
     log("workload %s: projections %s generated in %.1f s" % (args.config, proj.shape, time.perf_counter() - t0))
+    if args.probe:
+        probe(prm, proj, args.probe)
+        return
 
     log("FP32 batched form:")
     j32 = make_job(prm, proj, nd.FP32)
@@ -135,12 +197,24 @@ def main():
     dt32, r32, _ = timed(j32, args.steps)
     rate32 = args.steps / dt32
     log("  %d passes in %.3f s -> %.2f passes/s" % (args.steps, dt32, rate32))
-    dtk, _, prof = timed(j32, args.steps, args.kclass)
-    kms, klaunch, kbytes = prof
-    avg_us = 1e3 * kms / max(klaunch, 1)
-    achieved = kbytes / max(klaunch, 1) / (avg_us * 1e-6) / 1e9 if klaunch else 0.0
-    log("  %s: %d launches, avg %.1f us, %.1f GB/s algorithmic" % (KC_NAMES.get(args.kclass), klaunch, avg_us,
-                                                                 achieved))
+    # per-class HIP-event timing over separate timed runs; the dominant
+    # kernel is the class with the most device time per pass
+    classes = [args.kclass] if args.kclass else list(KC_CANDIDATES)
+    kern = {}
+    for kc in classes:
+        nsteps = max(2, args.steps // 2)
+        _, _, prof = timed(j32, nsteps, kc)
+        kms, klaunch, kbytes = prof
+        if not klaunch:
+            continue
+        avg_us = 1e3 * kms / klaunch
+        kern[kc] = {"name": KC_NAMES.get(kc), "ms_per_pass": kms / nsteps, "launches_per_pass": klaunch / nsteps,
+                    "avg_us": avg_us, "bytes_per_launch": kbytes / klaunch,
+                    "achieved_GBs": kbytes / klaunch / (avg_us * 1e-6) / 1e9}
+        log("  %s: %.2f ms/pass, %d launches, avg %.1f us, %.1f GB/s algorithmic"
+            % (KC_NAMES.get(kc), kms / nsteps, klaunch, avg_us, kern[kc]["achieved_GBs"]))
+    dom = max(kern, key=lambda k: kern[k]["ms_per_pass"])
+    kd = kern[dom]
 
     rate64 = None
     acc = {}
@@ -170,6 +244,23 @@ def main():
                                                                               acc["rel_l2_after_passes"]))
         j64.close()
     j32.close()
+    if not args.no_plain and not args.no_accuracy:
+        # The HIO + TV schedule above amplifies any rounding difference
+        # (TV ~1e3x per pass, HIO flips voxels near zero), so the FP32 error
+        # bar is checked on the continuous part of the map: ER passes without
+        # the denoiser, each form from its own initial INFFT.
+        prm_er = dict(prm, nhio=0, denoise=False)
+        npass = 3
+        vols, rs = [], []
+        for prec in (nd.FP32, nd.FP64_PLAIN):
+            j = make_job(prm_er, proj, prec)
+            rs.append(j.iterate(npass))
+            vols.append(j.volume())
+            j.close()
+        acc["er_no_denoise"] = {"passes": npass, "rel_l2": rel(vols[0], vols[1]),
+                                "max_abs_rfactor_drift": float(np.max(np.abs(rs[0] - rs[1])))}
+        log("  ER without TV, %d passes: rel L2 FP32 vs FP64 %.2e, R drift %.2e"
+            % (npass, acc["er_no_denoise"]["rel_l2"], acc["er_no_denoise"]["max_abs_rfactor_drift"]))
 
     cpu = None
     if not args.no_cpu:
@@ -179,7 +270,7 @@ def main():
             log("cpu baseline skipped: %s" % e)
 
     out = {
-        "metric": "ER/HIO passes/s at 256^3 (c3), FP32 batched form",
+        "metric": "ER iters/s at 256\u00b3 FP32-batched vs FP64 plain (\u22641e-4 rel L2); p50/p99 job latency",
         "value": rate32,
         "unit": "volume-iter/s",
         "n_gpus": 1,
@@ -197,9 +288,12 @@ def main():
         "fp64_plain_passes_per_s": rate64,
         "speedup_vs_fp64_plain": (rate32 / rate64) if rate64 else None,
         "accuracy": acc or None,
-        "roofline": {"kernel": KC_NAMES.get(args.kclass), "bound": "hbm", "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "avg_us": avg_us, "launches": klaunch},
+        "roofline": {"kernel": kd["name"], "bound": "hbm", "achieved": kd["achieved_GBs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": kd["achieved_GBs"] / HBM_PEAK_GBS,
+                     "traffic": traffic_from_profiles(dom), "algorithmic_bytes": kd["bytes_per_launch"],
+                     "avg_us": kd["avg_us"]},
+        "kernels": {KC_NAMES.get(k): {kk: round(vv, 4) for kk, vv in v.items() if kk != "name"}
+                    for k, v in kern.items()},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

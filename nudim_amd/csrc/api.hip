@@ -569,7 +569,7 @@ extern "C" int nudim_set_dump(nudim_ctx *c, int nn, double *before, double *afte
 }
 
 extern "C" int nudim_profile(nudim_ctx *c, int kernel_class) {
-  if (!c || kernel_class < 0 || kernel_class > KC_ZFFT) return fail(NUDIM_EINVAL, "invalid profile request");
+  if (!c || kernel_class < 0 || kernel_class > KC_TOEPLITZ_ROWI) return fail(NUDIM_EINVAL, "invalid profile request");
   c->prof.reset(kernel_class);
   return NUDIM_OK;
 }

@@ -104,6 +104,8 @@ enum KernelClass {
   KC_LINE_FFT = 4,      // other line FFT passes
   KC_TV = 5,            // TV sweep kernels
   KC_ZFFT = 6,          // z-axis transforms
+  KC_TOEPLITZ_ROWF = 7, // Toeplitz row forward pass with the CG prologue
+  KC_TOEPLITZ_ROWI = 8, // Toeplitz row inverse pass with the <p, q> row sums
 };
 
 // Records an event pair around every launch of one kernel class and

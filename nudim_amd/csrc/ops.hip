@@ -384,7 +384,10 @@ void ToeplitzOp<T>::apply_cg(const ToepCg<T> &cg, cx<T> *q, hipStream_t st) {
   cx<T> *x = X.as<cx<T>>();
   const cx<T> *tw = fft.tw.template as<cx<T>>();
   {
-    ProfScope ps(KC_LINE_FFT, st, (3.0 * N + 2.0 * L) * N * P * sizeof(cx<T>));
+    // algorithmic bytes: z read, p written, and with the recurrence p read,
+    // f read and written (N x N each); X written (N x L)
+    const double nvec = 2.0 + (cg.zz_old ? 1.0 : 0.0) + (cg.alpha ? 2.0 : 0.0);
+    ProfScope ps(KC_TOEPLITZ_ROWF, st, (nvec * N + L) * N * P * sizeof(cx<T>));
     toep_launch<T>(TK_ROWF, L, N, nullptr, x, NP, P, LP, P, P, st, tw, nullptr, cg, P);
   }
   {
@@ -392,7 +395,8 @@ void ToeplitzOp<T>::apply_cg(const ToepCg<T> &cg, cx<T> *q, hipStream_t st) {
     toep_launch<T>(TK_COL, L, L, x, x, P, LP, P, LP, P, st, tw, khat_re.template as<T>(), cg, P);
   }
   {
-    ProfScope ps(KC_LINE_FFT, st, (2.0 * N + L) * N * P * sizeof(cx<T>));
+    // algorithmic bytes: X read (N x L), p read, q written (N x N each)
+    ProfScope ps(KC_TOEPLITZ_ROWI, st, (2.0 * N + L) * N * P * sizeof(cx<T>));
     toep_launch<T>(TK_ROWI, L, N, x, q, LP, P, NP, P, P, st, tw, nullptr, cg, P);
   }
   NUDIM_HIP(hipGetLastError());
