@@ -46,7 +46,8 @@ struct NfftOp {
   DevBuf c0, c1, psi0, psi1, dec, cell_ptr, ent;
   // tiled gather map (used when n % kGatherTile == 0 and P % 64 == 0)
   bool tiled = false;
-  DevBuf tile_chunk, chunk_node, tnode, chunk_cell, tslot, tweight;
+  int nchunk = 0, ncomb = 0;
+  DevBuf chunk_tile, chunk_part, chunk_node, tnode, chunk_cell, tent, comb, part;
   DevBuf X, grid;  // [N][n][P], [n][n][P]
   size_t bytes() const;
   void init(int N, int n, int W, const double *x, int M, int P);

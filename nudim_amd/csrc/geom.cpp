@@ -204,6 +204,20 @@ void build_gather_tiles(const NfftTables &t, int ts, int k, GatherTiles *g) {
     g->tile_chunk[tl + 1] = g->tile_chunk[tl] + nch;
   }
   const int nchunk = g->tile_chunk[ntile];
+  g->chunk_tile.assign(nchunk, 0);
+  g->chunk_part.assign(nchunk, -1);
+  g->comb.clear();
+  g->npart = 0;
+  for (int tl = 0; tl < ntile; tl++) {
+    const int cb = g->tile_chunk[tl], ce = g->tile_chunk[tl + 1];
+    for (int c = cb; c < ce; c++) g->chunk_tile[c] = tl;
+    if (ce - cb != 1) {
+      g->comb.push_back(tl);
+      g->comb.push_back(g->npart);
+      g->comb.push_back(ce - cb);
+      for (int c = cb; c < ce; c++) g->chunk_part[c] = g->npart++;
+    }
+  }
   g->chunk_cell.assign((size_t)nchunk * (tt + 1), 0);
   g->slot.clear();
   g->weight.clear();

@@ -54,16 +54,21 @@ void build_nfft_tables(int N, int n, int W, const double *x, int M, bool with_ga
 // whose W x W footprint meets a tile, in increasing node order, are cut into
 // chunks of at most k nodes; a chunk's node values are what one block stages
 // in LDS.  Within a chunk the entries are grouped by cell (row-major within
-// the tile) and, per cell, kept in node order, so that summing chunk after
-// chunk visits each cell's taps in the same order as the per-cell map.
+// the tile) and, per cell, kept in node order.  A tile with one chunk is
+// summed straight into the grid; the chunks of a tile with several write
+// partial sums (slot chunk_part[c]) that a second pass adds in chunk order,
+// and that pass also clears the tiles no node meets.
 struct GatherTiles {
-  int ts = 0, k = 0, ntile = 0, max_chunk_entries = 0;
+  int ts = 0, k = 0, ntile = 0, max_chunk_entries = 0, npart = 0;
   std::vector<int> tile_chunk;      // [ntile+1]: chunk ids of each tile
   std::vector<int> chunk_node;      // [nchunk+1]: range of each chunk in node
   std::vector<int> node;            // node ids
   std::vector<int> chunk_cell;      // [nchunk][ts*ts+1]: absolute entry ranges per cell
   std::vector<uint16_t> slot;       // entry -> node slot within its chunk
   std::vector<double> weight;       // entry -> psi0[node][t0] * psi1[node][t1]
+  std::vector<int> chunk_tile;      // [nchunk]
+  std::vector<int> chunk_part;      // [nchunk]: partial-sum slot, -1 for a tile's only chunk
+  std::vector<int> comb;            // tiles with != 1 chunk: (tile, first slot, count) triples
 };
 void build_gather_tiles(const NfftTables &t, int ts, int k, GatherTiles *g);
 

@@ -146,62 +146,103 @@ __device__ __forceinline__ double lane_bcast(double v, int j) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-// Tiled gather (same sums, same order as k_gather_w): one block per TS x TS
-// tile of cells and 64 planes.  Lane = plane; wave w owns the tile's cells
-// w*CPW .. w*CPW+CPW-1 (row-major in the tile).  For each node chunk of the
-// tile (at most 64 nodes) the block stages the chunk's node values in LDS,
-// [slot][64 planes], so each node value is read from memory once per tile
-// instead of once per tap; then each wave walks its cells' entries, 64 at a
-// time in registers (slot, weight) and broadcast lane by lane.
+// One entry of the tiled gather map: the node's slot in its chunk and the
+// window weight psi0[node][t0] psi1[node][t1].
+template <typename T>
+struct GEnt {
+  int s;
+  T w;
+};
+
+// Tiled gather, one block per node chunk (at most 64 nodes) of a TS x TS
+// tile and 64 planes.  The block stages the chunk's node values in LDS,
+// [slot][64 planes] (65 rows: row 64 is a spare), so each node value is read once per tile it meets
+// instead of once per tap; lane = plane; wave w owns the tile's cells
+// w*CPW .. w*CPW+CPW-1 (row-major in the tile) and sums each cell's entries
+// of the chunk in node order.  The only chunk of a tile writes the grid; the
+// chunks of a tile with several write partial sums part[slot][cell][p] for
+// k_gather_combine.
 template <typename T, int TS>
 __global__ void __launch_bounds__(kThreads)
-k_gather_tile(const cx<T> *__restrict__ s, cx<T> *__restrict__ g, const int *__restrict__ tile_chunk,
-              const int *__restrict__ chunk_node, const int *__restrict__ node, const int *__restrict__ chunk_cell,
-              const uint16_t *__restrict__ slot, const T *__restrict__ wt, int n, int P) {
+k_gather_chunk(const cx<T> *__restrict__ s, cx<T> *__restrict__ g, cx<T> *__restrict__ part,
+               const int *__restrict__ chunk_tile, const int *__restrict__ chunk_part,
+               const int *__restrict__ chunk_node, const int *__restrict__ node, const int *__restrict__ chunk_cell,
+               const GEnt<T> *__restrict__ ent, int n, int P) {
   static_assert(kThreads == 256, "four wavefronts per block");
   constexpr int TT = TS * TS, CPW = TT / 4;
   extern __shared__ unsigned char smem_raw[];
   cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
-  const int nt = n / TS;
-  const int tile = (int)xcd_block(blockIdx.x, gridDim.x);
-  const int ty = tile / nt, tx = tile - ty * nt;
+  const int c = (int)xcd_block(blockIdx.x, gridDim.x);
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int pbase = blockIdx.y * 64;
-  const cx<T> *sp = s + pbase + lane;
+  const int a = chunk_node[c], nn = chunk_node[c + 1] - a;  // 1 <= nn <= 64
+  {
+    // wave w stages slots w, w+4, ..., w+60: sixteen unconditional loads in
+    // flight (slot clamped into the chunk) and sixteen unconditional stores
+    // (slots past the chunk go to the spare row 64), so that no load is
+    // sunk into a branch of its own
+    const cx<T> *sp = s + pbase + lane;
+    const int myn = node[a + min(lane, nn - 1)];
+    cx<T> v[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++)
+      v[j] = sp[(size_t)__builtin_amdgcn_readlane(myn, min(wv + 4 * j, nn - 1)) * P];
+#pragma unroll
+    for (int j = 0; j < 16; j++) lds[(wv + 4 * j < nn ? wv + 4 * j : 64) * 64 + lane] = v[j];
+  }
+  __syncthreads();
+  const int tile = chunk_tile[c], ps = chunk_part[c];
+  const int nt = n / TS, ty = tile / nt, tx = tile - ty * nt;
+  // the wave's cells hold entries [E0, E1) of the chunk, cell k from cc[k]:
+  // read 64 entries at a time lane-parallel, broadcast each with readlane
+  const int *cc = chunk_cell + (size_t)c * (TT + 1) + wv * CPW;
+  int cb[CPW + 1];
+#pragma unroll
+  for (int k = 0; k <= CPW; k++) cb[k] = __builtin_amdgcn_readfirstlane(cc[k]);
   cx<T> acc[CPW];
 #pragma unroll
   for (int k = 0; k < CPW; k++) acc[k] = mk((T)0, (T)0);
-  const int cb = tile_chunk[tile], ce = tile_chunk[tile + 1];
-  for (int c = cb; c < ce; c++) {
-    const int a = chunk_node[c], nn = chunk_node[c + 1] - a;  // 1 <= nn <= 64
-    const int myn = node[a + min(lane, nn - 1)];
-#pragma unroll 4
-    for (int i = wv; i < nn; i += 4) lds[i * 64 + lane] = sp[(size_t)__builtin_amdgcn_readlane(myn, i) * P];
-    __syncthreads();
-    const int *cc = chunk_cell + (size_t)c * (TT + 1) + wv * CPW;
-    const int E0 = cc[0], E1 = cc[CPW];
-    for (int base = E0; base < E1; base += 64) {
-      const int ei = min(base + lane, E1 - 1);
-      const int sl = (int)slot[ei];
-      const T w = wt[ei];
+  for (int base = cb[0]; base < cb[CPW]; base += 64) {
+    const GEnt<T> q = ent[min(base + lane, cb[CPW] - 1)];
 #pragma unroll
-      for (int k = 0; k < CPW; k++) {
-        const int ka = max(cc[k], base), kb = min(cc[k + 1], base + 64);
-        for (int e = ka; e < kb; e++) {
-          const int j = e - base;
-          const int si = __builtin_amdgcn_readlane(sl, j);
-          acc[k] = acc[k] + lane_bcast(w, j) * lds[si * 64 + lane];
-        }
+    for (int k = 0; k < CPW; k++) {
+      const int ka = max(cb[k], base), kb = min(cb[k + 1], base + 64);
+      for (int e = ka; e < kb; e++) {
+        const int j = e - base;
+        acc[k] = acc[k] + lane_bcast(q.w, j) * lds[__builtin_amdgcn_readlane(q.s, j) * 64 + lane];
       }
     }
-    __syncthreads();
   }
 #pragma unroll
   for (int k = 0; k < CPW; k++) {
     const int cl = wv * CPW + k;
+    if (ps < 0) {
+      const size_t cell = (size_t)(ty * TS + cl / TS) * n + (size_t)(tx * TS + cl % TS);
+      g[cell * P + pbase + lane] = acc[k];
+    } else {
+      part[((size_t)ps * TT + cl) * P + pbase + lane] = acc[k];
+    }
+  }
+}
+
+// Tiles with several node chunks (or none): grid cell = the chunks' partial
+// sums added in chunk order (zero for a tile no node meets).  comb holds
+// (tile, first slot, count) triples; one block per tile and 64 planes.
+template <typename T, int TS>
+__global__ void __launch_bounds__(kThreads)
+k_gather_combine(const cx<T> *__restrict__ part, cx<T> *__restrict__ g, const int *__restrict__ comb, int n, int P) {
+  constexpr int TT = TS * TS;
+  const int b = (int)xcd_block(blockIdx.x, gridDim.x);
+  const int tile = comb[3 * b], p0 = comb[3 * b + 1], cnt = comb[3 * b + 2];
+  const int nt = n / TS, ty = tile / nt, tx = tile - ty * nt;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int p = blockIdx.y * 64 + lane;
+  for (int cl = wv; cl < TT; cl += 4) {
+    cx<T> acc = mk((T)0, (T)0);
+    for (int i = 0; i < cnt; i++) acc = acc + part[((size_t)(p0 + i) * TT + cl) * P + p];
     const size_t cell = (size_t)(ty * TS + cl / TS) * n + (size_t)(tx * TS + cl % TS);
-    g[cell * P + pbase + lane] = acc[k];
+    g[cell * P + p] = acc;
   }
 }
 

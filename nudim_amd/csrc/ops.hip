@@ -160,7 +160,8 @@ void LineFft<T>::run(const cx<T> *in, cx<T> *out, int nlines, const LineIO &li, 
 template <typename T>
 size_t NfftOp<T>::bytes() const {
   return fft.tw.bytes + c0.bytes + c1.bytes + psi0.bytes + psi1.bytes + dec.bytes + cell_ptr.bytes + ent.bytes +
-         tile_chunk.bytes + chunk_node.bytes + tnode.bytes + chunk_cell.bytes + tslot.bytes + tweight.bytes +
+         chunk_tile.bytes + chunk_part.bytes + chunk_node.bytes + tnode.bytes + chunk_cell.bytes + tent.bytes +
+         comb.bytes + part.bytes +
          X.bytes + grid.bytes;
 }
 
@@ -180,16 +181,21 @@ void NfftOp<T>::init(int N_, int n_, int W_, const double *x, int M_, int P_) {
   upload(dec, t.dec);
   tiled = (n % kGatherTile == 0) && (P % 64 == 0);
   if (tiled) {
-    // node values of a chunk staged as [slot][64 planes]: 32 KB of LDS
+    // node values of a chunk staged as [slot][64 planes]: 64 nodes per chunk
     GatherTiles gt;
-    build_gather_tiles(t, kGatherTile, (int)(32768 / (64 * sizeof(cx<T>))), &gt);
-    upload(tile_chunk, gt.tile_chunk);
+    build_gather_tiles(t, kGatherTile, 64, &gt);
+    nchunk = (int)gt.chunk_tile.size();
+    ncomb = (int)gt.comb.size() / 3;
+    upload(chunk_tile, gt.chunk_tile);
+    upload(chunk_part, gt.chunk_part);
     upload(chunk_node, gt.chunk_node);
     upload(tnode, gt.node);
     upload(chunk_cell, gt.chunk_cell);
-    upload(tslot, gt.slot);
-    std::vector<T> wv(gt.weight.begin(), gt.weight.end());
-    upload(tweight, wv);
+    upload(comb, gt.comb);
+    std::vector<GEnt<T>> ev(gt.slot.size());
+    for (size_t e = 0; e < ev.size(); e++) ev[e] = GEnt<T>{(int)gt.slot[e], (T)gt.weight[e]};
+    upload(tent, ev);
+    part.alloc((size_t)std::max(gt.npart, 1) * kGatherTile * kGatherTile * P * sizeof(cx<T>));
   } else {
     upload(cell_ptr, t.cell_ptr);
     upload(ent, t.ent);
@@ -255,11 +261,15 @@ void NfftOp<T>::adjoint(const cx<T> *f, cx<T> *fhat, hipStream_t st) {
   int ncell = n * n;
   dim3 gr(cdiv(ncell, kThreads >> PCs), P >> PCs);
   if (tiled) {
-    const int nt = n / kGatherTile;
     ProfScope ps(KC_GATHER, st, ((double)M * sizeof(cx<T>) + 4.0 * M + (double)ncell * sizeof(cx<T>)) * P);
-    hipLaunchKernelGGL((k_gather_tile<T, kGatherTile>), dim3(nt * nt, P / 64), dim3(kThreads),
-                       32768, st, f, grid.as<cx<T>>(), tile_chunk.as<int>(), chunk_node.as<int>(),
-                       tnode.as<int>(), chunk_cell.as<int>(), tslot.as<uint16_t>(), tweight.as<T>(), n, P);
+    cx<T> *gp = grid.as<cx<T>>(), *pp = part.as<cx<T>>();
+    if (nchunk > 0)
+      hipLaunchKernelGGL((k_gather_chunk<T, kGatherTile>), dim3(nchunk, P / 64), dim3(kThreads),
+                         65 * 64 * sizeof(cx<T>), st, f, gp, pp, chunk_tile.as<int>(), chunk_part.as<int>(),
+                         chunk_node.as<int>(), tnode.as<int>(), chunk_cell.as<int>(), tent.as<GEnt<T>>(), n, P);
+    if (ncomb > 0)
+      hipLaunchKernelGGL((k_gather_combine<T, kGatherTile>), dim3(ncomb, P / 64), dim3(kThreads), 0, st, pp, gp,
+                         comb.as<int>(), n, P);
   } else {
   // algorithmic bytes: node values read, node ids of the inverse map, grid written
   ProfScope ps(KC_GATHER, st, ((double)M * sizeof(cx<T>) + 4.0 * M + (double)ncell * sizeof(cx<T>)) * P);

@@ -63,9 +63,10 @@ def test_nfft_fp32(hip, orc, polar32, n, W, tol):
 @pytest.mark.parametrize("fp64,n,W,tol", [(True, 128, 14, 1e-12), (False, 128, 8, 2e-6), (False, 96, 6, 1e-4)])
 def test_nfft_adjoint_tiled(hip, orc, fp64, n, W, tol):
     """64 planes on a grid whose edge is a multiple of 8 take the LDS-tiled
-    gather (the dense centre tiles need several node chunks); 63 planes take
-    the per-cell gather.  Both sum each cell's taps in node order, so the
-    shared planes agree bit for bit, and both match the NDFT."""
+    gather (the dense centre tiles need several node chunks, summed as
+    per-chunk partial sums added in chunk order); 63 planes take the
+    per-cell gather (one running sum per cell).  The shared planes agree to
+    rounding, and both match the NDFT."""
     N = 64
     pmin, pmax = orc.pad_range(-60, 3)
     x = orc.polar_nodes(pmin, pmax, 1.0, N)
@@ -75,7 +76,7 @@ def test_nfft_adjoint_tiled(hip, orc, fp64, n, W, tol):
     prec = hip.FP64_PLAIN if fp64 else hip.FP32
     a64 = hip.nfft_adjoint(x, g, N, n=n, W=W, precision=prec)
     a63 = hip.nfft_adjoint(x, g[:63], N, n=n, W=W, precision=prec)
-    assert np.array_equal(a64[:63], a63)
+    assert rel(a64[:63], a63) < (1e-14 if fp64 else 1e-6)
     # tol: FP64 rounding, FP32 rounding, window error of sigma 1.5 and W 6 (5.3e-5 in FP64)
     for p in (0, 37, 63):
         assert rel(a64[p], orc.ndft_adjoint(x, g[p], N)) < tol
