@@ -137,8 +137,9 @@ int nudim_constraint(int precision, double *img, const double *img_last, int nz,
 /* projection FT, src/reconstruction.c:306-367: proj [nmeas][nz][nx] -> tomo c128 [nmeas][nz][np] */
 int nudim_projection_ft(int precision, int nx, int nz, int np, int nmeas, const double *proj,
                         double *tomo);
-/* 2D1D simulator, src/projection_sim.c:282-405 without noise:
- * density [nz][ny][nx] -> proj [nslices][nz][np] = creal(slice)/(np*nz) */
+/* 2D1D simulator, src/projection_sim.c:282-405 without noise (FP64; nx == ny,
+ * nx, nz, np even): density [nz][ny][nx] -> proj [nslices][nz][np] =
+ * creal(slice)/(np*nz), nslices = (int)((gam_max-gam_min)/interval+0.5)+1 */
 int nudim_project_2d1d(int nx, int ny, int nz, int np, double gam_min, double gam_max,
                        double interval, const double *density, double *proj);
 

@@ -71,19 +71,22 @@ def _wait(procs):
             raise subprocess.CalledProcessError(p.returncode, p.args)
 
 
+# host programs (one per main) and the helpers every one of them links
+HOST_MAINS = ["reconstruction.c", "projection_sim.c"]
+HOST_HELPERS = ["textio.c"]
+
+
 def build_host(force=False):
     os.makedirs(BIN, exist_ok=True)
     out = []
-    if not os.path.isdir(HOST):
-        return out
-    for src in sorted(os.listdir(HOST)):
-        if not src.endswith(".c"):
-            continue
+    helpers = [os.path.join(HOST, h) for h in HOST_HELPERS]
+    deps = helpers + [os.path.join(HOST, h[:-2] + ".h") for h in HOST_HELPERS]
+    for src in HOST_MAINS:
         exe = os.path.join(BIN, src[:-2])
         s = os.path.join(HOST, src)
-        if force or _stale(exe, [s, LIB, os.path.join(INCLUDE, "nudim.h")]):
-            _run(["gcc", "-O2", "-std=gnu99", "-Wall", "-o", exe, s, "-I", INCLUDE, "-L", HERE, "-lnudim_hip",
-                  "-Wl,-rpath,$ORIGIN/..", "-lm"])
+        if force or _stale(exe, [s, LIB, os.path.join(INCLUDE, "nudim.h")] + deps):
+            _run(["gcc", "-O2", "-std=gnu99", "-Wall", "-o", exe, s] + helpers +
+                 ["-I", INCLUDE, "-I", HOST, "-L", HERE, "-lnudim_hip", "-Wl,-rpath,$ORIGIN/..", "-lm", "-lpthread"])
         out.append(exe)
     return out
 

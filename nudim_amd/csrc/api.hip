@@ -894,8 +894,101 @@ extern "C" int nudim_projection_ft(int precision, int nx, int nz, int np, int nm
   API_END
 }
 
+// ---------------------------------------------------------------------------
+// 2D1D forward simulator (projection_sim.c:282-405)
+// ---------------------------------------------------------------------------
+
+// density [z][y][x] (FP64) -> vol [y][x][z] with z taken at S(z) = (z+nz/2) mod nz:
+// the z-FFT kernel reads vol[S(z)] = density[z], so its planes are the
+// simulator's FFT along z (:319-322) at the shifted index (:336)
+__global__ void k_sim_volume(const double *__restrict__ d, double *__restrict__ vol, int nx, int ny, int nz) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)nx * ny * nz) return;
+  int z = (int)(i % nz);
+  long long t = i / nz;
+  int x = (int)(t % nx), y = (int)(t / nx);
+  int zs = (z + nz / 2) % nz;
+  vol[i] = d[((long long)zs * ny + y) * nx + x];
+}
+
+// c[(s*np + col)*nz + row] -> proj[s][row][col] = Re / (np * nz)  (:398-402)
+__global__ void k_sim_out(const cx<double> *__restrict__ c, double *__restrict__ proj, int np, int nz, int ns) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)ns * nz * np) return;
+  int col = (int)(i % np);
+  long long t = i / np;
+  int row = (int)(t % nz), s = (int)(t / nz);
+  proj[i] = c[((long long)s * np + col) * nz + row].x / ((double)np * (double)nz);
+}
+
+// FFT along z, per-plane 2D NFFT (N = nx = ny, the reference's n = 2
+// nextpow2(N) and 14 taps) at the tilt-range polar nodes, then per tilt the
+// inverse 2D FFT of the (nz x np) Fourier slice with the reference's shifts:
+// ifftshift of both axes going in (:362-367), fftshift of the columns coming
+// out (:376-390).  FP64 throughout.
 extern "C" int nudim_project_2d1d(int nx, int ny, int nz, int np, double gam_min, double gam_max, double interval,
                                   const double *density, double *proj) {
-  (void)nx; (void)ny; (void)nz; (void)np; (void)gam_min; (void)gam_max; (void)interval; (void)density; (void)proj;
-  return fail(NUDIM_EINVAL, "nudim_project_2d1d: not built yet");
+  API_BEGIN
+  NUDIM_CHECK(density && proj, "invalid argument");
+  NUDIM_CHECK(nx > 0 && ny > 0 && nz > 0 && np > 0 && interval > 0, "invalid dimensions or interval");
+  NUDIM_CHECK(nx == ny, "the 2D1D simulator here takes square x-y planes (nx == ny)");
+  NUDIM_CHECK(nx % 2 == 0 && nz % 2 == 0 && np % 2 == 0, "nx, nz and np must be even");
+  TmpStream ts;
+  const int N = nx, P = nz;
+  std::vector<double> x;
+  const int M = polar_nodes(gam_min, gam_max, interval, np, &x);
+  const int ns = M / np;
+  const long long V = (long long)nx * ny * nz;
+  DevBuf dd(V * 8), vol(V * 8), fh(V * sizeof(cx<double>)), f0(V * sizeof(cx<double>));
+  NUDIM_HIP(hipMemcpyAsync(dd.p, density, V * 8, hipMemcpyHostToDevice, ts.s));
+  hipLaunchKernelGGL(k_sim_volume, dim3(cdiv(V, 256)), dim3(256), 0, ts.s, dd.as<double>(), vol.as<double>(), nx, ny,
+                     nz);
+  NUDIM_HIP(hipGetLastError());
+  {
+    // z FFT, planes innermost: fh[y][x][p] = FFT_z(density)[(p + nz/2) mod nz]
+    FftDesc fdz = make_fft_desc(nz);
+    std::vector<double> t;
+    fft_twiddles(nz, &t);
+    DevBuf twz((size_t)nz * sizeof(cx<double>));
+    NUDIM_HIP(hipMemcpy(twz.p, t.data(), (size_t)nz * sizeof(cx<double>), hipMemcpyHostToDevice));
+    int Gs = line_chunk_log2(nz, 1 << 30, (int)sizeof(cx<double>), FftEpt<double>::value);
+    size_t lds = (size_t)nz * ((1 << Gs) + 1) * sizeof(cx<double>);
+    long long nlines = (long long)nx * ny;
+    hipLaunchKernelGGL(k_zfft_fwd<double>, dim3(cdiv(nlines, 1 << Gs)), dim3(kFftThreads), lds, ts.s,
+                       vol.as<double>(), fh.as<cx<double>>(), f0.as<cx<double>>(), fdz, twz.as<cx<double>>(), Gs,
+                       nlines);
+    NUDIM_HIP(hipGetLastError());
+  }
+  NUDIM_HIP(hipStreamSynchronize(ts.s));
+  dd.release();
+  vol.release();
+  f0.release();
+  // per-plane NFFT: f[m][p], node m = s*np + c
+  DevBuf f((size_t)M * P * sizeof(cx<double>)), g((size_t)M * P * sizeof(cx<double>));
+  {
+    NfftOp<double> op;
+    op.init(N, 2 * next_pow2(N), 14, x.data(), M, P);
+    op.trafo(fh.as<cx<double>>(), f.as<cx<double>>(), ts.s);
+    NUDIM_HIP(hipStreamSynchronize(ts.s));
+  }
+  fh.release();
+  // inverse FFT along z of each (s, c) line, input ifftshifted
+  LineFft<double> fz, fc;
+  fz.init(nz);
+  fc.init(np);
+  LineIO zi{nz, 1, nz, nz / 2, nullptr, nullptr}, zo{nz, 1, nz, 0, nullptr, nullptr};
+  fz.run(f.as<cx<double>>(), g.as<cx<double>>(), M, zi, zo, +1, 1, ts.s);
+  // inverse FFT along the columns of each tilt (rows = planes innermost),
+  // input ifftshifted, output fftshifted
+  const long long sso = (long long)np * nz;
+  LineIO ci{sso, nz, np, np / 2, nullptr, nullptr}, co{sso, nz, np, np / 2, nullptr, nullptr};
+  fc.run(g.as<cx<double>>(), f.as<cx<double>>(), ns, ci, co, +1, nz, ts.s);
+  DevBuf out((size_t)ns * nz * np * 8);
+  hipLaunchKernelGGL(k_sim_out, dim3(cdiv((long long)ns * nz * np, 256)), dim3(256), 0, ts.s, f.as<cx<double>>(),
+                     out.as<double>(), np, nz, ns);
+  NUDIM_HIP(hipGetLastError());
+  NUDIM_HIP(hipMemcpyAsync(proj, out.p, (size_t)ns * nz * np * 8, hipMemcpyDeviceToHost, ts.s));
+  NUDIM_HIP(hipStreamSynchronize(ts.s));
+  return NUDIM_OK;
+  API_END
 }

@@ -18,6 +18,10 @@
  * and exit code 1 (the reference dereferences a NULL FILE* or reads an
  * uninitialised enum); --random uses MT19937 seeded as GSL's default
  * generator (seed 0 -> 4357).
+ *
+ * Text files are read and written by several host threads (textio.c); the
+ * values parsed and the bytes written are those of the reference's
+ * fscanf / fprintf calls.
  */
 #include <getopt.h>
 #include <stdio.h>
@@ -25,6 +29,7 @@
 #include <string.h>
 
 #include "nudim.h"
+#include "textio.h"
 
 static void show_help(const char *s) {
   printf("\nSyntax: %s [options]\n\n", s);
@@ -211,22 +216,19 @@ int main(int argc, char *argv[]) {
   size_t per = (size_t)nx * nz;
   double *proj = (double *)malloc(sizeof(double) * per * nmeas);
   if (!proj) { fprintf(stderr, "out of memory\n"); return 1; }
+  const int io_threads = textio_threads();
   for (int j = 0; j < nmeas; j++) {
     char filename[1024];
     snprintf(filename, 1023, "./files/%s-%i.txt", prefix, j);
-    FILE *fp = fopen(filename, "r");
-    if (!fp) {
+    long got = textio_read_doubles(filename, proj + (size_t)j * per, (long)per, io_threads);
+    if (got < 0) {
       fprintf(stderr, "Error : Projection file %s doesn't exist!\n", filename);
       return 1;
     }
-    double *dst = proj + (size_t)j * per;
-    for (size_t i = 0; i < per; i++) {
-      if (fscanf(fp, "%le\n", &dst[i]) != 1) {
-        fprintf(stderr, "Error : Projection file %s holds fewer than %zu values!\n", filename, per);
-        return 1;
-      }
+    if ((size_t)got < per) {
+      fprintf(stderr, "Error : Projection file %s holds fewer than %zu values!\n", filename, per);
+      return 1;
     }
-    fclose(fp);
   }
   free(prefix);
 
@@ -238,10 +240,10 @@ int main(int argc, char *argv[]) {
   double *img = (double *)malloc(sizeof(double) * V);
   if (!img) { fprintf(stderr, "out of memory\n"); return 1; }
   if (nudim_get_volume(ctx, img) != 0) return fail_lib("nudim_get_volume");
-  FILE *fp = fopen("recovered-0.txt", "w");
-  if (!fp) { fprintf(stderr, "cannot write recovered-0.txt\n"); return 1; }
-  for (size_t i = 0; i < V; i++) fprintf(fp, "%le\n", img[i]);
-  fclose(fp);
+  if (textio_write_doubles("recovered-0.txt", img, (long)V, "%le\n", io_threads) != 0) {
+    fprintf(stderr, "cannot write recovered-0.txt\n");
+    return 1;
+  }
 
   double *before = NULL, *after = NULL;
   if (config_denoise && nhio + ner > 100) {
@@ -256,25 +258,27 @@ int main(int argc, char *argv[]) {
     if (nudim_iterate(ctx, 1, &r) != 0) return fail_lib("nudim_iterate");
     if (config_print) printf("Iteration: %d Measured Residual: %f\n", nn, r);
     if (nn == 100 && before) {
-      FILE *fb = fopen("grad_before.txt", "w"), *fa = fopen("grad_after.txt", "w");
-      if (!fb || !fa) { fprintf(stderr, "cannot write grad_before.txt / grad_after.txt\n"); return 1; }
-      for (size_t i = 0; i < V; i++) fprintf(fb, "%f\n", before[i]);
-      for (size_t i = 0; i < V; i++) fprintf(fa, "%f\n", after[i]);
-      fclose(fb);
-      fclose(fa);
+      if (textio_write_doubles("grad_before.txt", before, (long)V, "%f\n", io_threads) != 0 ||
+          textio_write_doubles("grad_after.txt", after, (long)V, "%f\n", io_threads) != 0) {
+        fprintf(stderr, "cannot write grad_before.txt / grad_after.txt\n");
+        return 1;
+      }
     }
   }
 
   if (nudim_get_volume(ctx, img) != 0) return fail_lib("nudim_get_volume");
-  fp = fopen("recovered.txt", "w");
-  if (!fp) { fprintf(stderr, "cannot write recovered.txt\n"); return 1; }
+  /* central nx x nx crop of every plane, in the reference's loop order */
+  size_t nc = 0;
   for (int i = 0; i < nz; i++)
     for (int j = 0; j < np; j++)
       for (int k = 0; k < np; k++) {
         if (j >= (np + nx) / 2 || j < (np - nx) / 2 || k >= (np + nx) / 2 || k < (np - nx) / 2) continue;
-        fprintf(fp, "%f\n", img[((size_t)i * np + j) * np + k]);
+        img[nc++] = img[((size_t)i * np + j) * np + k];
       }
-  fclose(fp);
+  if (textio_write_doubles("recovered.txt", img, (long)nc, "%f\n", io_threads) != 0) {
+    fprintf(stderr, "cannot write recovered.txt\n");
+    return 1;
+  }
 
   free(img);
   free(before);

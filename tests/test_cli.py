@@ -106,3 +106,98 @@ def test_cli_end_to_end(binary, orc, tmp_path, prec):
     assert np.max(np.abs(rec - crop)) < (1e-6 if prec == "fp64" else 1e-3 * np.abs(crop).max())
     printed = np.array([float(l.split()[-1]) for l in lines])
     assert np.max(np.abs(printed - rt)) < (1e-6 if prec == "fp64" else 1e-4)
+
+
+# ---- projection_sim ---------------------------------------------------------
+
+SIM = os.path.join(ROOT, "nudim_amd", "bin", "projection_sim")
+
+
+def run_sim(args, cwd):
+    return subprocess.run([SIM] + args, cwd=cwd, capture_output=True, text=True, timeout=600)
+
+
+@pytest.fixture(scope="module")
+def sim_binary(hip):
+    if not os.path.exists(SIM):
+        from nudim_amd import build
+        build.build_host()
+    return SIM
+
+
+# (arguments, message of src/projection_sim.c:204-245)
+SIM_BAD = [
+    (["--nx=8", "--ny=8", "--nz=8"], "You need to provide 3D density with --infile."),
+    (["-i", "d.txt", "--nx=8", "--nz=8"], "You need to provide valid dimensions with --nx, --ny, --nz."),
+    (["-i", "d.txt", "--nx=8", "--ny=8", "--nz=8"],
+     "You need to provide valid projection dimension in XY plane with --pad."),
+    (["-i", "d.txt", "--nx=8", "--ny=8", "--nz=8", "-p", "8"],
+     "You need to provide valid angular sampling interval with --interval."),
+    (["-i", "d.txt", "--nx=8", "--ny=8", "--nz=8", "-p", "8", "-t", "3"],
+     "You need to provide valid scope of tomogram angles (-90,90] with --max, --min."),
+    (["-i", "d.txt", "--nx=8", "--ny=8", "--nz=8", "-p", "8", "-t", "3", "--max=60", "--min=-60", "--noise"],
+     "You must provide valid signal to noise ratio with --snr or delete --noise."),
+    (["-i", "d.txt", "--nx=8", "--ny=8", "--nz=8", "-p", "8", "-t", "3", "--max=60", "--min=-60"],
+     "Error : Data file d.txt doesn't exist!"),
+    (["-i", "d.txt", "--nx=8", "--ny=8", "--nz=8", "-p", "8", "-t", "3", "--max=60", "--min=-60", "-m", "DFT"],
+     "is not built here"),
+    (["-i", "d.txt", "--nx=8", "--ny=6", "--nz=8", "-p", "8", "-t", "3", "--max=60", "--min=-60", "-m", "2D1D"],
+     "square even x-y planes"),
+    (["--ny=8y"], "Invalid Y dimension!"),
+    (["--snr=x"], "Invalid signal to noise ratio!"),
+]
+
+
+@pytest.mark.parametrize("args,msg", SIM_BAD)
+def test_sim_validation_messages(sim_binary, tmp_path, args, msg):
+    r = run_sim(args, tmp_path)
+    assert r.returncode == 1
+    assert msg in r.stderr
+
+
+def test_sim_help_and_default_method(sim_binary, tmp_path):
+    r = run_sim(["-h"], tmp_path)
+    assert r.returncode == 0 and "--snr" in r.stdout
+    # no --method: the reference's notice, then the density check
+    r = run_sim(["-i", "d.txt", "--nx=8", "--ny=8", "--nz=8", "-p", "8", "-t", "3", "--max=60", "--min=-60"],
+                tmp_path)
+    assert "using the 2D1D method, which is fastest." in r.stderr.replace("  ", " ")
+
+
+def test_sim_short_density_file(sim_binary, tmp_path):
+    np.savetxt(tmp_path / "d.txt", np.zeros(100), fmt="%le")
+    r = run_sim(["-i", "d.txt", "--nx=8", "--ny=8", "--nz=8", "-p", "8", "-t", "30", "--max=60", "--min=-60"],
+                tmp_path)
+    assert r.returncode == 1 and "holds 100 of the 512 values" in r.stderr
+
+
+@pytest.mark.gpu
+def test_sim_end_to_end(sim_binary, orc, tmp_path):
+    """density text map -> ./files/prj-<j>.txt ("%f"), against the oracle's
+    2D1D simulator on the same (text-rounded) density."""
+    n, nz, np_ = 24, 16, 32
+    rng = np.random.default_rng(33)
+    dens = blob_phantom((nz, n, n), 6, rng, sig=(1.5, 3.0), slab=12)
+    np.savetxt(tmp_path / "d.txt", dens.ravel(), fmt="%.17e")
+    (tmp_path / "files").mkdir()
+    r = run_sim(["-i", "d.txt", "--nx=%d" % n, "--ny=%d" % n, "--nz=%d" % nz, "-p", str(np_), "-t", "10",
+                 "--max=60", "--min=-60", "-m", "2D1D"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    ref = orc.project_2d1d(np.loadtxt(tmp_path / "d.txt").reshape(nz, n, n), np_, -60, 60, 10)
+    assert ref.shape[0] == 13
+    for j in range(ref.shape[0]):
+        got = np.loadtxt(tmp_path / "files" / ("prj-%d.txt" % j)).reshape(nz, np_)
+        # "%f": 6 decimals
+        assert np.max(np.abs(got - ref[j])) <= 5.1e-7
+    # with noise: the same files plus Gaussian noise of the stated level
+    (tmp_path / "noisy").mkdir()
+    (tmp_path / "noisy" / "files").mkdir()
+    np.savetxt(tmp_path / "noisy" / "d.txt", dens.ravel(), fmt="%.17e")
+    r = run_sim(["-i", "d.txt", "--nx=%d" % n, "--ny=%d" % n, "--nz=%d" % nz, "-p", str(np_), "-t", "10",
+                 "--max=60", "--min=-60", "--noise", "--snr=4"], tmp_path / "noisy")
+    assert r.returncode == 0, r.stderr
+    for j in (0, 6, 12):
+        clean = ref[j].ravel()
+        noisy = np.loadtxt(tmp_path / "noisy" / "files" / ("prj-%d.txt" % j))
+        ratio = np.std(noisy - clean) / np.std(clean)
+        assert 0.15 < ratio < 0.35  # sigma_noise = sigma_signal / snr (src/projection_sim.c:58-62)

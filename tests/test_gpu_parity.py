@@ -146,6 +146,24 @@ def test_cgnr_register_fft_sizes(hip, orc, N):
     assert rel(got32, ref) < 1e-4
 
 
+@pytest.mark.parametrize("n,nz,np_,t", [(32, 16, 48, 5), (24, 32, 32, 3), (16, 8, 16, 30)])
+def test_project_2d1d(hip, orc, n, nz, np_, t):
+    """GPU 2D1D simulator against the oracle's (same algorithm, FP64), and
+    against the reference's own brute-force DFT mode (which uses PI =
+    3.1415926 in its exponent: agreement to ~1e-7) on the smallest case."""
+    rng = np.random.default_rng(n + nz)
+    dens = blob_phantom((nz, n, n), 6, rng, sig=(1.5, 3.0), slab=n // 2)
+    got = hip.project_2d1d(dens, np_, -60, 60, t)
+    ref = orc.project_2d1d(dens, np_, -60, 60, t)
+    assert got.shape == ref.shape
+    assert rel(got, ref) < 1e-12
+    if n == 16:
+        assert rel(got, orc.project_dft(dens, np_, -60, 60, t)) < 1e-6
+        # 0 degrees: the line integral along y, centred in the row
+        zero = got[2]
+        assert np.max(np.abs(zero - dens.sum(axis=1))) < 1e-10
+
+
 def test_projection_ft(hip, orc):
     rng = np.random.default_rng(4)
     proj = rng.standard_normal((5, 16, 24))
