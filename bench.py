@@ -166,6 +166,72 @@ def probe(prm, proj, copy_mib):
     job.close()
 
 
+def serve(args, precision):
+    """Serving leg: a Poisson trace of jobs drawn from the serving mix (128^3
+    50 %, 256^3 35 %, 512^3 15 %; nhio=20 + ner=40 with TV), submitted at
+    their arrival times to the continuous-batching server (admission by HBM
+    budget, args.serve_slots jobs in flight).  Latency = finish - arrival."""
+    rng = np.random.default_rng(args.serve_seed)
+    names = [m[0] for m in phantom.SERVE_MIX]
+    frac = np.array([m[1] for m in phantom.SERVE_MIX])
+    cls = rng.choice(len(names), args.serve_jobs, p=frac / frac.sum())
+    gaps = rng.exponential(1.0 / args.serve_rate, args.serve_jobs)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    # a few distinct phantoms per class, cycled over the trace
+    inputs = {}
+    t0 = time.perf_counter()
+    for c in sorted(set(cls)):
+        inputs[c] = [phantom.config_workload(names[c], seed=7000 + 10 * c + k)
+                     for k in range(1 if names[c] == "s512" else 2)]
+    log("  serving: %d jobs (%s), inputs generated in %.1f s" % (
+        args.serve_jobs, ", ".join("%s x%d" % (names[c], int((cls == c).sum())) for c in sorted(set(cls))),
+        time.perf_counter() - t0))
+    reps = {}
+    with nd.Server(max_active=args.serve_slots) as srv:
+        meta = {}
+        start = time.perf_counter()
+        k = 0
+        while k < args.serve_jobs or len(reps) < args.serve_jobs:
+            now = time.perf_counter() - start
+            while k < args.serve_jobs and arrivals[k] <= now:
+                prm, proj = inputs[cls[k]][k % len(inputs[cls[k]])]
+                p = nd.make_params(prm["nx"], prm["nz"], prm["np_"], prm["gam_min"], prm["gam_max"],
+                                   prm["interval"], padangle=prm["padangle"], weight=prm["weight"],
+                                   nhio=prm["nhio"], ner=prm["ner"], denoise=True, method=nd.TV,
+                                   height=prm["height"], center=prm["center"], precision=precision)
+                jid = srv.submit(p, proj)
+                meta[jid] = (names[cls[k]], prm["nhio"] + prm["ner"])
+                k += 1
+            wait_ms = 50 if k >= args.serve_jobs else max(1, int(1e3 * (arrivals[k] - now)))
+            got = srv.next(timeout_ms=min(wait_ms, 50))
+            if got is not None:
+                rep = got[0]
+                if rep["status"] != 0:
+                    raise RuntimeError("serving job failed: %s" % rep["error"])
+                reps[rep["job_id"]] = rep
+    lat = np.array([r["t_end"] - r["t_submit"] for r in reps.values()])
+    t_first = min(r["t_submit"] for r in reps.values())
+    t_last = max(r["t_end"] for r in reps.values())
+    span = t_last - t_first
+    passes = sum(meta[j][1] for j in reps)
+    by = {}
+    for j, r in reps.items():
+        by.setdefault(meta[j][0], []).append(r["t_end"] - r["t_submit"])
+    out = {"jobs": len(reps), "arrival_rate_jobs_per_s": args.serve_rate, "slots": args.serve_slots,
+           "offered_jobs_per_s": args.serve_jobs / max(arrivals[-1], 1e-9) if args.serve_jobs > 1 else None,
+           "achieved_jobs_per_s": len(reps) / span, "volume_iters_per_s": passes / span,
+           "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+           "mean_latency_s": float(lat.mean()), "makespan_s": span,
+           "p50_latency_s_by_class": {k: float(np.percentile(v, 50)) for k, v in sorted(by.items())},
+           "precision": "f32" if precision == nd.FP32 else "f64",
+           "mix": "%s; seed %d" % (", ".join("%s %.0f%%" % (n, 100 * f) for n, f in phantom.SERVE_MIX),
+                                   args.serve_seed)}
+    log("  serving %s: %d jobs in %.1f s: p50 %.2f s, p99 %.2f s, %.1f volume-iter/s, achieved %.2f jobs/s "
+        "(offered %.2f)" % (out["precision"], len(reps), span, out["p50_latency_s"], out["p99_latency_s"],
+                            out["volume_iters_per_s"], out["achieved_jobs_per_s"], out["offered_jobs_per_s"] or 0))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -177,6 +243,12 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=12, help="jobs in the serving trace (0: no serving leg)")
+    ap.add_argument("--serve-rate", type=float, default=4.0, help="mean arrivals per second (Poisson)")
+    ap.add_argument("--serve-slots", type=int, default=4, help="jobs in flight at once")
+    ap.add_argument("--serve-seed", type=int, default=5)
+    ap.add_argument("--serve-fp64", action="store_true", help="also serve the trace on the plain FP64 path")
+    ap.add_argument("--serve-only", action="store_true", help="run only the serving leg")
     ap.add_argument("--probe", type=int, default=0, metavar="MIB",
                     help="run one FP32 pass and a MIB-MiB device copy, then exit (for counter runs)")
     args = ap.parse_args()
@@ -189,6 +261,12 @@ def main():
     log("workload %s: projections %s generated in %.1f s" % (args.config, proj.shape, time.perf_counter() - t0))
     if args.probe:
         probe(prm, proj, args.probe)
+        return
+    if args.serve_only:
+        out = {"serving": serve(args, nd.FP32)}
+        if args.serve_fp64:
+            out["serving_fp64"] = serve(args, nd.FP64_PLAIN)
+        print(json.dumps(out), flush=True)
         return
 
     log("FP32 batched form:")
@@ -262,6 +340,8 @@ def main():
         log("  ER without TV, %d passes: rel L2 FP32 vs FP64 %.2e, R drift %.2e"
             % (npass, acc["er_no_denoise"]["rel_l2"], acc["er_no_denoise"]["max_abs_rfactor_drift"]))
 
+    serving = serve(args, nd.FP32) if args.serve_jobs > 0 else None
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -294,6 +374,7 @@ def main():
                      "avg_us": kd["avg_us"]},
         "kernels": {KC_NAMES.get(k): {kk: round(vv, 4) for kk, vv in v.items() if kk != "name"}
                     for k, v in kern.items()},
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

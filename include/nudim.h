@@ -26,7 +26,7 @@ extern "C" {
 #endif
 
 enum { NUDIM_OK = 0, NUDIM_EINVAL = -1, NUDIM_ENOMEM = -2, NUDIM_EHIP = -3,
-       NUDIM_ESTATE = -4, NUDIM_EBUDGET = -5 };
+       NUDIM_ESTATE = -4, NUDIM_EBUDGET = -5, NUDIM_EAGAIN = -6 };
 
 /* arithmetic of the device path */
 enum { NUDIM_FP64_PLAIN = 0, /* FP64 everywhere, literal NFFT-based CGNR, reference NFFT sizes */
@@ -142,6 +142,39 @@ int nudim_projection_ft(int precision, int nx, int nz, int np, int nmeas, const 
  * creal(slice)/(np*nz), nslices = (int)((gam_max-gam_min)/interval+0.5)+1 */
 int nudim_project_2d1d(int nx, int ny, int nz, int np, double gam_min, double gam_max,
                        double interval, const double *density, double *proj);
+
+/* ---- serving: continuous batching of reconstruction jobs ----
+ * (No reference counterpart: the reference runs one job per process,
+ * src/reconstruction.c main.)  Jobs are admitted first fit in arrival order
+ * while their device memory (nudim_estimate_bytes) fits the budget; each
+ * runs on its own stream and host slot thread, so jobs overlap on the
+ * device.  A job's result does not depend on what else runs. */
+typedef struct nudim_server nudim_server;
+typedef struct {
+  long long hbm_budget; /* bytes the admitted jobs may hold; <= 0: 90% of free device memory */
+  int max_active;       /* jobs in flight at once (slot threads), >= 1 */
+  int device;
+} nudim_server_opts;
+typedef struct {
+  long long job_id;
+  int status;                      /* NUDIM_OK or the failing call's code */
+  double t_submit, t_start, t_end; /* seconds since the server was created */
+  long long device_bytes;          /* the admission estimate */
+  char error[256];
+} nudim_job_report;
+int nudim_server_create(const nudim_server_opts *opts, nudim_server **out);
+/* Copies proj [nmeas][nz][nx]; volume [nz][np][np] and rfactors [nhio+ner]
+ * (or NULL) must stay valid until the job's report is returned.  A job whose
+ * estimate exceeds the whole budget is refused with NUDIM_EBUDGET. */
+int nudim_server_submit(nudim_server *s, const nudim_params *p, const double *proj, double *volume,
+                        double *rfactors, long long *job_id);
+/* the next finished job, in completion order; timeout_ms < 0 waits for one,
+ * NUDIM_EAGAIN when none finished within timeout_ms */
+int nudim_server_next(nudim_server *s, int timeout_ms, nudim_job_report *report);
+int nudim_server_stats(nudim_server *s, int *pending, int *active, long long *bytes_in_use);
+/* runs the pending and running jobs to completion, then frees the server
+ * (reports not yet taken are dropped) */
+void nudim_server_destroy(nudim_server *s);
 
 #ifdef __cplusplus
 }

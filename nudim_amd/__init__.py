@@ -50,7 +50,22 @@ EXPORTS = [
     "nudim_estimate_bytes", "nudim_synchronize", "nudim_profile", "nudim_profile_read", "nudim_nfft_trafo", "nudim_nfft_adjoint",
     "nudim_cgnr", "nudim_zfft", "nudim_tv", "nudim_tv_height", "nudim_diffusion2d",
     "nudim_constraint", "nudim_projection_ft", "nudim_project_2d1d",
+    "nudim_server_create", "nudim_server_submit", "nudim_server_next", "nudim_server_stats",
+    "nudim_server_destroy",
 ]
+EAGAIN = -6
+
+
+class ServerOpts(C.Structure):
+    """nudim_server_opts (include/nudim.h)"""
+    _fields_ = [("hbm_budget", C.c_longlong), ("max_active", C.c_int), ("device", C.c_int)]
+
+
+class JobReport(C.Structure):
+    """nudim_job_report (include/nudim.h)"""
+    _fields_ = [("job_id", C.c_longlong), ("status", C.c_int), ("t_submit", C.c_double),
+                ("t_start", C.c_double), ("t_end", C.c_double), ("device_bytes", C.c_longlong),
+                ("error", C.c_char * 256)]
 
 
 def lib():
@@ -95,6 +110,11 @@ def _declare(L):
         "nudim_constraint": (i, [i, dp, dp, i, i, i, i, i, i, i]),
         "nudim_projection_ft": (i, [i, i, i, i, i, dp, dp]),
         "nudim_project_2d1d": (i, [i, i, i, i, d, d, d, dp, dp]),
+        "nudim_server_create": (i, [C.POINTER(ServerOpts), C.POINTER(v)]),
+        "nudim_server_submit": (i, [v, C.POINTER(Params), dp, dp, dp, C.POINTER(C.c_longlong)]),
+        "nudim_server_next": (i, [v, i, C.POINTER(JobReport)]),
+        "nudim_server_stats": (i, [v, C.POINTER(i), C.POINTER(i), C.POINTER(C.c_longlong)]),
+        "nudim_server_destroy": (None, [v]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -326,3 +346,63 @@ def _np2(n):
     while p < n:
         p <<= 1
     return p
+
+
+class Server:
+    """Continuous-batching job server (nudim_server_*): jobs of mixed size are
+    admitted first fit in arrival order under an HBM budget and run
+    concurrently, one stream and host slot thread each."""
+
+    def __init__(self, max_active=4, hbm_budget=0, device=0):
+        o = ServerOpts(hbm_budget, max_active, device)
+        h = C.c_void_p()
+        _check(lib().nudim_server_create(C.byref(o), C.byref(h)))
+        self._h = h
+        self._jobs = {}
+
+    def submit(self, params, proj):
+        """Queue a job; returns its id.  The volume and R trace are returned
+        by next() when it finishes."""
+        n = params.nz * params.np * params.np
+        vol = np.zeros(n)
+        rf = np.zeros(max(params.nhio + params.ner, 1))
+        proj = _f64(proj)
+        jid = C.c_longlong()
+        _check(lib().nudim_server_submit(self._h, C.byref(params), _ptr(proj), _ptr(vol), _ptr(rf), C.byref(jid)))
+        self._jobs[jid.value] = (vol, rf, params)
+        return jid.value
+
+    def next(self, timeout_ms=-1):
+        """(report dict, volume [nz][np][np], R trace) of the next finished
+        job, or None after timeout_ms without one."""
+        r = JobReport()
+        rc = lib().nudim_server_next(self._h, timeout_ms, C.byref(r))
+        if rc == EAGAIN:
+            return None
+        _check(rc)
+        vol, rf, p = self._jobs.pop(r.job_id)
+        rep = {"job_id": r.job_id, "status": r.status, "t_submit": r.t_submit, "t_start": r.t_start,
+               "t_end": r.t_end, "device_bytes": r.device_bytes, "error": r.error.decode(errors="replace")}
+        return rep, vol.reshape(p.nz, p.np, p.np), rf[: p.nhio + p.ner]
+
+    def stats(self):
+        pend, act, used = C.c_int(), C.c_int(), C.c_longlong()
+        _check(lib().nudim_server_stats(self._h, C.byref(pend), C.byref(act), C.byref(used)))
+        return {"pending": pend.value, "active": act.value, "bytes_in_use": used.value}
+
+    def close(self):
+        if self._h:
+            lib().nudim_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -29,6 +29,8 @@ static int fail(int code, const std::string &msg) {
 
 extern "C" const char *nudim_last_error(void) { return g_last_error.c_str(); }
 
+void nudim::set_last_error(const std::string &msg) { g_last_error = msg; }
+
 // ---------------------------------------------------------------------------
 // device kernels of the set-up stage
 // ---------------------------------------------------------------------------

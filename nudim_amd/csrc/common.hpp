@@ -42,6 +42,9 @@ __host__ __device__ __forceinline__ cx<T> mul_i(cx<T> a, int sign) {
   return sign < 0 ? mk(a.y, -a.x) : mk(-a.y, a.x);
 }
 
+// the calling thread's message for nudim_last_error (api.hip)
+void set_last_error(const std::string &msg);
+
 struct Error : std::runtime_error {
   int code;
   Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}

@@ -63,7 +63,8 @@ struct ToeplitzOp {
   LineFft<T> fft;
   DevBuf khat;  // cx<T>[L][L], stored [column][row], scaled by 1/L^2
   DevBuf khat_re;  // T[L][L]: its real part (the kernel is Hermitian, so khat is real)
-  DevBuf X;     // [N][L][P]
+  DevBuf X;     // [N][L][P]; for the fused passes [P/pc][N][L][pc]
+  int pc = 0;   // planes per chunk of X in the fused passes
   size_t bytes() const;
   void init(int N, const double *x, const double *w, int M, int P, hipStream_t st);
   void apply(const cx<T> *p, cx<T> *q, hipStream_t st);

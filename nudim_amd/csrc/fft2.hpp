@@ -70,11 +70,29 @@ constexpr CS cis2pi(long e, long R) {
   }
 }
 
+// The R roots of unity exp(S 2 pi i e / R) of one DFT length in the
+// transform's precision, evaluated while compiling.  A function-local
+// constexpr table is a private constant, so a read at an index known after
+// unrolling folds to an immediate; evaluating cis2pi in the kernel body
+// instead leaves FP64 series code in it once the unrolled DFT is large.
+template <typename T, int S, int R>
+struct TwTable {
+  T c[R], s[R];
+  constexpr TwTable() : c(), s() {
+    for (int e = 0; e < R; e++) {
+      CS v = cis2pi(e, R);
+      c[e] = (T)v.c;
+      s[e] = (T)(S * v.s);
+    }
+  }
+};
+
 // w_R^e = exp(S 2 pi i e / R)
-template <typename T, int S>
-__device__ __forceinline__ cx<T> wconst(long e, long R) {
-  CS v = cis2pi(e, R);
-  return mk((T)v.c, (T)(S * v.s));
+template <typename T, int S, int R>
+__device__ __forceinline__ cx<T> wconst(int e) {
+  constexpr TwTable<T, S, R> tab{};
+  const int i = ((e % R) + R) % R;
+  return mk(tab.c[i], tab.s[i]);
 }
 
 constexpr int split_factor(int R) {
@@ -101,7 +119,7 @@ __device__ __forceinline__ void dft_reg(cx<T> *v) {
       for (int j1 = 0; j1 < A; j1++) a[j1] = v[j2 + B * j1];
       dft_reg<T, A, S>(a);
 #pragma unroll
-      for (int k1 = 0; k1 < A; k1++) t[j2 + B * k1] = (j2 * k1 == 0) ? a[k1] : a[k1] * wconst<T, S>(j2 * k1, R);
+      for (int k1 = 0; k1 < A; k1++) t[j2 + B * k1] = (j2 * k1 == 0) ? a[k1] : a[k1] * wconst<T, S, R>(j2 * k1);
     }
 #pragma unroll
     for (int k1 = 0; k1 < A; k1++) {
@@ -126,7 +144,7 @@ __device__ __forceinline__ void dft_half_in(cx<T> *v) {
 #pragma unroll
   for (int j = 0; j < H; j++) {
     e[j] = v[j];
-    o[j] = (j == 0) ? v[0] : v[j] * wconst<T, S>(j, R);
+    o[j] = (j == 0) ? v[0] : v[j] * wconst<T, S, R>(j);
   }
   dft_reg<T, H, S>(e);
   dft_reg<T, H, S>(o);
@@ -151,7 +169,7 @@ __device__ __forceinline__ void dft_half_out(cx<T> *v) {
   dft_reg<T, H, S>(e);
   dft_reg<T, H, S>(o);
 #pragma unroll
-  for (int m = 0; m < H; m++) v[m] = e[m] + ((m == 0) ? o[0] : o[m] * wconst<T, S>(m, R));
+  for (int m = 0; m < H; m++) v[m] = e[m] + ((m == 0) ? o[0] : o[m] * wconst<T, S, R>(m));
 }
 
 // Exchange of the four-step transform through LDS.  SPLIT: through a real

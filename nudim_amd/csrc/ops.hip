@@ -117,7 +117,7 @@ static void upload(DevBuf &b, const std::vector<U> &v) {
 
 int line_chunk_log2(int n, int P, int elem_bytes, int ept) {
   int cap = (kFftThreads * ept) / n;  // register stash bound
-  int lds_cap = (64 * 1024) / (n * elem_bytes) - 1;  // lines + one twiddle table
+  int lds_cap = (160 * 1024) / (n * elem_bytes) - 1;  // lines + one twiddle table in 160 KB (gfx950)
   cap = std::min(cap, lds_cap);
   cap = std::min(cap, 16);
   NUDIM_CHECK(cap >= 1, "line transform of length " + std::to_string(n) + " does not fit in LDS");
@@ -373,6 +373,11 @@ void ToeplitzOp<T>::init(int N_, const double *x, const double *w, int M, int P_
   NUDIM_HIP(hipGetLastError());
   NUDIM_HIP(hipStreamSynchronize(st));
   X.alloc((size_t)N * L * P * sizeof(cx<T>));
+  // The column pass walks X's rows with stride L * pc; keep a row of a
+  // chunk within 2 MB (one large page) so that a block's rows stay within
+  // TLB reach: chunks of pc planes, a power of two dividing P, at least 64.
+  pc = P;
+  while ((size_t)L * pc * sizeof(cx<T>) > ((size_t)2 << 20) && pc % 2 == 0 && pc / 2 >= 64) pc /= 2;
 }
 
 template <typename T>
@@ -390,24 +395,36 @@ void ToeplitzOp<T>::apply(const cx<T> *p, cx<T> *q, hipStream_t st) {
 
 template <typename T>
 void ToeplitzOp<T>::apply_cg(const ToepCg<T> &cg, cx<T> *q, hipStream_t st) {
-  const long long NP = (long long)N * P, LP = (long long)L * P;
-  cx<T> *x = X.as<cx<T>>();
+  const long long NP = (long long)N * P, LC = (long long)L * pc;
   const cx<T> *tw = fft.tw.template as<cx<T>>();
-  {
-    // algorithmic bytes: z read, p written, and with the recurrence p read,
-    // f read and written (N x N each); X written (N x L)
-    const double nvec = 2.0 + (cg.zz_old ? 1.0 : 0.0) + (cg.alpha ? 2.0 : 0.0);
-    ProfScope ps(KC_TOEPLITZ_ROWF, st, (nvec * N + L) * N * P * sizeof(cx<T>));
-    toep_launch<T>(TK_ROWF, L, N, nullptr, x, NP, P, LP, P, P, st, tw, nullptr, cg, P);
-  }
-  {
-    ProfScope ps(KC_TOEPLITZ_COL, st, 2.0 * N * L * P * sizeof(cx<T>) + (double)L * L * sizeof(T));
-    toep_launch<T>(TK_COL, L, L, x, x, P, LP, P, LP, P, st, tw, khat_re.template as<T>(), cg, P);
-  }
-  {
-    // algorithmic bytes: X read (N x L), p read, q written (N x N each)
-    ProfScope ps(KC_TOEPLITZ_ROWI, st, (2.0 * N + L) * N * P * sizeof(cx<T>));
-    toep_launch<T>(TK_ROWI, L, N, x, q, LP, P, NP, P, P, st, tw, nullptr, cg, P);
+  const T *kr = khat_re.template as<T>();
+  // one chunk of pc planes at a time; X of the chunk is [N][L][pc]
+  for (int p0 = 0; p0 < P; p0 += pc) {
+    cx<T> *x = X.as<cx<T>>() + (long long)p0 * N * L;
+    ToepCg<T> c = cg;
+    c.z += p0;
+    c.p += p0;
+    if (c.f) c.f += p0;
+    c.zz += p0;
+    if (c.zz_old) c.zz_old += p0;
+    if (c.alpha) c.alpha += p0;
+    c.part += p0;
+    {
+      // algorithmic bytes: z read, p written, and with the recurrence p read,
+      // f read and written (N x N each); X written (N x L)
+      const double nvec = 2.0 + (cg.zz_old ? 1.0 : 0.0) + (cg.alpha ? 2.0 : 0.0);
+      ProfScope ps(KC_TOEPLITZ_ROWF, st, (nvec * N + L) * N * pc * sizeof(cx<T>));
+      toep_launch<T>(TK_ROWF, L, N, nullptr, x, NP, P, LC, pc, pc, st, tw, nullptr, c, P);
+    }
+    {
+      ProfScope ps(KC_TOEPLITZ_COL, st, 2.0 * N * L * pc * sizeof(cx<T>) + (double)L * L * sizeof(T));
+      toep_launch<T>(TK_COL, L, L, x, x, pc, LC, pc, LC, pc, st, tw, kr, c, P);
+    }
+    {
+      // algorithmic bytes: X read (N x L), p read, q written (N x N each)
+      ProfScope ps(KC_TOEPLITZ_ROWI, st, (2.0 * N + L) * N * pc * sizeof(cx<T>));
+      toep_launch<T>(TK_ROWI, L, N, x, q + p0, LC, pc, NP, P, pc, st, tw, nullptr, c, P);
+    }
   }
   NUDIM_HIP(hipGetLastError());
 }

@@ -1,0 +1,235 @@
+// serve.hip -- continuous-batching job server over the job API.
+//
+// The reference runs one reconstruction per process (src/reconstruction.c
+// main).  Here a server admits jobs of mixed size while others run: every
+// job gets a slot thread and its own stream, so one job's host set-up
+// (geometry and window tables) and transfers overlap other jobs' kernels,
+// and small jobs' kernels overlap large ones'.  Admission is first fit in
+// arrival order against an HBM budget (nudim_estimate_bytes per job).  A
+// job's arithmetic does not depend on what else runs: its result is the same
+// alone and in any mix.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <memory>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "common.hpp"
+#include "nudim.h"
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+struct ServerJob {
+  long long id = 0;
+  nudim_params p{};
+  std::vector<double> proj;
+  double *volume = nullptr, *rfactors = nullptr;
+  size_t bytes = 0;
+  double t_submit = 0, t_start = 0, t_end = 0;
+  int status = NUDIM_OK;
+  std::string error;
+};
+
+}  // namespace
+
+struct nudim_server {
+  nudim_server_opts o{};
+  clk::time_point epoch;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<ServerJob *> pending;
+  std::deque<ServerJob *> done;
+  std::vector<std::thread> slots;
+  size_t used = 0;
+  long long next_id = 1;
+  int active = 0;
+  bool stop = false;
+
+  double now() const { return std::chrono::duration<double>(clk::now() - epoch).count(); }
+
+  // first pending job (arrival order) that fits the free budget; caller holds mu
+  ServerJob *take() {
+    for (auto it = pending.begin(); it != pending.end(); ++it) {
+      if (used + (*it)->bytes <= (size_t)o.hbm_budget) {
+        ServerJob *j = *it;
+        pending.erase(it);
+        used += j->bytes;
+        active++;
+        return j;
+      }
+    }
+    return nullptr;
+  }
+
+  void run(ServerJob *j) {
+    nudim_ctx *c = nullptr;
+    auto check = [&](int rc) {
+      if (rc != NUDIM_OK && j->status == NUDIM_OK) {
+        j->status = rc;
+        j->error = nudim_last_error();
+      }
+      return rc == NUDIM_OK;
+    };
+    if (check(nudim_create(&j->p, &c)) && check(nudim_load_projections(c, j->proj.data())) &&
+        check(nudim_initial(c))) {
+      const int npass = j->p.nhio + j->p.ner;
+      std::vector<double> r(npass > 0 ? npass : 1);
+      if (check(nudim_iterate(c, npass, r.data())) && check(nudim_get_volume(c, j->volume)) && j->rfactors)
+        std::memcpy(j->rfactors, r.data(), sizeof(double) * npass);
+    }
+    nudim_destroy(c);
+    std::vector<double>().swap(j->proj);
+  }
+
+  void slot_main() {
+    for (;;) {
+      ServerJob *j = nullptr;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        // a stopping server still runs what is pending
+        cv_work.wait(lk, [&] { return (j = take()) != nullptr || (stop && pending.empty()); });
+        if (!j) return;
+      }
+      j->t_start = now();
+      run(j);
+      j->t_end = now();
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        used -= j->bytes;
+        active--;
+        done.push_back(j);
+      }
+      cv_done.notify_all();
+      cv_work.notify_all();
+    }
+  }
+};
+
+extern "C" int nudim_server_create(const nudim_server_opts *o, nudim_server **out) {
+  try {
+    if (!o || !out || o->max_active < 1) {
+      nudim::set_last_error("invalid server options");
+      return NUDIM_EINVAL;
+    }
+    *out = nullptr;
+    std::unique_ptr<nudim_server> s(new nudim_server());
+    s->o = *o;
+    if (s->o.hbm_budget <= 0) {
+      if (hipSetDevice(o->device) != hipSuccess) {
+        nudim::set_last_error("hipSetDevice failed");
+        return NUDIM_EHIP;
+      }
+      size_t fr = 0, tot = 0;
+      if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+        nudim::set_last_error("hipMemGetInfo failed");
+        return NUDIM_EHIP;
+      }
+      s->o.hbm_budget = (long long)(0.9 * (double)fr);
+    }
+    s->epoch = clk::now();
+    for (int i = 0; i < o->max_active; i++) s->slots.emplace_back([p = s.get()] { p->slot_main(); });
+    *out = s.release();
+    return NUDIM_OK;
+  } catch (const std::exception &e) {
+    nudim::set_last_error(e.what());
+    return NUDIM_ENOMEM;
+  }
+}
+
+extern "C" int nudim_server_submit(nudim_server *s, const nudim_params *p, const double *proj, double *volume,
+                                   double *rfactors, long long *job_id) {
+  try {
+    if (!s || !p || !proj || !volume) {
+      nudim::set_last_error("invalid argument");
+      return NUDIM_EINVAL;
+    }
+    size_t bytes = 0;
+    int rc = nudim_estimate_bytes(p, &bytes);
+    if (rc != NUDIM_OK) return rc;
+    if ((long long)bytes > s->o.hbm_budget) {
+      nudim::set_last_error("job needs " + std::to_string(bytes) + " bytes of device memory, over the server budget");
+      return NUDIM_EBUDGET;
+    }
+    // the tilt count the job reads (reconstruction.c:304)
+    const int nmeas = (int)((p->gam_max - p->gam_min) / p->interval + 0.5) + 1;
+    std::unique_ptr<ServerJob> j(new ServerJob());
+    j->p = *p;
+    j->proj.assign(proj, proj + (size_t)nmeas * p->nz * p->nx);
+    j->volume = volume;
+    j->rfactors = rfactors;
+    j->bytes = bytes;
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      if (s->stop) {
+        nudim::set_last_error("server is shutting down");
+        return NUDIM_ESTATE;
+      }
+      j->id = s->next_id++;
+      j->t_submit = s->now();
+      if (job_id) *job_id = j->id;
+      s->pending.push_back(j.release());
+    }
+    s->cv_work.notify_all();
+    return NUDIM_OK;
+  } catch (const std::exception &e) {
+    nudim::set_last_error(e.what());
+    return NUDIM_ENOMEM;
+  }
+}
+
+extern "C" int nudim_server_next(nudim_server *s, int timeout_ms, nudim_job_report *rep) {
+  if (!s || !rep) {
+    nudim::set_last_error("invalid argument");
+    return NUDIM_EINVAL;
+  }
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto ready = [&] { return !s->done.empty(); };
+  if (timeout_ms < 0) s->cv_done.wait(lk, ready);
+  else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready)) return NUDIM_EAGAIN;
+  ServerJob *j = s->done.front();
+  s->done.pop_front();
+  lk.unlock();
+  std::memset(rep, 0, sizeof(*rep));
+  rep->job_id = j->id;
+  rep->status = j->status;
+  rep->t_submit = j->t_submit;
+  rep->t_start = j->t_start;
+  rep->t_end = j->t_end;
+  rep->device_bytes = (long long)j->bytes;
+  std::strncpy(rep->error, j->error.c_str(), sizeof(rep->error) - 1);
+  delete j;
+  return NUDIM_OK;
+}
+
+extern "C" int nudim_server_stats(nudim_server *s, int *pending, int *active, long long *bytes_in_use) {
+  if (!s) {
+    nudim::set_last_error("invalid argument");
+    return NUDIM_EINVAL;
+  }
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (pending) *pending = (int)s->pending.size();
+  if (active) *active = s->active;
+  if (bytes_in_use) *bytes_in_use = (long long)s->used;
+  return NUDIM_OK;
+}
+
+extern "C" void nudim_server_destroy(nudim_server *s) {
+  if (!s) return;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  for (auto &t : s->slots) t.join();
+  for (ServerJob *j : s->pending) delete j;
+  for (ServerJob *j : s->done) delete j;
+  delete s;
+}

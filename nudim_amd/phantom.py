@@ -94,6 +94,7 @@ def config_workload(cfg, seed=None):
     c2: c1 phantom, SNR=1, nhio=20 + ner=20, TV with height 64 (centre 64).
     c3: 256^3, 800 blobs, slab 128, np=384, -70..70 step 2, SNR=2, nhio=30 + ner=50,
         TV with height 128 (centre 192); reconstruction grid 384 x 384 x 256.
+    s128, s256, s512: the serving mix (SERVE_CLASSES).
     """
     if cfg == "c1":
         bp = blob_params(200, (128, 128, 128), 64, 1000 if seed is None else seed)
@@ -110,6 +111,23 @@ def config_workload(cfg, seed=None):
         proj = add_noise(projections(bp, 384, -70, 70, 2), 2.0, 3001)
         prm = dict(nx=384, nz=256, np_=384, gam_min=-70, gam_max=70, interval=2, padangle=1.0, weight=True,
                    nhio=30, ner=50, denoise=True, height=128, center=192)
+    elif cfg in SERVE_CLASSES:
+        n, np_, nblobs, g, t, nmax = SERVE_CLASSES[cfg]
+        bp = blob_params(nblobs, (n, n, n), n // 2, 5000 + n if seed is None else seed)
+        proj = add_noise(projections(bp, np_, -g, g, t), 2.0, (5001 + n if seed is None else seed + 1))
+        prm = dict(nx=np_, nz=n, np_=np_, gam_min=-g, gam_max=g, interval=t, padangle=1.0, weight=True,
+                   nhio=20, ner=40, denoise=True, height=n // 2, center=np_ // 2)
     else:
         raise ValueError(cfg)
     return prm, proj
+
+
+# Job classes of the serving mix (128^3, 256^3, 512^3 blob phantoms, np = 1.5 n,
+# SNR 2, nhio=20 + ner=40 with TV over the slab): name -> (n, np, blobs,
+# tilt range +-g, step, -)
+SERVE_CLASSES = {
+    "s128": (128, 192, 200, 60, 3, 0),
+    "s256": (256, 384, 800, 70, 2, 0),
+    "s512": (512, 768, 3200, 60, 1, 0),
+}
+SERVE_MIX = (("s128", 0.50), ("s256", 0.35), ("s512", 0.15))

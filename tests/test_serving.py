@@ -1,0 +1,79 @@
+"""Job server (nudim_server_*): continuous batching under an HBM budget.
+
+Each job runs on its own stream with its own buffers and fixed-order
+reductions, so its result must be bit-identical alone and in a mix."""
+import numpy as np
+import pytest
+
+from tests.util import blob_phantom
+
+pytestmark = pytest.mark.gpu
+
+
+def job_inputs(hip, n, nz, seed, interval=5):
+    rng = np.random.default_rng(seed)
+    dens = blob_phantom((nz, n, n), 8, rng, sig=(1.5, 3.0), slab=n // 2)
+    return hip.project_2d1d(dens, n, -60, 60, interval)
+
+
+def job_params(hip, n, nz, prec, denoise, method, interval=5):
+    return hip.make_params(n, nz, n, -60, 60, interval, padangle=1.0, weight=True, nhio=2, ner=2,
+                           denoise=denoise, method=method, height=n // 2, center=n // 2, precision=prec)
+
+
+# (n = np, nz, precision, denoise, method); DIFFUSION needs nz == np
+CASES = [(32, 32, 1, True, 1), (48, 48, 0, True, 0), (64, 64, 1, False, 1), (32, 48, 0, True, 1),
+         (64, 64, 0, False, 1), (48, 32, 1, True, 1)]
+
+
+def alone(hip, p, proj):
+    with hip.Reconstruction(p) as job:
+        job.load_projections(proj)
+        job.initial()
+        r = job.iterate(p.nhio + p.ner)
+        return job.volume(), r
+
+
+def test_alone_and_in_mix_identical(hip):
+    jobs = []
+    for k, (n, nz, prec, den, meth) in enumerate(CASES):
+        p = job_params(hip, n, nz, prec, den, meth)
+        proj = job_inputs(hip, n, nz, 100 + k)
+        jobs.append((p, proj, alone(hip, p, proj)))
+    with hip.Server(max_active=4) as srv:
+        ids = {srv.submit(p, proj): k for k, (p, proj, _) in enumerate(jobs)}
+        seen = set()
+        for _ in jobs:
+            rep, vol, rf = srv.next(timeout_ms=120000)
+            assert rep["status"] == 0, rep["error"]
+            k = ids[rep["job_id"]]
+            seen.add(k)
+            v_alone, r_alone = jobs[k][2]
+            assert np.array_equal(vol, v_alone), "job %d differs in the mix" % k
+            assert np.array_equal(rf, r_alone)
+            assert rep["t_submit"] <= rep["t_start"] <= rep["t_end"]
+        assert seen == set(range(len(jobs)))
+
+
+def test_budget_serialises_admission(hip):
+    p = job_params(hip, 32, 32, 1, False, 1)
+    proj = job_inputs(hip, 32, 32, 7)
+    need = hip.estimate_bytes(p)
+    # room for one job at a time: every job starts after the previous ended
+    with hip.Server(max_active=4, hbm_budget=int(need * 1.5)) as srv:
+        for _ in range(3):
+            srv.submit(p, proj)
+        reps = [srv.next(timeout_ms=120000)[0] for _ in range(3)]
+    reps.sort(key=lambda r: r["t_start"])
+    for a, b in zip(reps, reps[1:]):
+        assert b["t_start"] >= a["t_end"] - 1e-6
+    # a job larger than the whole budget is refused at submission
+    with hip.Server(max_active=1, hbm_budget=int(need // 2)) as srv:
+        with pytest.raises(hip.NudimError):
+            srv.submit(p, proj)
+
+
+def test_next_times_out_when_idle(hip):
+    with hip.Server(max_active=1) as srv:
+        assert srv.next(timeout_ms=10) is None
+        assert srv.stats()["pending"] == 0
