@@ -187,7 +187,8 @@ def serve(args, precision):
         args.serve_jobs, ", ".join("%s x%d" % (names[c], int((cls == c).sum())) for c in sorted(set(cls))),
         time.perf_counter() - t0))
     reps = {}
-    with nd.Server(max_active=args.serve_slots) as srv:
+    policy = nd.SERVE_FIFO if args.serve_fifo else nd.SERVE_SHORTEST
+    with nd.Server(max_active=args.serve_slots, policy=policy) as srv:
         meta = {}
         start = time.perf_counter()
         k = 0
@@ -224,12 +225,52 @@ def serve(args, precision):
            "mean_latency_s": float(lat.mean()), "makespan_s": span,
            "p50_latency_s_by_class": {k: float(np.percentile(v, 50)) for k, v in sorted(by.items())},
            "precision": "f32" if precision == nd.FP32 else "f64",
+           "policy": "fifo" if args.serve_fifo else "shortest work first, 30 s aging",
            "mix": "%s; seed %d" % (", ".join("%s %.0f%%" % (n, 100 * f) for n, f in phantom.SERVE_MIX),
                                    args.serve_seed)}
     log("  serving %s: %d jobs in %.1f s: p50 %.2f s, p99 %.2f s, %.1f volume-iter/s, achieved %.2f jobs/s "
         "(offered %.2f)" % (out["precision"], len(reps), span, out["p50_latency_s"], out["p99_latency_s"],
                             out["volume_iters_per_s"], out["achieved_jobs_per_s"], out["offered_jobs_per_s"] or 0))
     return out
+
+
+def batch(args):
+    """Uneven batch: 48 independent volumes (phantom.batch_instances) submitted
+    together to the job server, which co-schedules them (args.batch_slots in
+    flight, least work first); against the same jobs one at a time."""
+    t0 = time.perf_counter()
+    inst = phantom.batch_instances(args.batch_jobs)
+    log("  batch: %d instances generated in %.1f s" % (len(inst), time.perf_counter() - t0))
+
+    def params(prm):
+        return nd.make_params(prm["nx"], prm["nz"], prm["np_"], prm["gam_min"], prm["gam_max"], prm["interval"],
+                              padangle=prm["padangle"], weight=prm["weight"], nhio=prm["nhio"], ner=prm["ner"],
+                              denoise=True, method=nd.TV, height=prm["height"], center=prm["center"],
+                              precision=nd.FP32)
+
+    passes = sum(prm["nhio"] + prm["ner"] for prm, _ in inst)
+    res = {}
+    for slots in (1, args.batch_slots):
+        with nd.Server(max_active=slots) as srv:
+            t = time.perf_counter()
+            for prm, proj in inst:
+                srv.submit(params(prm), proj)
+            for _ in inst:
+                rep = srv.next(timeout_ms=-1)[0]
+                if rep["status"] != 0:
+                    raise RuntimeError("batch job failed: %s" % rep["error"])
+            dt = time.perf_counter() - t
+        res[slots] = dt
+        log("  batch, %d in flight: %d volumes (%d volume-iters) in %.2f s -> %.1f volume-iter/s"
+            % (slots, len(inst), passes, dt, passes / dt))
+    sizes = {}
+    for prm, _ in inst:
+        k = "%d^3 np %d" % (prm["nz"], prm["np_"])
+        sizes[k] = sizes.get(k, 0) + 1
+    return {"volumes": len(inst), "volume_iters": passes, "slots": args.batch_slots,
+            "seconds": res[args.batch_slots], "volume_iters_per_s": passes / res[args.batch_slots],
+            "seconds_one_at_a_time": res[1], "speedup_over_one_at_a_time": res[1] / res[args.batch_slots],
+            "sizes": sizes}
 
 
 def main():
@@ -247,8 +288,12 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=4.0, help="mean arrivals per second (Poisson)")
     ap.add_argument("--serve-slots", type=int, default=4, help="jobs in flight at once")
     ap.add_argument("--serve-seed", type=int, default=5)
+    ap.add_argument("--serve-fifo", action="store_true", help="admit in arrival order (default: least work first)")
     ap.add_argument("--serve-fp64", action="store_true", help="also serve the trace on the plain FP64 path")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving leg")
+    ap.add_argument("--batch", action="store_true", help="run only the uneven-batch leg (48 volumes)")
+    ap.add_argument("--batch-jobs", type=int, default=48)
+    ap.add_argument("--batch-slots", type=int, default=8)
     ap.add_argument("--probe", type=int, default=0, metavar="MIB",
                     help="run one FP32 pass and a MIB-MiB device copy, then exit (for counter runs)")
     args = ap.parse_args()
@@ -261,6 +306,9 @@ def main():
     log("workload %s: projections %s generated in %.1f s" % (args.config, proj.shape, time.perf_counter() - t0))
     if args.probe:
         probe(prm, proj, args.probe)
+        return
+    if args.batch:
+        print(json.dumps({"batch": batch(args)}), flush=True)
         return
     if args.serve_only:
         out = {"serving": serve(args, nd.FP32)}

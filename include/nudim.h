@@ -145,15 +145,20 @@ int nudim_project_2d1d(int nx, int ny, int nz, int np, double gam_min, double ga
 
 /* ---- serving: continuous batching of reconstruction jobs ----
  * (No reference counterpart: the reference runs one job per process,
- * src/reconstruction.c main.)  Jobs are admitted first fit in arrival order
+ * src/reconstruction.c main.)  Jobs are admitted by the server's policy
  * while their device memory (nudim_estimate_bytes) fits the budget; each
  * runs on its own stream and host slot thread, so jobs overlap on the
  * device.  A job's result does not depend on what else runs. */
 typedef struct nudim_server nudim_server;
+enum { NUDIM_SERVE_FIFO = 0,      /* first pending job (arrival order) that fits */
+       NUDIM_SERVE_SHORTEST = 1 }; /* least work (np^2 nz passes) that fits; a job waiting
+                                      longer than max_wait_s goes first */
 typedef struct {
   long long hbm_budget; /* bytes the admitted jobs may hold; <= 0: 90% of free device memory */
   int max_active;       /* jobs in flight at once (slot threads), >= 1 */
   int device;
+  int policy;           /* NUDIM_SERVE_* */
+  double max_wait_s;    /* aging bound of NUDIM_SERVE_SHORTEST; <= 0: 30 s */
 } nudim_server_opts;
 typedef struct {
   long long job_id;

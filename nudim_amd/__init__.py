@@ -56,9 +56,13 @@ EXPORTS = [
 EAGAIN = -6
 
 
+SERVE_FIFO, SERVE_SHORTEST = 0, 1
+
+
 class ServerOpts(C.Structure):
     """nudim_server_opts (include/nudim.h)"""
-    _fields_ = [("hbm_budget", C.c_longlong), ("max_active", C.c_int), ("device", C.c_int)]
+    _fields_ = [("hbm_budget", C.c_longlong), ("max_active", C.c_int), ("device", C.c_int),
+                ("policy", C.c_int), ("max_wait_s", C.c_double)]
 
 
 class JobReport(C.Structure):
@@ -350,11 +354,12 @@ def _np2(n):
 
 class Server:
     """Continuous-batching job server (nudim_server_*): jobs of mixed size are
-    admitted first fit in arrival order under an HBM budget and run
-    concurrently, one stream and host slot thread each."""
+    admitted under an HBM budget (least work first with an aging bound, or
+    arrival order) and run concurrently, one stream and host slot thread
+    each."""
 
-    def __init__(self, max_active=4, hbm_budget=0, device=0):
-        o = ServerOpts(hbm_budget, max_active, device)
+    def __init__(self, max_active=4, hbm_budget=0, device=0, policy=SERVE_SHORTEST, max_wait_s=30.0):
+        o = ServerOpts(hbm_budget, max_active, device, policy, max_wait_s)
         h = C.c_void_p()
         _check(lib().nudim_server_create(C.byref(o), C.byref(h)))
         self._h = h

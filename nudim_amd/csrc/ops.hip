@@ -373,11 +373,14 @@ void ToeplitzOp<T>::init(int N_, const double *x, const double *w, int M, int P_
   NUDIM_HIP(hipGetLastError());
   NUDIM_HIP(hipStreamSynchronize(st));
   X.alloc((size_t)N * L * P * sizeof(cx<T>));
-  // The column pass walks X's rows with stride L * pc; keep a row of a
-  // chunk within 2 MB (one large page) so that a block's rows stay within
-  // TLB reach: chunks of pc planes, a power of two dividing P, at least 64.
+  // The column pass walks X's rows with stride L * pc: chunks of pc planes
+  // (P halved while even and at least 64) keep a chunk's row within
+  // NUDIM_TOEP_ROW_BYTES (default 2 MB).  Each plane's arithmetic is the
+  // same for every chunking.
+  const char *rb = getenv("NUDIM_TOEP_ROW_BYTES");
+  const size_t row_cap = rb ? (size_t)strtoull(rb, nullptr, 10) : ((size_t)2 << 20);
   pc = P;
-  while ((size_t)L * pc * sizeof(cx<T>) > ((size_t)2 << 20) && pc % 2 == 0 && pc / 2 >= 64) pc /= 2;
+  while ((size_t)L * pc * sizeof(cx<T>) > row_cap && pc % 2 == 0 && pc / 2 >= 64) pc /= 2;
 }
 
 template <typename T>

@@ -4,12 +4,14 @@
 // main).  Here a server admits jobs of mixed size while others run: every
 // job gets a slot thread and its own stream, so one job's host set-up
 // (geometry and window tables) and transfers overlap other jobs' kernels,
-// and small jobs' kernels overlap large ones'.  Admission is first fit in
-// arrival order against an HBM budget (nudim_estimate_bytes per job).  A
+// and small jobs' kernels overlap large ones'.  Admission follows the
+// server's policy (arrival order, or least work with an aging bound) among
+// the jobs that fit an HBM budget (nudim_estimate_bytes per job).  A
 // job's arithmetic does not depend on what else runs: its result is the same
 // alone and in any mix.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <memory>
 #include <condition_variable>
@@ -33,6 +35,7 @@ struct ServerJob {
   std::vector<double> proj;
   double *volume = nullptr, *rfactors = nullptr;
   size_t bytes = 0;
+  double work = 0;  // np^2 nz passes
   double t_submit = 0, t_start = 0, t_end = 0;
   int status = NUDIM_OK;
   std::string error;
@@ -55,18 +58,27 @@ struct nudim_server {
 
   double now() const { return std::chrono::duration<double>(clk::now() - epoch).count(); }
 
-  // first pending job (arrival order) that fits the free budget; caller holds mu
+  // the next job to admit among those that fit the free budget (caller
+  // holds mu): FIFO takes the first in arrival order; SHORTEST takes the
+  // least work, except that a job waiting longer than max_wait_s (oldest
+  // first) goes before it
   ServerJob *take() {
+    auto best = pending.end();
+    const double t = now();
     for (auto it = pending.begin(); it != pending.end(); ++it) {
-      if (used + (*it)->bytes <= (size_t)o.hbm_budget) {
-        ServerJob *j = *it;
-        pending.erase(it);
-        used += j->bytes;
-        active++;
-        return j;
+      if (used + (*it)->bytes > (size_t)o.hbm_budget) continue;
+      if (o.policy != NUDIM_SERVE_SHORTEST || t - (*it)->t_submit > o.max_wait_s) {
+        best = it;
+        break;
       }
+      if (best == pending.end() || (*it)->work < (*best)->work) best = it;
     }
-    return nullptr;
+    if (best == pending.end()) return nullptr;
+    ServerJob *j = *best;
+    pending.erase(best);
+    used += j->bytes;
+    active++;
+    return j;
   }
 
   void run(ServerJob *j) {
@@ -122,6 +134,11 @@ extern "C" int nudim_server_create(const nudim_server_opts *o, nudim_server **ou
     *out = nullptr;
     std::unique_ptr<nudim_server> s(new nudim_server());
     s->o = *o;
+    if (s->o.max_wait_s <= 0) s->o.max_wait_s = 30.0;
+    if (s->o.policy != NUDIM_SERVE_FIFO && s->o.policy != NUDIM_SERVE_SHORTEST) {
+      nudim::set_last_error("unknown server policy");
+      return NUDIM_EINVAL;
+    }
     if (s->o.hbm_budget <= 0) {
       if (hipSetDevice(o->device) != hipSuccess) {
         nudim::set_last_error("hipSetDevice failed");
@@ -166,6 +183,7 @@ extern "C" int nudim_server_submit(nudim_server *s, const nudim_params *p, const
     j->volume = volume;
     j->rfactors = rfactors;
     j->bytes = bytes;
+    j->work = (double)p->np * p->np * p->nz * (double)std::max(1, p->nhio + p->ner);
     {
       std::lock_guard<std::mutex> lk(s->mu);
       if (s->stop) {

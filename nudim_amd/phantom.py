@@ -131,3 +131,26 @@ SERVE_CLASSES = {
     "s512": (512, 768, 3200, 60, 1, 0),
 }
 SERVE_MIX = (("s128", 0.50), ("s256", 0.35), ("s512", 0.15))
+
+
+def batch_instances(count=48, seed=4000):
+    """The uneven batch: instance i (seed 4000 + i) has nx = nz from
+    {64, 96, 128, 192, 256}, np = nx * {1, 1.25, 1.5}, tilts +-{50, 60, 70} at
+    step 2 or 3, a blob phantom (slab nx/2, blobs ~ 200 (nx/128)^3), noise at
+    SNR 1 or 4, ner = 30 with TV over the slab.  Returns [(params dict,
+    projections)]."""
+    out = []
+    for i in range(count):
+        rng = np.random.Generator(np.random.PCG64(seed + i))
+        nx = int(rng.choice([64, 96, 128, 192, 256]))
+        np_ = int(round(nx * rng.choice([1.0, 1.25, 1.5]) / 2)) * 2
+        g = int(rng.choice([50, 60, 70]))
+        t = int(rng.choice([2, 3]))
+        snr = float(rng.choice([1.0, 4.0]))
+        nb = max(8, int(200 * (nx / 128) ** 3))
+        bp = blob_params(nb, (nx, nx, nx), nx // 2, seed + i)
+        proj = add_noise(projections(bp, np_, -g, g, t), snr, seed + 1000 + i)
+        prm = dict(nx=np_, nz=nx, np_=np_, gam_min=-g, gam_max=g, interval=t, padangle=1.0, weight=True,
+                   nhio=0, ner=30, denoise=True, height=nx // 2, center=np_ // 2)
+        out.append((prm, proj))
+    return out

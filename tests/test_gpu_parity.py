@@ -146,6 +146,27 @@ def test_cgnr_register_fft_sizes(hip, orc, N):
     assert rel(got32, ref) < 1e-4
 
 
+def test_toeplitz_plane_chunks(hip, orc, monkeypatch):
+    """The fused Toeplitz passes over plane chunks (long rows of the 2N x 2N
+    intermediate, forced here by a small row budget) equal the single-chunk
+    passes bit for bit, and the oracle's CGNR to the FP32 bar."""
+    N, P = 64, 128
+    pmin, pmax = orc.pad_range(-60, 3)
+    x = orc.polar_nodes(pmin, pmax, 2.0, N)
+    M = x.shape[0]
+    w = orc.weights(N, M // N, 2.0)
+    rng = np.random.default_rng(17)
+    y = rng.standard_normal((P, M)) + 1j * rng.standard_normal((P, M))
+    f0 = rng.standard_normal((P, N, N))
+    one = hip.cgnr(x, y, f0, w, n=2 * N, W=8, precision=hip.FP32, form=hip.CG_TOEPLITZ)
+    monkeypatch.setenv("NUDIM_TOEP_ROW_BYTES", "4096")  # 2N * 64 planes * 8 B > 4 KB: two chunks of 64
+    two = hip.cgnr(x, y, f0, w, n=2 * N, W=8, precision=hip.FP32, form=hip.CG_TOEPLITZ)
+    assert np.array_equal(one, two)
+    plan = orc.NFFT(x, N)
+    for p in (0, 77, P - 1):
+        assert rel(two[p], plan.cgnr(y[p], f0[p], w, 10)) < 1e-4
+
+
 @pytest.mark.parametrize("n,nz,np_,t", [(32, 16, 48, 5), (24, 32, 32, 3), (16, 8, 16, 30)])
 def test_project_2d1d(hip, orc, n, nz, np_, t):
     """GPU 2D1D simulator against the oracle's (same algorithm, FP64), and

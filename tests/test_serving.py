@@ -77,3 +77,26 @@ def test_next_times_out_when_idle(hip):
     with hip.Server(max_active=1) as srv:
         assert srv.next(timeout_ms=10) is None
         assert srv.stats()["pending"] == 0
+
+
+@pytest.mark.parametrize("policy", ["shortest", "fifo"])
+def test_admission_policy_order(hip, policy):
+    """One slot: while the first job runs, a larger and then a smaller job
+    arrive; least-work-first runs the smaller one next, FIFO the larger."""
+    pol = hip.SERVE_SHORTEST if policy == "shortest" else hip.SERVE_FIFO
+    big = job_params(hip, 64, 64, 1, False, 1)
+    small = job_params(hip, 32, 32, 1, False, 1)
+    pb, ps = job_inputs(hip, 64, 64, 1), job_inputs(hip, 32, 32, 2)
+    with hip.Server(max_active=1, policy=pol) as srv:
+        first = srv.submit(big, pb)
+        later_big = srv.submit(big, pb)
+        later_small = srv.submit(small, ps)
+        reps = {}
+        for _ in range(3):
+            rep = srv.next(timeout_ms=120000)[0]
+            reps[rep["job_id"]] = rep
+    assert reps[first]["t_start"] <= min(reps[later_big]["t_start"], reps[later_small]["t_start"])
+    if policy == "shortest":
+        assert reps[later_small]["t_start"] < reps[later_big]["t_start"]
+    else:
+        assert reps[later_big]["t_start"] < reps[later_small]["t_start"]
