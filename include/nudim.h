@@ -57,6 +57,9 @@ typedef struct nudim_params {
   int nfft_w;                         /* window taps per dimension (0: 14 for FP64, tuned FP32) */
   int cg_form;                        /* NUDIM_CG_* (0: NFFT for FP64, Toeplitz for FP32) */
   int use_graph;                      /* capture each iteration into a hipGraph (FP32 only) */
+  /* denoising variants of src/reconstruction-parallel.c (0 / 0 for the serial program) */
+  int tv_full;                        /* TV over the whole volume even with a height band (:766) */
+  float diffusion_lambda;             /* aniso_diffusion_2d lambda; 0: 0.25f (:764 passes 0.4) */
 } nudim_params;
 
 typedef struct nudim_ctx nudim_ctx;

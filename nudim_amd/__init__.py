@@ -40,7 +40,7 @@ class Params(C.Structure):
                 ("ncenter", C.c_int), ("denoise", C.c_int), ("method", C.c_int),
                 ("nhio", C.c_int), ("ner", C.c_int), ("precision", C.c_int), ("device", C.c_int),
                 ("nfft_n", C.c_int), ("nfft_w", C.c_int), ("cg_form", C.c_int),
-                ("use_graph", C.c_int)]
+                ("use_graph", C.c_int), ("tv_full", C.c_int), ("diffusion_lambda", C.c_float)]
 
 
 EXPORTS = [
@@ -146,7 +146,7 @@ def _f64(a):
 def make_params(nx, nz, np_, gam_min, gam_max, interval, padangle=1.0, weight=True, random=False,
                 seed=0, height=None, center=None, denoise=False, method=TV, nhio=0, ner=20,
                 precision=FP64_PLAIN, device=0, nfft_n=0, nfft_w=0, cg_form=CG_AUTO,
-                use_graph=False):
+                use_graph=False, tv_full=False, diffusion_lambda=0.0):
     """Parameters with the meaning of reconstruction's flags (src/reconstruction.c:91-112)."""
     p = Params()
     lib().nudim_default_params(C.byref(p))
@@ -161,6 +161,7 @@ def make_params(nx, nz, np_, gam_min, gam_max, interval, padangle=1.0, weight=Tr
     p.nhio, p.ner = nhio, ner
     p.precision, p.device = precision, device
     p.nfft_n, p.nfft_w, p.cg_form, p.use_graph = nfft_n, nfft_w, cg_form, int(use_graph)
+    p.tv_full, p.diffusion_lambda = int(tv_full), diffusion_lambda
     return p
 
 

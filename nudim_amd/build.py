@@ -71,21 +71,23 @@ def _wait(procs):
             raise subprocess.CalledProcessError(p.returncode, p.args)
 
 
-# host programs (one per main) and the helpers every one of them links
-HOST_MAINS = ["reconstruction.c", "projection_sim.c"]
-HOST_HELPERS = ["textio.c"]
+# host programs and their sources (nudim_amd/host)
+HOST_PROGRAMS = {
+    "reconstruction": ["reconstruction.c", "recon_cli.c", "textio.c"],
+    "reconstruction_parallel": ["reconstruction_parallel.c", "recon_cli.c", "textio.c"],
+    "projection_sim": ["projection_sim.c", "textio.c"],
+}
 
 
 def build_host(force=False):
     os.makedirs(BIN, exist_ok=True)
     out = []
-    helpers = [os.path.join(HOST, h) for h in HOST_HELPERS]
-    deps = helpers + [os.path.join(HOST, h[:-2] + ".h") for h in HOST_HELPERS]
-    for src in HOST_MAINS:
-        exe = os.path.join(BIN, src[:-2])
-        s = os.path.join(HOST, src)
-        if force or _stale(exe, [s, LIB, os.path.join(INCLUDE, "nudim.h")] + deps):
-            _run(["gcc", "-O2", "-std=gnu99", "-Wall", "-o", exe, s] + helpers +
+    headers = [os.path.join(HOST, h) for h in sorted(os.listdir(HOST)) if h.endswith(".h")]
+    for prog, srcs in HOST_PROGRAMS.items():
+        exe = os.path.join(BIN, prog)
+        paths = [os.path.join(HOST, s) for s in srcs]
+        if force or _stale(exe, paths + headers + [LIB, os.path.join(INCLUDE, "nudim.h")]):
+            _run(["gcc", "-O2", "-std=gnu99", "-Wall", "-o", exe] + paths +
                  ["-I", INCLUDE, "-I", HOST, "-L", HERE, "-lnudim_hip", "-Wl,-rpath,$ORIGIN/..", "-lm", "-lpthread"])
         out.append(exe)
     return out

@@ -331,7 +331,7 @@ struct Job : JobBase {
 
   void tv(T *u) {
     int ylo = 0, yhi = N - 1;
-    if (prm.config_height) {
+    if (prm.config_height && !prm.tv_full) {
       ylo = std::max(ylo, (2 * prm.ncenter - prm.nheight) / 2);
       yhi = std::min(yhi, (2 * prm.ncenter + prm.nheight) / 2);
     }
@@ -344,7 +344,7 @@ struct Job : JobBase {
 
   void diffusion(T *u) {
     const int niter = 80;
-    double lam = (double)0.25f, kappa = (double)0.005f;
+    double lam = (double)(prm.diffusion_lambda > 0 ? prm.diffusion_lambda : 0.25f), kappa = (double)0.005f;
     int ny = N, nx = N;
     int taumax = 2 * (niter - 1) + (ny - 1) + (nx - 1);
     for (int tau = 0; tau <= taumax; tau++) {

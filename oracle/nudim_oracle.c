@@ -879,8 +879,8 @@ int oracle_iterate(const oracle_params *p, double *img, double *img_last_out, co
   int rc = 0;
   if (p->denoise) {
     if (p->method == 0) {
-      rc = oracle_diffusion2d(img, 0.25f, 0.005f, 80, nz, np, np);
-    } else if (p->config_height) {
+      rc = oracle_diffusion2d(img, p->diffusion_lambda > 0 ? p->diffusion_lambda : 0.25f, 0.005f, 80, nz, np, np);
+    } else if (p->config_height && !p->tv_full) {
       oracle_tv_height(img, 0.2f, 40, nz, np, np, p->nheight, p->ncenter);
     } else {
       oracle_tv(img, 0.2f, 40, nz, np, np);

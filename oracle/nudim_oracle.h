@@ -73,6 +73,8 @@ typedef struct oracle_params {
   int config_height, nheight, ncenter;
   int nhio, ner;
   int nthreads;
+  int tv_full;            /* reconstruction-parallel.c:766: TV over the whole volume */
+  float diffusion_lambda; /* 0: 0.25f; reconstruction-parallel.c:764 passes 0.4 */
 } oracle_params;
 /* proj[nmeas][nz][nx]; img0/img out [nz][np][np] (may be NULL); rtrace[nhio+ner] (may be NULL) */
 int oracle_reconstruct(const oracle_params *p, const double *proj, double *img0, double *img,

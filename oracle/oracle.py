@@ -42,7 +42,8 @@ class OracleParams(C.Structure):
                 ("interval", C.c_double), ("padangle", C.c_double),
                 ("weight", C.c_int), ("denoise", C.c_int), ("method", C.c_int),
                 ("config_height", C.c_int), ("nheight", C.c_int), ("ncenter", C.c_int),
-                ("nhio", C.c_int), ("ner", C.c_int), ("nthreads", C.c_int)]
+                ("nhio", C.c_int), ("ner", C.c_int), ("nthreads", C.c_int),
+                ("tv_full", C.c_int), ("diffusion_lambda", C.c_float)]
 
 
 def _declare(L):
@@ -233,11 +234,12 @@ def constraint(img, img_last, nx, is_hio, config_height=0, nheight=-1, ncenter=-
 # ---- end to end ------------------------------------------------------------
 
 def params(nx, nz, np_, gam_min, gam_max, interval, padangle=1.0, weight=1, denoise=0,
-           method=1, height=None, center=None, nhio=0, ner=1, nthreads=0):
+           method=1, height=None, center=None, nhio=0, ner=1, nthreads=0, tv_full=False,
+           diffusion_lambda=0.0):
     ch = height is not None or center is not None
     return OracleParams(nx, nz, np_, gam_min, gam_max, interval, padangle, int(weight),
                         int(denoise), int(method), int(ch), height or -1, center or -1,
-                        nhio, ner, nthreads)
+                        nhio, ner, nthreads, int(tv_full), diffusion_lambda)
 
 
 def reconstruct(p, proj):

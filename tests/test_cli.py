@@ -201,3 +201,95 @@ def test_sim_end_to_end(sim_binary, orc, tmp_path):
         noisy = np.loadtxt(tmp_path / "noisy" / "files" / ("prj-%d.txt" % j))
         ratio = np.std(noisy - clean) / np.std(clean)
         assert 0.15 < ratio < 0.35  # sigma_noise = sigma_signal / snr (src/projection_sim.c:58-62)
+
+
+# ---- reconstruction_parallel -----------------------------------------------
+
+PAR = os.path.join(ROOT, "nudim_amd", "bin", "reconstruction_parallel")
+
+
+def run_par(args, cwd, env=None):
+    e = dict(os.environ)
+    if env:
+        e.update(env)
+    return subprocess.run([PAR] + args, cwd=cwd, capture_output=True, text=True, env=e, timeout=600)
+
+
+def grad_mag(u):
+    """show_grad_mag of denoise.c:381-402 on u[z][y][x]: the density on the
+    border, the norm of the three backward differences inside."""
+    g = u.copy()
+    c = u[1:-1, 1:-1, 1:-1]
+    g[1:-1, 1:-1, 1:-1] = np.sqrt((c - u[:-2, 1:-1, 1:-1]) ** 2 + (c - u[1:-1, :-2, 1:-1]) ** 2 +
+                                  (c - u[1:-1, 1:-1, :-2]) ** 2)
+    return g
+
+
+@pytest.fixture(scope="module")
+def par_binary(hip):
+    if not os.path.exists(PAR):
+        from nudim_amd import build
+        build.build_host()
+    return PAR
+
+
+def test_par_validation(par_binary, tmp_path):
+    base = ["--prefix=prj", "--nx=8", "--nz=8", "-p", "8", "--nhio=0", "--ner=1", "-t", "3", "--padangle=1",
+            "--max=60", "--min=-60"]
+    r = run_par(base + ["--height=0"], tmp_path)
+    assert r.returncode == 1 and "You must provide valid sample height --height." in r.stderr
+    # no --center in the parallel program (reconstruction-parallel.c:108-127)
+    assert run_par(base + ["--height=4", "--center=4"], tmp_path).returncode == 1
+    r = run_par(["-h"], tmp_path)
+    assert r.returncode == 0 and "--height.                 Sample height for constraint." in r.stdout
+    assert "--center" not in r.stdout
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("method,passes", [("DIFFUSION", 2), ("TV", 1)])
+def test_par_end_to_end(par_binary, hip, orc, tmp_path, method, passes):
+    """FP64: the parallel program's numerics (band centred at np/2, TV over
+    the whole volume, DIFFUSION with lambda 0.4f) against the oracle given the
+    same switches; one TV pass keeps TV's rounding amplification ~1e-9."""
+    np_ = nz = 32
+    rng = np.random.default_rng(23)
+    dens = blob_phantom((nz, np_, np_), 10, rng, sig=(1.5, 3.0), slab=16)
+    proj = orc.project_2d1d(dens, np_, -60, 60, 5)
+    files = tmp_path / "files"
+    files.mkdir()
+    for j in range(proj.shape[0]):
+        np.savetxt(files / ("prj-%d.txt" % j), proj[j].ravel(), fmt="%f")
+    args = ["--prefix=prj", "--pad=32", "--nx=32", "--nz=32", "--max=60", "--min=-60", "-t", "5",
+            "--padangle=1", "--nhio=0", "--ner=%d" % passes, "--weight", "--height=14", "--denoise",
+            "--method=%s" % method, "--print"]
+    r = run_par(args, tmp_path, env={"NUDIM_PRECISION": "fp64"})
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.strip().splitlines()[-1].startswith("Total time used : ")
+    proj_in = np.stack([np.loadtxt(files / ("prj-%d.txt" % j)).reshape(32, 32) for j in range(proj.shape[0])])
+    op = orc.params(32, nz, np_, -60, 60, 5, padangle=1.0, weight=1, nhio=0, ner=passes, denoise=1,
+                    method=0 if method == "DIFFUSION" else 1, height=14, center=16, tv_full=True,
+                    diffusion_lambda=0.4 if method == "DIFFUSION" else 0.0)
+    img0, img, rt = orc.reconstruct(op, proj_in)
+    r0 = np.loadtxt(tmp_path / "recovered-0-parallel.txt").reshape(nz, np_, np_)
+    rec = np.loadtxt(tmp_path / "recovered-parallel.txt").reshape(nz, 32, 32)
+    assert rel(r0, img0) < 1e-6
+    assert np.max(np.abs(rec - img)) < 1e-5
+    # gradient-magnitude dumps of the density before / after the denoiser at
+    # pass 0: the library's dumps of the same job, the denoiser step against
+    # the oracle, grad_mag restated from denoise.c:381-402
+    hp = hip.make_params(32, nz, np_, -60, 60, 5, padangle=1.0, weight=True, nhio=0, ner=passes, denoise=True,
+                         method=hip.DIFFUSION if method == "DIFFUSION" else hip.TV, height=14, center=16,
+                         tv_full=True, diffusion_lambda=0.4 if method == "DIFFUSION" else 0.0)
+    before, after = np.zeros((nz, np_, np_)), np.zeros((nz, np_, np_))
+    with hip.Reconstruction(hp) as job:
+        job.load_projections(proj_in)
+        job.initial()
+        job.set_dump(0, before, after)
+        job.iterate(passes)
+    ref_after = orc.diffusion2d(before, lam=0.4) if method == "DIFFUSION" else orc.tv(before)
+    assert rel(after, ref_after) < 1e-9
+    for name, v in (("grad_before.txt", before), ("grad_after.txt", after)):
+        g = np.loadtxt(tmp_path / name)
+        assert np.max(np.abs(g - grad_mag(v).ravel())) < 1e-5
+    # the band is centred at np/2: rows outside (32-14)/2 .. (32+14)/2 are zero after ER
+    assert np.all(rec[:, :9, :] == 0) and np.all(rec[:, 24:, :] == 0)
