@@ -71,11 +71,12 @@ def _wait(procs):
             raise subprocess.CalledProcessError(p.returncode, p.args)
 
 
-# host programs and their sources (nudim_amd/host)
+# host programs, their sources (nudim_amd/host) and whether they link libnudim_hip
 HOST_PROGRAMS = {
-    "reconstruction": ["reconstruction.c", "recon_cli.c", "textio.c"],
-    "reconstruction_parallel": ["reconstruction_parallel.c", "recon_cli.c", "textio.c"],
-    "projection_sim": ["projection_sim.c", "textio.c"],
+    "reconstruction": (["reconstruction.c", "recon_cli.c", "textio.c"], True),
+    "reconstruction_parallel": (["reconstruction_parallel.c", "recon_cli.c", "textio.c"], True),
+    "projection_sim": (["projection_sim.c", "textio.c"], True),
+    "mrcread": (["mrcread.c", "textio.c"], False),
 }
 
 
@@ -83,12 +84,13 @@ def build_host(force=False):
     os.makedirs(BIN, exist_ok=True)
     out = []
     headers = [os.path.join(HOST, h) for h in sorted(os.listdir(HOST)) if h.endswith(".h")]
-    for prog, srcs in HOST_PROGRAMS.items():
+    for prog, (srcs, uses_lib) in HOST_PROGRAMS.items():
         exe = os.path.join(BIN, prog)
         paths = [os.path.join(HOST, s) for s in srcs]
-        if force or _stale(exe, paths + headers + [LIB, os.path.join(INCLUDE, "nudim.h")]):
-            _run(["gcc", "-O2", "-std=gnu99", "-Wall", "-o", exe] + paths +
-                 ["-I", INCLUDE, "-I", HOST, "-L", HERE, "-lnudim_hip", "-Wl,-rpath,$ORIGIN/..", "-lm", "-lpthread"])
+        libs = ["-L", HERE, "-lnudim_hip", "-Wl,-rpath,$ORIGIN/.."] if uses_lib else []
+        if force or _stale(exe, paths + headers + ([LIB] if uses_lib else []) + [os.path.join(INCLUDE, "nudim.h")]):
+            _run(["gcc", "-O2", "-std=gnu99", "-Wall", "-o", exe] + paths + ["-I", INCLUDE, "-I", HOST] + libs +
+                 ["-lm", "-lpthread"])
         out.append(exe)
     return out
 

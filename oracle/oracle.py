@@ -17,7 +17,8 @@ _LIB = None
 
 
 def build():
-    subprocess.run(["make", "-s", "-C", _HERE, "liboracle.so"], check=True)
+    """liboracle.so, and _ref/mrcread where the reference tree is present."""
+    subprocess.run(["make", "-s", "-C", _HERE, "all"], check=True)
 
 
 def lib():
@@ -312,3 +313,28 @@ def project_dft(density, np_, gam_min, gam_max, interval):
 
 def set_threads(n):
     lib().oracle_set_threads(n)
+
+
+# ---- MRC converter (examples/exp) ------------------------------------------
+
+def mrc_convert(stack, shift_x=61, shift_y=1):
+    """examples/exp/mrcread.c:38-80 on stack float32 [nz][ny][nx] (the
+    sections as mrc_read_slice converts them, mrcfile.c:182-207): section 0
+    dropped; for k >= 1 value (j, i) moves to column (i - 61) mod nx and, for
+    i < 61, to row (j - 1) mod ny; output max - value + min in float64 with
+    max/min the section's extremes started at -100000 / 100000.  Returns
+    {output index nz-1-k: (values [ny*nx] float64, max, min)}."""
+    nz, ny, nx = stack.shape
+    out = {}
+    i = np.arange(nx)
+    for k in range(1, nz):
+        sec = stack[k].astype(np.float64)
+        mx = max(-100000.0, float(sec.max()))
+        mn = min(100000.0, float(sec.min()))
+        moved = np.empty_like(sec)
+        wide = i >= shift_x
+        moved[:, (i[wide] - shift_x) % nx] = sec[:, wide]
+        rows = (np.arange(ny) - shift_y) % ny
+        moved[rows[:, None], ((i[~wide] - shift_x) % nx)[None, :]] = sec[:, ~wide]
+        out[nz - 1 - k] = ((mx - moved) + mn).ravel(), mx, mn
+    return out
