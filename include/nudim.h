@@ -56,7 +56,10 @@ typedef struct nudim_params {
                                          autotuned for FP32) */
   int nfft_w;                         /* window taps per dimension (0: 14 for FP64, tuned FP32) */
   int cg_form;                        /* NUDIM_CG_* (0: NFFT for FP64, Toeplitz for FP32) */
-  int use_graph;                      /* capture each iteration into a hipGraph (FP32 only) */
+  int use_graph;                      /* run passes as hipGraphs: a pass is captured once per
+                                         starting state (volume buffers, HIO/ER) and replayed
+                                         for later passes that start in it; passes that time
+                                         kernels (nudim_profile) or dump run directly */
   /* denoising variants of src/reconstruction-parallel.c (0 / 0 for the serial program) */
   int tv_full;                        /* TV over the whole volume even with a height band (:766) */
   float diffusion_lambda;             /* aniso_diffusion_2d lambda; 0: 0.25f (:764 passes 0.4) */

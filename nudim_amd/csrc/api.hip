@@ -145,6 +145,10 @@ struct JobBase {
   virtual void get_volume(double *img) = 0;
   virtual void set_volume(const double *img) = 0;
   virtual size_t bytes() const = 0;
+  // the host-side state a pass reads and changes: which buffer holds the
+  // current, the previous and the spare volume
+  virtual void get_state(void *s[3]) const = 0;
+  virtual void set_state(void *const s[3]) = 0;
   double *dump_before = nullptr, *dump_after = nullptr;
   int dump_nn = -1;
 };
@@ -393,6 +397,17 @@ struct Job : JobBase {
 
   void get_volume(double *out) override { copy_to_host_ref(img, out); }
 
+  void get_state(void *s[3]) const override {
+    s[0] = img;
+    s[1] = img_last;
+    s[2] = spare;
+  }
+  void set_state(void *const s[3]) override {
+    img = (T *)s[0];
+    img_last = (T *)s[1];
+    spare = (T *)s[2];
+  }
+
   void set_volume(const double *in) override {
     DevBuf tmp(V * 8);
     NUDIM_HIP(hipMemcpyAsync(tmp.p, in, V * 8, hipMemcpyHostToDevice, st));
@@ -400,6 +415,16 @@ struct Job : JobBase {
     NUDIM_HIP(hipGetLastError());
     NUDIM_HIP(hipStreamSynchronize(st));
   }
+};
+
+// One pass captured as a hipGraph.  A pass's launches depend on the host
+// state only through the three volume pointers and the HIO/ER switch, so a
+// graph captured from one state replays any later pass that starts in it;
+// the pointers cycle with period at most 6, so a job holds at most 12.
+struct PassGraph {
+  void *pre[3], *post[3];
+  int hio;
+  hipGraphExec_t exec;
 };
 
 struct nudim_ctx {
@@ -411,8 +436,54 @@ struct nudim_ctx {
   bool loaded = false, initialized = false;
   DevBuf rtrace;
   int rcap = 0;
+  DevBuf rcur;  // R of a replayed pass, copied to its trace slot after the graph
+  std::vector<PassGraph> graphs;
   KernelProfiler prof;
 };
+
+static void destroy_graphs(nudim_ctx *c) {
+  for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+  c->graphs.clear();
+}
+
+// pass nn through the graph captured for the job's current state
+static void iterate_graph(nudim_ctx *c, int nn, double *rslot) {
+  void *pre[3];
+  c->job->get_state(pre);
+  const int hio = nn < c->p.nhio ? 1 : 0;
+  const PassGraph *hit = nullptr;
+  for (const auto &g : c->graphs)
+    if (g.hio == hio && std::equal(pre, pre + 3, g.pre)) hit = &g;
+  if (!hit) {
+    if (!c->rcur.p) c->rcur.alloc(sizeof(double));
+    PassGraph g{};
+    std::copy(pre, pre + 3, g.pre);
+    g.hio = hio;
+    hipGraph_t graph = nullptr;
+    NUDIM_HIP(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+    try {
+      c->job->iterate(nn, c->rcur.as<double>());
+    } catch (...) {
+      (void)hipStreamEndCapture(c->st, &graph);
+      if (graph) (void)hipGraphDestroy(graph);
+      c->job->set_state(pre);
+      throw;
+    }
+    NUDIM_HIP(hipStreamEndCapture(c->st, &graph));
+    const hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+      c->job->set_state(pre);
+      throw nudim::Error(NUDIM_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    c->job->get_state(g.post);
+    c->graphs.push_back(g);
+    hit = &c->graphs.back();
+  }
+  NUDIM_HIP(hipGraphLaunch(hit->exec, c->st));
+  c->job->set_state(hit->post);
+  NUDIM_HIP(hipMemcpyAsync(rslot, c->rcur.p, sizeof(double), hipMemcpyDeviceToDevice, c->st));
+}
 
 // makes the context's profiler current for the calling thread
 struct ProfBind {
@@ -458,6 +529,7 @@ extern "C" void nudim_destroy(nudim_ctx *c) {
   try {
     (void)hipSetDevice(c->p.device);
     if (c->st) (void)hipStreamSynchronize(c->st);
+    destroy_graphs(c);
     c->job.reset();
     c->rtrace.release();
     if (c->st) (void)hipStreamDestroy(c->st);
@@ -531,7 +603,14 @@ extern "C" int nudim_iterate(nudim_ctx *c, int count, double *rfactor) {
   }
   {
     ProfBind pb(c);
-    for (int i = 0; i < count; i++) c->job->iterate(c->nn + i, c->rtrace.as<double>() + i);
+    // passes that time kernels or dump volumes run their launches directly
+    for (int i = 0; i < count; i++) {
+      const int nn = c->nn + i;
+      if (c->p.use_graph && !c->prof.cls && nn != c->job->dump_nn)
+        iterate_graph(c, nn, c->rtrace.as<double>() + i);
+      else
+        c->job->iterate(nn, c->rtrace.as<double>() + i);
+    }
   }
   if (rfactor && count > 0)
     NUDIM_HIP(hipMemcpyAsync(rfactor, c->rtrace.p, (size_t)count * 8, hipMemcpyDeviceToHost, c->st));
