@@ -21,7 +21,7 @@ ARCH = os.environ.get("NUDIM_OFFLOAD_ARCH", "gfx950")
 # Packed FP32 instructions are switched off for all device code.
 HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-            "-Wall", "-Wno-unused-function", "-Wno-unused-variable"]
+            "-Wall", "-Wshadow", "-Wno-unused-function", "-Wno-unused-variable"]
 
 
 def _run(cmd, cwd=None):
@@ -51,7 +51,7 @@ def build_lib(force=False, jobs=4):
             if src.endswith(".hip"):
                 cmd = [hipcc, "-c", "-x", "hip", s, "-o", o, "-I", INCLUDE, "-I", CSRC] + HIPFLAGS
             else:
-                cmd = ["g++", "-c", s, "-o", o, "-O3", "-std=c++17", "-fPIC", "-Wall", "-I", INCLUDE, "-I", CSRC]
+                cmd = ["g++", "-c", s, "-o", o, "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wshadow", "-I", INCLUDE, "-I", CSRC]
             print("+", " ".join(cmd), flush=True)
             procs.append(subprocess.Popen(cmd))
             if len(procs) >= jobs:

@@ -204,6 +204,19 @@ static void resolve_knobs(const nudim_params *p, int *n, int *W, int *form) {
   }
 }
 
+// The Toeplitz-form CGNR of a pass starts from z = A^H W (y - A f0) with
+// y = K m + U A fh (the Fourier projector: measured values at the known
+// nodes K, the current transform elsewhere).  Since A^H W K m is the same in
+// every pass, z = b_K + T_U fh - T f0 with b_K = A^H W K m computed once,
+// T_U = A^H W U A and T = A^H W A: two Toeplitz products on the fused passes
+// instead of the adjoint NFFT and one product.  NUDIM_CG_START=nfft keeps the
+// adjoint NFFT form.
+static bool projected_start(int form, int N) {
+  const char *e = getenv("NUDIM_CG_START");
+  const bool off = e && std::string(e) == "nfft";
+  return !off && form == NUDIM_CG_TOEPLITZ && g_use_fft2 && toep_supported(2 * N);
+}
+
 template <typename T>
 struct Job : JobBase {
   nudim_params prm;
@@ -220,6 +233,8 @@ struct Job : JobBase {
   DevBuf d1, d2, tvpart_g, tvpart_a, tvsums;
   FftDesc fdz{};
   DevBuf twz;
+  bool proj_start = false;  // projected_start(): b_K in bk
+  DevBuf bk;
   T *img = nullptr, *img_last = nullptr, *spare = nullptr;
   static constexpr int kTvBlocks = 1024;
 
@@ -238,6 +253,15 @@ struct Job : JobBase {
     }
     if (form == NUDIM_CG_TOEPLITZ) toep.init(N, g.x.data(), g.w.empty() ? nullptr : g.w.data(), M, P, st);
     cg.init(N, M, P, form, &op, form == NUDIM_CG_TOEPLITZ ? &toep : nullptr, dw.as<double>());
+    proj_start = projected_start(form, N);
+    if (proj_start) {
+      // kernel of T_U: the weights of the nodes the projector does not
+      // replace by measured values (kind != 1)
+      std::vector<double> wu(M);
+      for (int m = 0; m < M; m++) wu[m] = g.kind[m] == 1 ? 0.0 : (g.w.empty() ? 1.0 : g.w[m]);
+      toep.init_second(g.x.data(), wu.data(), M, st);
+      bk.alloc(NN * sizeof(cx<T>));
+    }
     dkind.alloc(M * sizeof(int));
     NUDIM_HIP(hipMemcpy(dkind.p, g.kind.data(), M * sizeof(int), hipMemcpyHostToDevice));
     dorder.alloc(M * sizeof(int));
@@ -274,8 +298,18 @@ struct Job : JobBase {
     NUDIM_HIP(hipMemsetAsync(img, 0, V * sizeof(T), st));
   }
 
+  // b_K = A^H W K m (the known nodes' measured values, weighted)
+  void known_adjoint() {
+    NUDIM_CHECK(proj_start && bk.p, "b_K requested without the projected CG start");
+    cx<T> *t = cg.t.template as<cx<T>>();
+    hipLaunchKernelGGL(k_known_weighted<T>, dim3(cdiv(MM, 256)), dim3(256), 0, st, meas.as<cx<T>>(), dkind.as<int>(),
+                       dw.as<double>(), t, MM, P);
+    NUDIM_HIP(hipGetLastError());
+    op.adjoint(t, bk.as<cx<T>>(), st);
+  }
+
   size_t bytes() const override {
-    return op.bytes() + toep.bytes() + cg.bytes() + dw.bytes + dkind.bytes + dorder.bytes + meas.bytes + y.bytes +
+    return op.bytes() + toep.bytes() + cg.bytes() + dw.bytes + dkind.bytes + dorder.bytes + meas.bytes + y.bytes + bk.bytes +
            Fh.bytes + F0.bytes + img_a.bytes + img_b.bytes + img_c.bytes + grad.bytes + grad2.bytes + d1.bytes +
            d2.bytes + tvpart_g.bytes +
            tvpart_a.bytes + tvsums.bytes + twz.bytes;
@@ -287,6 +321,7 @@ struct Job : JobBase {
     hipLaunchKernelGGL(k_cx_from_planes<T>, dim3(cdiv(MM, 256)), dim3(256), 0, st, tmp.as<double>(), meas.as<cx<T>>(),
                        (long long)M, P);
     NUDIM_HIP(hipGetLastError());
+    if (proj_start) known_adjoint();
     NUDIM_HIP(hipStreamSynchronize(st));
   }
 
@@ -303,6 +338,7 @@ struct Job : JobBase {
     hipLaunchKernelGGL(k_assemble<T>, dim3(cdiv(MM, 256)), dim3(256), 0, st, dtomo.as<cx<double>>(),
                        dkind.as<int>(), dorder.as<int>(), drnd.as<double>(), meas.as<cx<T>>(), M, N, P);
     NUDIM_HIP(hipGetLastError());
+    if (proj_start) known_adjoint();
     NUDIM_HIP(hipStreamSynchronize(st));
   }
 
@@ -376,11 +412,18 @@ struct Job : JobBase {
     cx<T> *v = cg.v.template as<cx<T>>();
     op.trafo(Fh.as<cx<T>>(), v, st);
     cg.red.template rfactor<T>(v, meas.as<cx<T>>(), dkind.as<int>(), M, d1.as<double>(), d2.as<double>(), st);
-    hipLaunchKernelGGL(k_projector<T>, dim3(cdiv(MM, 256)), dim3(256), 0, st, v, meas.as<cx<T>>(), dkind.as<int>(),
-                       y.as<cx<T>>(), MM, P);
+    if (!proj_start)
+      hipLaunchKernelGGL(k_projector<T>, dim3(cdiv(MM, 256)), dim3(256), 0, st, v, meas.as<cx<T>>(),
+                         dkind.as<int>(), y.as<cx<T>>(), MM, P);
     hipLaunchKernelGGL(k_rfactor_final, dim3(1), dim3(1), 0, st, d1.as<double>(), d2.as<double>(), P, rslot);
     NUDIM_HIP(hipGetLastError());
-    cg.solve(y.as<cx<T>>(), F0.as<cx<T>>(), 10, false, st);
+    if (proj_start) {
+      // z = b_K + T_U Fh - T f0, then the CG steps
+      toep.apply_start(Fh.as<cx<T>>(), F0.as<cx<T>>(), bk.as<cx<T>>(), cg.z.template as<cx<T>>(), st);
+      cg.solve_from_residual(F0.as<cx<T>>(), 10, st);
+    } else {
+      cg.solve(y.as<cx<T>>(), F0.as<cx<T>>(), 10, false, st);
+    }
     zfft_inverse(F0.as<cx<T>>(), img);
     if (prm.denoise) {
       if (nn == dump_nn && dump_before) copy_to_host_ref(img, dump_before);
@@ -688,6 +731,7 @@ extern "C" int nudim_estimate_bytes(const nudim_params *p, size_t *bytes) {
   b += (size_t)N * n * P * cb + (size_t)n * n * P * cb;           // NFFT X, grid
   b += (size_t)M * W * W * (4 + rb) + (size_t)n * n * 4 + M * W * 16;  // gather maps, window tables
   if (form == NUDIM_CG_TOEPLITZ) b += (size_t)4 * N * N * (cb + rb) + (size_t)N * 2 * N * P * cb + NN * cb;
+  if (projected_start(form, (int)N)) b += (size_t)4 * N * N * rb + (size_t)N * 2 * N * P * cb + NN * cb;
   b += 2 * NN * cb + 3 * MM * cb;                                   // CG z, p; r, v, t
   b += 2 * MM * cb + 2 * NN * cb;                                   // meas, y, Fh, F0
   b += 2 * NN * rb + ((p->denoise && p->method == NUDIM_TV) ? 3 * NN * rb : 0);  // volumes, TV gradients

@@ -65,8 +65,14 @@ struct ToeplitzOp {
   DevBuf khat_re;  // T[L][L]: its real part (the kernel is Hermitian, so khat is real)
   DevBuf X;     // [N][L][P]; for the fused passes [P/pc][N][L][pc]
   int pc = 0;   // planes per chunk of X in the fused passes
+  // second kernel (init_second) and intermediate for apply_start
+  DevBuf khat2_re, X2;
   size_t bytes() const;
   void init(int N, const double *x, const double *w, int M, int P, hipStream_t st);
+  // the kernel of the weights w2[M] (host) as a second operator T2
+  void init_second(const double *x, const double *w2, int M, hipStream_t st);
+  // z = add + T2 a - T b on the fused passes (fused() and init_second done)
+  void apply_start(const cx<T> *a, const cx<T> *b, const cx<T> *add, cx<T> *z, hipStream_t st);
   void apply(const cx<T> *p, cx<T> *q, hipStream_t st);
   // fused CGNR step on the fixed-length passes (when fused()): p from z and
   // the old p (see ToepCg), q = T p, part[N][P] = row sums of Re <p, q>
@@ -113,6 +119,9 @@ struct Cgnr {
   void init(int N, int M, int P, int form, NfftOp<T> *op, ToeplitzOp<T> *toep, const double *w);
   // solver_before_loop_complex + nsteps loop_one_step (fhat in/out; f0_zero skips A*0)
   void solve(const cx<T> *y, cx<T> *fhat, int nsteps, bool f0_zero, hipStream_t st);
+  // the same on the fused Toeplitz passes with the starting residual given
+  // in z (z = A^H W (y - A fhat))
+  void solve_from_residual(cx<T> *fhat, int nsteps, hipStream_t st);
 };
 
 }  // namespace nudim

@@ -191,13 +191,17 @@ struct LineIO {
 
 // Passes of the Toeplitz normal operator on the fixed-length kernels:
 // row forward (with the CGNR prologue), column (forward, product, inverse),
-// row inverse (with the <p, q> partial sums).
-enum { TK_ROWF = 0, TK_COL = 1, TK_ROWI = 2 };
+// row inverse (with the <p, q> partial sums), and a column pass over two
+// inputs with two kernels (the difference of two Toeplitz products).
+enum { TK_ROWF = 0, TK_COL = 1, TK_ROWI = 2, TK_COL2 = 3 };
 
 // CGNR vectors carried by the row passes, [N][N][P] with planes innermost.
 //   row forward: p = z (zz_old null) or p = (zz/zz_old) p + z, after
-//                f += alpha p with the old p (alpha non-null); then FFT of p
-//   row inverse: q = the cropped inverse; part[row][plane] = sum Re conj(p) q
+//                f += alpha p with the old p (alpha non-null); then FFT of p.
+//                With z null the pass transforms its input as it is.
+//   row inverse: q = the cropped inverse (+ add[] when add is non-null);
+//                with p non-null part[row][plane] = sum Re conj(p) q
+//   two-input column pass: IFFT(mid FFT(in) - mid2 FFT(in2))
 template <typename T>
 struct ToepCg {
   const cx<T> *z;
@@ -205,6 +209,9 @@ struct ToepCg {
   cx<T> *f;
   const double *zz, *zz_old, *alpha;
   double *part;
+  const cx<T> *in2;
+  const T *mid2;
+  const cx<T> *add;
 };
 
 // Batched line FFT over a plane-innermost array.  Block b (renumbered so

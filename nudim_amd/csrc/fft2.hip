@@ -62,6 +62,10 @@ void launch_toep(int kind, int nlines, const cx<T> *in, cx<T> *out, long long in
       hipLaunchKernelGGL((k_toep<T, N1, N2, TK_COL>), grid, block, lds, st, in, out, in_so, in_sk, out_so, out_sk,
                          tw, Gs, nchunk, mid, cg, pstride);
       break;
+    case TK_COL2:
+      hipLaunchKernelGGL((k_toep<T, N1, N2, TK_COL2>), grid, block, lds, st, in, out, in_so, in_sk, out_so, out_sk,
+                         tw, Gs, nchunk, mid, cg, pstride);
+      break;
     default:
       hipLaunchKernelGGL((k_toep<T, N1, N2, TK_ROWI>), grid, block, lds, st, in, out, in_so, in_sk, out_so, out_sk,
                          tw, Gs, nchunk, mid, cg, pstride);

@@ -404,7 +404,7 @@ k_toep(const cx<T> *in, cx<T> *out, long long in_so, long long in_sk, long long 
     } else {
       // positions r + N2 j1 < N for j1 < N1/2; the rest are zero
       const long long base = (long long)o * in_so + plane;
-      if constexpr (KIND == TK_ROWF) {
+      if (KIND == TK_ROWF && cg.z) {
         cx<T> zv[H1];
 #pragma unroll
         for (int j1 = 0; j1 < H1; j1++) zv[j1] = cg.z[base + (long long)(r + N2 * j1) * in_sk];
@@ -457,26 +457,39 @@ k_toep(const cx<T> *in, cx<T> *out, long long in_so, long long in_sk, long long 
   } else if constexpr (KIND == TK_ROWI) {
     // outputs r + N1 k2 < N for k2 < N2/2
     dft_half_out<T, N2, +1>(b);
-    cx<T> *dst = out + (long long)o * out_so + plane;
-    const cx<T> *pp = cg.p + (long long)o * out_so + plane;
-    cx<T> pv[H2];
+    const long long ob = (long long)o * out_so + plane;
+    cx<T> *dst = out + ob;
+    if (cg.add) {
+      cx<T> av[H2];
 #pragma unroll
-    for (int k2 = 0; k2 < H2; k2++) pv[k2] = pp[(long long)(r + N1 * k2) * out_sk];
-    double s = 0;
+      for (int k2 = 0; k2 < H2; k2++) av[k2] = cg.add[ob + (long long)(r + N1 * k2) * out_sk];
 #pragma unroll
-    for (int k2 = 0; k2 < H2; k2++) {
-      dst[(long long)(r + N1 * k2) * out_sk] = b[k2];
-      s += (double)pv[k2].x * b[k2].x + (double)pv[k2].y * b[k2].y;
+      for (int k2 = 0; k2 < H2; k2++) b[k2] = av[k2] + b[k2];
     }
-    // fixed-order sum over the N1 threads of each plane
-    __syncthreads();  // exchange buffer free
-    double *sh = reinterpret_cast<double *>(smem_raw);
-    sh[r * G + g] = s;
-    __syncthreads();
-    if (r == 0) {
-      double t = 0;
-      for (int rr = 0; rr < N1; rr++) t += sh[rr * G + g];
-      cg.part[(long long)o * pstride + plane] = t;
+    if (cg.p) {
+      const cx<T> *pp = cg.p + ob;
+      cx<T> pv[H2];
+#pragma unroll
+      for (int k2 = 0; k2 < H2; k2++) pv[k2] = pp[(long long)(r + N1 * k2) * out_sk];
+      double s = 0;
+#pragma unroll
+      for (int k2 = 0; k2 < H2; k2++) {
+        dst[(long long)(r + N1 * k2) * out_sk] = b[k2];
+        s += (double)pv[k2].x * b[k2].x + (double)pv[k2].y * b[k2].y;
+      }
+      // fixed-order sum over the N1 threads of each plane
+      __syncthreads();  // exchange buffer free
+      double *sh = reinterpret_cast<double *>(smem_raw);
+      sh[r * G + g] = s;
+      __syncthreads();
+      if (r == 0) {
+        double t = 0;
+        for (int rr = 0; rr < N1; rr++) t += sh[rr * G + g];
+        cg.part[(long long)o * pstride + plane] = t;
+      }
+    } else {
+#pragma unroll
+      for (int k2 = 0; k2 < H2; k2++) dst[(long long)(r + N1 * k2) * out_sk] = b[k2];
     }
   } else {
     // forward N2-point DFT, real product, inverse N2-point DFT, twiddle, exchange
@@ -487,6 +500,27 @@ k_toep(const cx<T> *in, cx<T> *out, long long in_so, long long in_sk, long long 
     for (int k2 = 0; k2 < N2; k2++) mv[k2] = mrow[r + N1 * k2];
 #pragma unroll
     for (int k2 = 0; k2 < N2; k2++) b[k2] = mv[k2] * b[k2];
+    if constexpr (KIND == TK_COL2) {
+      // the second input's column through the same forward steps, times
+      // mid2, subtracted
+      __syncthreads();  // every thread has read the exchange buffer
+      if (actA) {
+        const cx<T> *src = cg.in2 + (long long)o * in_so + plane;
+#pragma unroll
+        for (int j1 = 0; j1 < H1; j1++) a[j1] = src[(long long)(r + N2 * j1) * in_sk];
+        dft_half_in<T, N1, -1>(a);
+#pragma unroll
+        for (int k1 = 0; k1 < N1; k1++) a[k1] = a[k1] * stw[r * k1];
+      }
+      cx<T> b2[N2];
+      xchg<T, N1, N2, SPLIT>(lx, a, actA, r * rowA + g, G, b2, true, r * G + g, rowA);
+      dft_reg<T, N2, -1>(b2);
+      const T *mrow2 = cg.mid2 + (long long)o * n;
+#pragma unroll
+      for (int k2 = 0; k2 < N2; k2++) mv[k2] = mrow2[r + N1 * k2];
+#pragma unroll
+      for (int k2 = 0; k2 < N2; k2++) b[k2] = b[k2] - mv[k2] * b2[k2];
+    }
     dft_reg<T, N2, +1>(b);
 #pragma unroll
     for (int m2 = 0; m2 < N2; m2++) b[m2] = b[m2] * conjx(stw[r * m2]);

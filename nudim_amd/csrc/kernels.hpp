@@ -458,6 +458,16 @@ __global__ void k_resid_init(const cx<T> *__restrict__ y, const cx<T> *__restric
   t[i] = w ? (T)w[k] * rv : rv;
 }
 
+// t = known ? w m : 0, the weighted known-node data of the projected CG start
+template <typename T>
+__global__ void k_known_weighted(const cx<T> *__restrict__ meas, const int *__restrict__ kind,
+                                 const double *__restrict__ w, cx<T> *__restrict__ t, long long total, int P) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  long long k = i / P;
+  t[i] = kind[k] == 1 ? (w ? (T)w[k] * meas[i] : meas[i]) : mk((T)0, (T)0);
+}
+
 // Fourier projector, reconstruction.c:574-582: y = known ? measured : f
 template <typename T>
 __global__ void k_projector(const cx<T> *__restrict__ f, const cx<T> *__restrict__ meas, const int *__restrict__ kind,
@@ -498,6 +508,42 @@ __device__ __forceinline__ T tv_rs(T t, T ss) {
   else return t / sqrt(ss);
 }
 
+// Thread walk of the TV kernels over one band row y: flat (x, z) positions
+// rem = x nz + z from bxk*kThreads + threadIdx.x in steps of the row's block
+// count times kThreads, with x and z carried along (one division per thread)
+// and 32-bit byte offsets (the volume holds fewer than 2^32 bytes).
+struct TvWalk {
+  unsigned rem, x, z, stride, sdx, sdz;
+  __device__ TvWalk(unsigned bxk, unsigned nblk, unsigned nz) {
+    rem = bxk * kThreads + threadIdx.x;
+    x = rem / nz;
+    z = rem - x * nz;
+    stride = nblk * kThreads;
+    sdx = stride / nz;
+    sdz = stride - sdx * nz;
+  }
+  __device__ void next(unsigned nz) {
+    rem += stride;
+    x += sdx;
+    z += sdz;
+    if (z >= nz) {
+      z -= nz;
+      x++;
+    }
+  }
+};
+
+// element at a 32-bit byte offset from p (the global_load form with a scalar
+// base and a 32-bit vector offset)
+template <typename T>
+__device__ __forceinline__ T ld_at(const T *__restrict__ p, unsigned b) {
+  return *(const T *)((const char *)p + b);
+}
+template <typename T>
+__device__ __forceinline__ void st_at(T *__restrict__ p, unsigned b, T v) {
+  *(T *)((char *)p + b) = v;
+}
+
 // grad over rows ylo..yhi (inclusive; clipped), FP64 partial sums of grad^2
 // and of u^2 over u < 0 (the pre-update dA, denoise.c:128-133)
 template <typename T>
@@ -505,44 +551,42 @@ __global__ void __launch_bounds__(kThreads)
 k_tv_grad(const T *__restrict__ u, T *__restrict__ grad, int ny, int nx, int nz, int ylo, int yhi,
           double *__restrict__ part_g, double *__restrict__ part_a) {
   __shared__ double sg[kThreads], sa[kThreads];
-  const long long rowlen = (long long)nx * nz;
-  const long long total = (long long)(yhi - ylo + 1) * rowlen;
   const T eps = (T)1e-8;
   double accg = 0, acca = 0;
   // blocks: (chunk of one row (x, z), band row), renumbered so that each XCD
-  // takes a contiguous run of rows (the y neighbours then share its L2);
-  // 32-bit index math
+  // takes a contiguous run of rows (the y neighbours then share its L2)
   const unsigned lb = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
   const unsigned bxk = lb % gridDim.x;
   const int y = ylo + (int)(lb / gridDim.x);
-  const unsigned rl = (unsigned)rowlen, unz = (unsigned)nz;
-  for (unsigned rem = bxk * kThreads + threadIdx.x; rem < rl; rem += gridDim.x * kThreads) {
-    int x = (int)(rem / unz), z = (int)(rem - (unsigned)x * unz);
-    const long long sy = rowlen, sx = nz;
-    const long long o = (long long)y * sy + (long long)x * sx + z;
-    const long long dzm = z > 0 ? -1 : 0, dzp = z < nz - 1 ? 1 : 0;
-    const long long dym = y > 0 ? -sy : 0, dyp = y < ny - 1 ? sy : 0;
-    const long long dxm = x > 0 ? -sx : 0, dxp = x < nx - 1 ? sx : 0;
-    T c = u[o];
+  constexpr unsigned E = sizeof(T);
+  const unsigned unz = (unsigned)nz, rl = (unsigned)nx * unz, sx = unz * E, sy = rl * E;
+  const unsigned ybase = (unsigned)y * sy;
+  const unsigned dym = y > 0 ? sy : 0u, dyp = y < ny - 1 ? sy : 0u;
+  auto at = [&](unsigned b) { return ld_at(u, b); };
+  for (TvWalk w(bxk, gridDim.x, unz); w.rem < rl; w.next(unz)) {
+    const unsigned o = ybase + w.rem * E;
+    const unsigned dzm = w.z > 0 ? E : 0u, dzp = w.z + 1 < unz ? E : 0u;
+    const unsigned dxm = w.x > 0 ? sx : 0u, dxp = w.x + 1 < (unsigned)nx ? sx : 0u;
+    T c = at(o);
     // reference (i,j,k) = (z,y,x)
-    T t1 = c - u[o + dzm];
-    T t2 = c - u[o + dym];
-    T t3 = c - u[o + dxm];
-    T uzp = u[o + dzp];
+    T t1 = c - at(o - dzm);
+    T t2 = c - at(o - dym);
+    T t3 = c - at(o - dxm);
+    T uzp = at(o + dzp);
     T t4 = uzp - c;
-    T t5 = uzp - u[o + dzp + dym];
-    T t6 = uzp - u[o + dzp + dxm];
-    T uyp = u[o + dyp];
-    T t7 = uyp - u[o + dyp + dzm];
+    T t5 = uzp - at(o + dzp - dym);
+    T t6 = uzp - at(o + dzp - dxm);
+    T uyp = at(o + dyp);
+    T t7 = uyp - at(o + dyp - dzm);
     T t8 = uyp - c;
-    T t9 = uyp - u[o + dyp + dxm];
-    T uxp = u[o + dxp];
-    T t10 = uxp - u[o + dxp + dzm];
-    T t11 = uxp - u[o + dxp + dym];
+    T t9 = uyp - at(o + dyp - dxm);
+    T uxp = at(o + dxp);
+    T t10 = uxp - at(o + dxp - dzm);
+    T t11 = uxp - at(o + dxp - dym);
     T t12 = uxp - c;
     T gv = tv_rs(t1 + t2 + t3, eps + t1 * t1 + t2 * t2 + t3 * t3) - tv_rs(t4, eps + t4 * t4 + t5 * t5 + t6 * t6) -
            tv_rs(t8, eps + t7 * t7 + t8 * t8 + t9 * t9) - tv_rs(t12, eps + t10 * t10 + t11 * t11 + t12 * t12);
-    grad[o] = gv;
+    st_at(grad, o, gv);
     accg += (double)gv * (double)gv;
     if (c < (T)0) acca += (double)c * (double)c;
   }
@@ -595,8 +639,6 @@ k_tv_fused(const T *__restrict__ u, const T *__restrict__ g, T *__restrict__ un,
            int nz, int ylo, int yhi, const double *__restrict__ sums, double lam, double *__restrict__ part_g,
            double *__restrict__ part_a) {
   __shared__ double sg[kThreads], sa[kThreads];
-  const long long rowlen = (long long)nx * nz;
-  const long long total = (long long)(yhi - ylo + 1) * rowlen;
   const T eps = (T)1e-8;
   const double s1d = sqrt(sums[0]);
   const T s1 = (T)s1d, inv_s1 = (T)(1.0 / s1d);
@@ -604,45 +646,44 @@ k_tv_fused(const T *__restrict__ u, const T *__restrict__ g, T *__restrict__ un,
   // u_{s+1} at flat offset o.  The gradient buffers hold zeros outside the
   // band (they are cleared when allocated and only band rows are written),
   // so the update needs no row test and every stencil read is unconditional.
-  auto nxt = [&](long long o, int) -> T {
-    T v = u[o], gg = g[o];
+  auto nxt = [&](unsigned b) -> T {
+    T v = ld_at(u, b), gg = ld_at(g, b);
     if constexpr (sizeof(T) == 4) return v - c * (gg * inv_s1);
     else return v - c * (gg / s1);
   };
   double accg = 0, acca = 0;
-  // blocks as in k_tv_grad
+  // blocks and walk as in k_tv_grad
   const unsigned lb = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
   const unsigned bxk = lb % gridDim.x;
   const int y = ylo + (int)(lb / gridDim.x);
-  const unsigned rl = (unsigned)rowlen, unz = (unsigned)nz;
-  for (unsigned rem = bxk * kThreads + threadIdx.x; rem < rl; rem += gridDim.x * kThreads) {
-    int x = (int)(rem / unz), z = (int)(rem - (unsigned)x * unz);
-    const long long sy = rowlen, sx = nz;
-    const long long o = (long long)y * sy + (long long)x * sx + z;
-    const long long dzm = z > 0 ? -1 : 0, dzp = z < nz - 1 ? 1 : 0;
-    const long long dym = y > 0 ? -sy : 0, dyp = y < ny - 1 ? sy : 0;
-    const long long dxm = x > 0 ? -sx : 0, dxp = x < nx - 1 ? sx : 0;
-    const int ym = y > 0 ? y - 1 : 0, yp = y < ny - 1 ? y + 1 : y;
-    T cc = nxt(o, y);
-    T t1 = cc - nxt(o + dzm, y);
-    T t2 = cc - nxt(o + dym, ym);
-    T t3 = cc - nxt(o + dxm, y);
-    T uzp = nxt(o + dzp, y);
+  constexpr unsigned E = sizeof(T);
+  const unsigned unz = (unsigned)nz, rl = (unsigned)nx * unz, sx = unz * E, sy = rl * E;
+  const unsigned ybase = (unsigned)y * sy;
+  const unsigned dym = y > 0 ? sy : 0u, dyp = y < ny - 1 ? sy : 0u;
+  for (TvWalk w(bxk, gridDim.x, unz); w.rem < rl; w.next(unz)) {
+    const unsigned o = ybase + w.rem * E;
+    const unsigned dzm = w.z > 0 ? E : 0u, dzp = w.z + 1 < unz ? E : 0u;
+    const unsigned dxm = w.x > 0 ? sx : 0u, dxp = w.x + 1 < (unsigned)nx ? sx : 0u;
+    T cc = nxt(o);
+    T t1 = cc - nxt(o - dzm);
+    T t2 = cc - nxt(o - dym);
+    T t3 = cc - nxt(o - dxm);
+    T uzp = nxt(o + dzp);
     T t4 = uzp - cc;
-    T t5 = uzp - nxt(o + dzp + dym, ym);
-    T t6 = uzp - nxt(o + dzp + dxm, y);
-    T uyp = nxt(o + dyp, yp);
-    T t7 = uyp - nxt(o + dyp + dzm, yp);
+    T t5 = uzp - nxt(o + dzp - dym);
+    T t6 = uzp - nxt(o + dzp - dxm);
+    T uyp = nxt(o + dyp);
+    T t7 = uyp - nxt(o + dyp - dzm);
     T t8 = uyp - cc;
-    T t9 = uyp - nxt(o + dyp + dxm, yp);
-    T uxp = nxt(o + dxp, y);
-    T t10 = uxp - nxt(o + dxp + dzm, y);
-    T t11 = uxp - nxt(o + dxp + dym, ym);
+    T t9 = uyp - nxt(o + dyp - dxm);
+    T uxp = nxt(o + dxp);
+    T t10 = uxp - nxt(o + dxp - dzm);
+    T t11 = uxp - nxt(o + dxp - dym);
     T t12 = uxp - cc;
     T gv = tv_rs(t1 + t2 + t3, eps + t1 * t1 + t2 * t2 + t3 * t3) - tv_rs(t4, eps + t4 * t4 + t5 * t5 + t6 * t6) -
            tv_rs(t8, eps + t7 * t7 + t8 * t8 + t9 * t9) - tv_rs(t12, eps + t10 * t10 + t11 * t11 + t12 * t12);
-    un[o] = cc;
-    gn[o] = gv;
+    st_at(un, o, cc);
+    st_at(gn, o, gv);
     accg += (double)gv * (double)gv;
     if (cc < (T)0) acca += (double)cc * (double)cc;
   }

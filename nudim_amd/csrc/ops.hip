@@ -328,23 +328,19 @@ __global__ void k_toeplitz_finish(const cx<double> *__restrict__ spec, cx<T> *__
   if (i >= (long long)L * L) return;
   int r = (int)(i / L), c = (int)(i % L);
   cx<double> v = spec[i];
-  khat[(long long)c * L + r] = mk((T)(v.x * scale), (T)(v.y * scale));
+  if (khat) khat[(long long)c * L + r] = mk((T)(v.x * scale), (T)(v.y * scale));
   khat_re[(long long)c * L + r] = (T)(v.x * scale);
 }
 
+// Spectrum of the Toeplitz kernel T(d) = sum_m w_m exp(2 pi i d.x_m), d in
+// [-N, N)^2, embedded in the 2N x 2N circulant, scaled by 1/L^2: one FP64
+// adjoint NFFT of the weights on a 2N grid (reference-accuracy window:
+// n = 2*nextpow2(2N), 14 taps), the embedding, an FP64 2D FFT.  khat (may be
+// null) gets the complex spectrum, khat_re its real part.
 template <typename T>
-size_t ToeplitzOp<T>::bytes() const {
-  return fft.tw.bytes + khat.bytes + khat_re.bytes + X.bytes;
-}
-
-template <typename T>
-void ToeplitzOp<T>::init(int N_, const double *x, const double *w, int M, int P_, hipStream_t st) {
-  N = N_;
-  L = 2 * N;
-  P = P_;
-  fft.init(L);
-  // Toeplitz kernel by one FP64 adjoint NFFT of the weights on a 2N grid
-  // (reference-accuracy window: n = 2*nextpow2(2N), 14 taps)
+static void toeplitz_kernel(int N, const double *x, const double *w, int M, DevBuf *khat, DevBuf &khat_re,
+                            hipStream_t st) {
+  const int L = 2 * N;
   NfftOp<double> big;
   big.init(L, 2 * next_pow2(L), 14, x, M, 1);
   std::vector<double> wv(2 * (size_t)M);
@@ -366,12 +362,64 @@ void ToeplitzOp<T>::init(int N_, const double *x, const double *w, int M, int P_
   f64.run(emb.as<cx<double>>(), tmp.as<cx<double>>(), L, ri, ri, -1, 1, st);
   LineIO ci{1, L, L, 0, nullptr, nullptr};
   f64.run(tmp.as<cx<double>>(), emb.as<cx<double>>(), L, ci, ci, -1, 1, st);
-  khat.alloc((size_t)L * L * sizeof(cx<T>));
+  if (khat) khat->alloc((size_t)L * L * sizeof(cx<T>));
   khat_re.alloc((size_t)L * L * sizeof(T));
   hipLaunchKernelGGL(k_toeplitz_finish<T>, dim3(cdiv((size_t)L * L, 256)), dim3(256), 0, st, emb.as<cx<double>>(),
-                     khat.as<cx<T>>(), khat_re.as<T>(), L, 1.0 / ((double)L * (double)L));
+                     khat ? khat->as<cx<T>>() : nullptr, khat_re.as<T>(), L, 1.0 / ((double)L * (double)L));
   NUDIM_HIP(hipGetLastError());
   NUDIM_HIP(hipStreamSynchronize(st));
+}
+
+template <typename T>
+size_t ToeplitzOp<T>::bytes() const {
+  return fft.tw.bytes + khat.bytes + khat_re.bytes + X.bytes + khat2_re.bytes + X2.bytes;
+}
+
+template <typename T>
+void ToeplitzOp<T>::init_second(const double *x, const double *w2, int M, hipStream_t st) {
+  toeplitz_kernel<T>(N, x, w2, M, nullptr, khat2_re, st);
+  X2.alloc((size_t)N * L * P * sizeof(cx<T>));
+}
+
+template <typename T>
+void ToeplitzOp<T>::apply_start(const cx<T> *a, const cx<T> *b, const cx<T> *add, cx<T> *z, hipStream_t st) {
+  NUDIM_CHECK(fused() && khat2_re.p, "apply_start needs the fused passes and a second kernel");
+  const long long NP = (long long)N * P, LC = (long long)L * pc;
+  const cx<T> *tw = fft.tw.template as<cx<T>>();
+  for (int p0 = 0; p0 < P; p0 += pc) {
+    cx<T> *xa = X2.as<cx<T>>() + (long long)p0 * N * L, *xb = X.as<cx<T>>() + (long long)p0 * N * L;
+    ToepCg<T> c{};
+    {
+      // algorithmic bytes: input read (N x N), X written (N x L), twice
+      ProfScope ps(KC_TOEPLITZ_ROWF, st, 2.0 * (N + L) * N * pc * sizeof(cx<T>));
+      toep_launch<T>(TK_ROWF, L, N, a + p0, xa, NP, P, LC, pc, pc, st, tw, nullptr, c, P);
+      toep_launch<T>(TK_ROWF, L, N, b + p0, xb, NP, P, LC, pc, pc, st, tw, nullptr, c, P);
+    }
+    {
+      // X2 and X read, X written (N x L each), both kernels read
+      ProfScope ps(KC_TOEPLITZ_COL, st, 3.0 * N * L * pc * sizeof(cx<T>) + 2.0 * L * L * sizeof(T));
+      c.in2 = xb;
+      c.mid2 = khat_re.template as<T>();
+      toep_launch<T>(TK_COL2, L, L, xa, xb, pc, LC, pc, LC, pc, st, tw, khat2_re.template as<T>(), c, P);
+    }
+    {
+      // X read (N x L), add read and z written (N x N each)
+      ProfScope ps(KC_TOEPLITZ_ROWI, st, (2.0 * N + L) * N * pc * sizeof(cx<T>));
+      ToepCg<T> ci{};
+      ci.add = add + p0;
+      toep_launch<T>(TK_ROWI, L, N, xb, z + p0, LC, pc, NP, P, pc, st, tw, nullptr, ci, P);
+    }
+  }
+  NUDIM_HIP(hipGetLastError());
+}
+
+template <typename T>
+void ToeplitzOp<T>::init(int N_, const double *x, const double *w, int M, int P_, hipStream_t st) {
+  N = N_;
+  L = 2 * N;
+  P = P_;
+  fft.init(L);
+  toeplitz_kernel<T>(N, x, w, M, &khat, khat_re, st);
   X.alloc((size_t)N * L * P * sizeof(cx<T>));
   // The column pass walks X's rows with stride L * pc: chunks of pc planes
   // (P halved while even and at least 64) keep a chunk's row within
@@ -508,6 +556,41 @@ void Cgnr<T>::init(int N_, int M_, int P_, int form_, NfftOp<T> *op_, ToeplitzOp
 }
 
 template <typename T>
+void Cgnr<T>::solve_from_residual(cx<T> *fhat, int nsteps, hipStream_t st) {
+  NUDIM_CHECK(form == NUDIM_CG_TOEPLITZ && toep->fused(), "the fused Toeplitz passes are not available");
+  const long long NN = (long long)N * N * P;
+  const unsigned bN = cdiv(NN, 256);
+  cx<T> *zp = z.as<cx<T>>(), *pp = pv.as<cx<T>>();
+  double *dz = dot_z.as<double>(), *cf = coef.as<double>(), *s = sc.as<double>();
+  red.dot<T>(zp, nullptr, (long long)N * N, s, st);
+  hipLaunchKernelGGL(k_cg_scalars, dim3(cdiv(P, 64)), dim3(64), 0, st, dz, cf, s, P, 0);
+  NUDIM_HIP(hipGetLastError());
+  // step k: row pass forms p_k = z_k + beta_k p_{k-1} (p_0 = z_0) and
+  // applies the previous step's f += alpha p_{k-1}; q = T p_k with the
+  // row sums of <p_k, q>; alpha_k = |z_k|^2 / <p_k, q>; z -= alpha_k q;
+  // |z_{k+1}|^2.  The last step's f update follows the loop.
+  cx<T> *qp = q.as<cx<T>>();
+  double *zz = dz, *zz_old = zz2.as<double>(), *pq = s, *al = alpha.as<double>();
+  const int PCs = plane_group_log2(P);
+  const dim3 rg(kRedBlocks, P >> PCs);
+  for (int it = 0; it < nsteps; it++) {
+    ToepCg<T> cg{zp, pp, fhat, zz, it ? zz_old : nullptr, it ? al : nullptr, rowpart.as<double>()};
+    toep->apply_cg(cg, qp, st);
+    hipLaunchKernelGGL(k_sum_partials, dim3(cdiv(P, 8)), dim3(256), 0, st, rowpart.as<double>(), N, P, pq);
+    hipLaunchKernelGGL(k_cg_zupdate<T>, rg, dim3(kThreads), 0, st, zp, qp, zz, pq, al, (long long)N * N, P, PCs,
+                       red.part.as<double>());
+    hipLaunchKernelGGL(k_sum_partials, dim3(cdiv(P, 8)), dim3(256), 0, st, red.part.as<double>(), kRedBlocks, P,
+                       zz_old);
+    NUDIM_HIP(hipGetLastError());
+    std::swap(zz, zz_old);
+  }
+  if (nsteps > 0) {
+    hipLaunchKernelGGL(k_axpy_planes<T>, dim3(bN), dim3(256), 0, st, fhat, pp, al, 1.0, NN, P);
+    NUDIM_HIP(hipGetLastError());
+  }
+}
+
+template <typename T>
 void Cgnr<T>::solve(const cx<T> *y, cx<T> *fhat, int nsteps, bool f0_zero, hipStream_t st) {
   const long long NN = (long long)N * N * P, MM = (long long)M * P;
   const unsigned bN = cdiv(NN, 256), bM = cdiv(MM, 256);
@@ -533,35 +616,13 @@ void Cgnr<T>::solve(const cx<T> *y, cx<T> *fhat, int nsteps, bool f0_zero, hipSt
                        tp, MM, P);
     op->adjoint(tp, zp, st);
   }
+  if (fused) {
+    solve_from_residual(fhat, nsteps, st);
+    return;
+  }
   red.dot<T>(zp, nullptr, (long long)N * N, s, st);
   hipLaunchKernelGGL(k_cg_scalars, dim3(cdiv(P, 64)), dim3(64), 0, st, dz, cf, s, P, 0);
   NUDIM_HIP(hipGetLastError());
-  if (fused) {
-    // step k: row pass forms p_k = z_k + beta_k p_{k-1} (p_0 = z_0) and
-    // applies the previous step's f += alpha p_{k-1}; q = T p_k with the
-    // row sums of <p_k, q>; alpha_k = |z_k|^2 / <p_k, q>; z -= alpha_k q;
-    // |z_{k+1}|^2.  The last step's f update follows the loop.
-    cx<T> *qp = q.as<cx<T>>();
-    double *zz = dz, *zz_old = zz2.as<double>(), *pq = s, *al = alpha.as<double>();
-    const int PCs = plane_group_log2(P);
-    const dim3 rg(kRedBlocks, P >> PCs);
-    for (int it = 0; it < nsteps; it++) {
-      ToepCg<T> cg{zp, pp, fhat, zz, it ? zz_old : nullptr, it ? al : nullptr, rowpart.as<double>()};
-      toep->apply_cg(cg, qp, st);
-      hipLaunchKernelGGL(k_sum_partials, dim3(cdiv(P, 8)), dim3(256), 0, st, rowpart.as<double>(), N, P, pq);
-      hipLaunchKernelGGL(k_cg_zupdate<T>, rg, dim3(kThreads), 0, st, zp, qp, zz, pq, al, (long long)N * N, P, PCs,
-                         red.part.as<double>());
-      hipLaunchKernelGGL(k_sum_partials, dim3(cdiv(P, 8)), dim3(256), 0, st, red.part.as<double>(), kRedBlocks, P,
-                         zz_old);
-      NUDIM_HIP(hipGetLastError());
-      std::swap(zz, zz_old);
-    }
-    if (nsteps > 0) {
-      hipLaunchKernelGGL(k_axpy_planes<T>, dim3(bN), dim3(256), 0, st, fhat, pp, al, 1.0, NN, P);
-      NUDIM_HIP(hipGetLastError());
-    }
-    return;
-  }
   hipLaunchKernelGGL(k_copy_cx<T>, dim3(bN), dim3(256), 0, st, pp, zp, NN);
   NUDIM_HIP(hipGetLastError());
   if (form == NUDIM_CG_TOEPLITZ) {
@@ -620,6 +681,7 @@ T *tv_sweeps(T *u0, T *u1, T *g0, T *g1, int ny, int nx, int nz, int ylo, int yh
   const int gx = (int)std::min<long long>(cdiv(rowlen, kThreads), 64);
   const int np_ = nrows * gx;
   NUDIM_CHECK(np_ <= nb, "TV partial-sum buffer too small");
+  NUDIM_CHECK((double)V * sizeof(T) < 4294967296.0, "TV kernels address volumes of fewer than 2^32 bytes");
   const dim3 grid(gx, nrows);
   // rows outside the band keep their values in both buffers
   hipLaunchKernelGGL(k_copy_real<T>, dim3(cdiv(V, 256)), dim3(256), 0, st, u1, u0, V);
