@@ -235,6 +235,8 @@ struct Job : JobBase {
   DevBuf twz;
   bool proj_start = false;  // projected_start(): b_K in bk
   DevBuf bk;
+  DevBuf dknown;  // with proj_start: the known nodes (R needs A fh only there)
+  int nknown = 0;
   T *img = nullptr, *img_last = nullptr, *spare = nullptr;
   static constexpr int kTvBlocks = 1024;
 
@@ -261,6 +263,14 @@ struct Job : JobBase {
       for (int m = 0; m < M; m++) wu[m] = g.kind[m] == 1 ? 0.0 : (g.w.empty() ? 1.0 : g.w[m]);
       toep.init_second(g.x.data(), wu.data(), M, st);
       bk.alloc(NN * sizeof(cx<T>));
+      std::vector<int> kn;
+      for (int m = 0; m < M; m++)
+        if (g.kind[m] == 1) kn.push_back(m);
+      nknown = (int)kn.size();
+      if (nknown > 0) {
+        dknown.alloc(kn.size() * sizeof(int));
+        NUDIM_HIP(hipMemcpy(dknown.p, kn.data(), kn.size() * sizeof(int), hipMemcpyHostToDevice));
+      }
     }
     dkind.alloc(M * sizeof(int));
     NUDIM_HIP(hipMemcpy(dkind.p, g.kind.data(), M * sizeof(int), hipMemcpyHostToDevice));
@@ -309,7 +319,8 @@ struct Job : JobBase {
   }
 
   size_t bytes() const override {
-    return op.bytes() + toep.bytes() + cg.bytes() + dw.bytes + dkind.bytes + dorder.bytes + meas.bytes + y.bytes + bk.bytes +
+    return op.bytes() + toep.bytes() + cg.bytes() + dw.bytes + dkind.bytes + dorder.bytes + meas.bytes + y.bytes +
+           bk.bytes + dknown.bytes +
            Fh.bytes + F0.bytes + img_a.bytes + img_b.bytes + img_c.bytes + grad.bytes + grad2.bytes + d1.bytes +
            d2.bytes + tvpart_g.bytes +
            tvpart_a.bytes + tvsums.bytes + twz.bytes;
@@ -410,7 +421,11 @@ struct Job : JobBase {
     std::swap(img, img_last);  // img_last = img (reconstruction.c:551)
     zfft_forward(img_last, Fh.as<cx<T>>(), F0.as<cx<T>>());
     cx<T> *v = cg.v.template as<cx<T>>();
-    op.trafo(Fh.as<cx<T>>(), v, st);
+    // A fh: everywhere for the projector, or only at the known nodes for R
+    if (proj_start && nknown > 0)
+      op.trafo(Fh.as<cx<T>>(), v, st, dknown.as<int>(), nknown);
+    else
+      op.trafo(Fh.as<cx<T>>(), v, st);
     cg.red.template rfactor<T>(v, meas.as<cx<T>>(), dkind.as<int>(), M, d1.as<double>(), d2.as<double>(), st);
     if (!proj_start)
       hipLaunchKernelGGL(k_projector<T>, dim3(cdiv(MM, 256)), dim3(256), 0, st, v, meas.as<cx<T>>(),

@@ -51,7 +51,8 @@ struct NfftOp {
   DevBuf X, grid;  // [N][n][P], [n][n][P]
   size_t bytes() const;
   void init(int N, int n, int W, const double *x, int M, int P);
-  void trafo(const cx<T> *fhat, cx<T> *f, hipStream_t st);
+  // f = A fhat; with sel only at the msel nodes sel[] (other rows untouched)
+  void trafo(const cx<T> *fhat, cx<T> *f, hipStream_t st, const int *sel = nullptr, int msel = 0);
   void adjoint(const cx<T> *f, cx<T> *fhat, hipStream_t st);
 };
 

@@ -217,7 +217,7 @@ static int plane_group_log2(int P, int cap = 64) {
 static const int g_gather_lanes = getenv("NUDIM_GATHER_LANES") ? atoi(getenv("NUDIM_GATHER_LANES")) : 64;
 
 template <typename T>
-void NfftOp<T>::trafo(const cx<T> *fhat, cx<T> *f, hipStream_t st) {
+void NfftOp<T>::trafo(const cx<T> *fhat, cx<T> *f, hipStream_t st, const int *sel, int msel) {
   const long long NP = (long long)N * P, nP = (long long)n * P;
   const double *d = dec.as<double>();
   // rows: deconvolve, place at (k - N/2) mod n, FFT along x
@@ -229,15 +229,16 @@ void NfftOp<T>::trafo(const cx<T> *fhat, cx<T> *f, hipStream_t st) {
   LineIO co{P, nP, n, 0, nullptr, nullptr};
   fft.run(X.as<cx<T>>(), grid.as<cx<T>>(), n, ci, co, -1, P, st);
   int PCs = plane_group_log2(P);
-  dim3 gr(cdiv(M, kThreads >> PCs), P >> PCs);
+  const int Mc = sel ? msel : M;
+  dim3 gr(cdiv(Mc, kThreads >> PCs), P >> PCs);
   // algorithmic bytes: grid read once, node values written, node coordinates
-  ProfScope ps(KC_INTERP, st, ((double)n * n * sizeof(cx<T>) + (double)M * sizeof(cx<T>) + 2.0 * M * sizeof(T)) * P);
+  ProfScope ps(KC_INTERP, st, ((double)n * n * sizeof(cx<T>) + (double)Mc * sizeof(cx<T>) + 2.0 * Mc * sizeof(T)) * P);
   const cx<T> *gp = grid.as<cx<T>>();
   const int *q0 = c0.as<int>(), *q1 = c1.as<int>();
   const double *p0 = psi0.as<double>(), *p1 = psi1.as<double>();
 #define NUDIM_INTERP_W(WW)                                                                                      \
   case WW:                                                                                                      \
-    hipLaunchKernelGGL((k_interp_w<T, WW>), gr, dim3(kThreads), 0, st, gp, f, q0, q1, p0, p1, M, n, P, PCs); \
+    hipLaunchKernelGGL((k_interp_w<T, WW>), gr, dim3(kThreads), 0, st, gp, f, q0, q1, p0, p1, Mc, n, P, PCs, sel); \
     break;
   switch (W) {
     NUDIM_INTERP_W(4)
@@ -248,7 +249,7 @@ void NfftOp<T>::trafo(const cx<T> *fhat, cx<T> *f, hipStream_t st) {
     NUDIM_INTERP_W(14)
     NUDIM_INTERP_W(16)
     default:
-      hipLaunchKernelGGL(k_interp<T>, gr, dim3(kThreads), 0, st, gp, f, q0, q1, p0, p1, M, n, W, P, PCs);
+      hipLaunchKernelGGL(k_interp<T>, gr, dim3(kThreads), 0, st, gp, f, q0, q1, p0, p1, Mc, n, W, P, PCs, sel);
   }
 #undef NUDIM_INTERP_W
   NUDIM_HIP(hipGetLastError());
@@ -665,7 +666,7 @@ void Cgnr<T>::solve(const cx<T> *y, cx<T> *fhat, int nsteps, bool f0_zero, hipSt
 // a final update, each followed by the fixed-order sum of the two norms
 // ---------------------------------------------------------------------------
 
-int tv_blocks(int ny, int nx, int nz) { return ny * 64; }
+int tv_blocks(int ny, int nx, int nz) { return ny * 128; }
 
 template <typename T>
 T *tv_sweeps(T *u0, T *u1, T *g0, T *g1, int ny, int nx, int nz, int ylo, int yhi, float lam, int niter,
@@ -680,7 +681,10 @@ T *tv_sweeps(T *u0, T *u1, T *g0, T *g1, int ny, int nx, int nz, int ylo, int yh
   const int nrows = yhi - ylo + 1;
   const int gx = (int)std::min<long long>(cdiv(rowlen, kThreads), 64);
   const int np_ = nrows * gx;
-  NUDIM_CHECK(np_ <= nb, "TV partial-sum buffer too small");
+  // sweeps on z quads when nz % 4 == 0 (k_tv_fused4): one quad per thread
+  const bool quads = nz % 4 == 0;
+  const int gx4 = (int)std::min<long long>(cdiv(rowlen / 4, kThreads), 128);
+  NUDIM_CHECK(np_ <= nb && nrows * gx4 <= nb, "TV partial-sum buffer too small");
   NUDIM_CHECK((double)V * sizeof(T) < 4294967296.0, "TV kernels address volumes of fewer than 2^32 bytes");
   const dim3 grid(gx, nrows);
   // rows outside the band keep their values in both buffers
@@ -694,10 +698,14 @@ T *tv_sweeps(T *u0, T *u1, T *g0, T *g1, int ny, int nx, int nz, int ylo, int yh
   for (int s = 0; s + 1 < niter; s++) {
     {
       ProfScope ps(KC_TV, st, 2.0 * vb);
-      hipLaunchKernelGGL(k_tv_fused<T>, grid, dim3(kThreads), 0, st, uc, gc, un, gn, ny, nx, nz, ylo, yhi, sums, lamd,
-                         part_g, part_a);
+      if (quads)
+        hipLaunchKernelGGL(k_tv_fused4<T>, dim3(gx4, nrows), dim3(kThreads), 0, st, uc, gc, un, gn, ny, nx, nz, ylo,
+                           yhi, sums, lamd, part_g, part_a);
+      else
+        hipLaunchKernelGGL(k_tv_fused<T>, grid, dim3(kThreads), 0, st, uc, gc, un, gn, ny, nx, nz, ylo, yhi, sums,
+                           lamd, part_g, part_a);
     }
-    hipLaunchKernelGGL(k_sum_scalars, dim3(1), dim3(256), 0, st, part_g, part_a, np_, sums);
+    hipLaunchKernelGGL(k_sum_scalars, dim3(1), dim3(256), 0, st, part_g, part_a, quads ? nrows * gx4 : np_, sums);
     std::swap(uc, un);
     std::swap(gc, gn);
   }
