@@ -33,10 +33,12 @@ from nudim_amd import phantom  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 # profiler classes of the library (nudim_profile) and the kernels they time
 KC_NAMES = {1: "k_toep<float,32,24,1> (Toeplitz column pass)", 2: "k_interp_w<float,8>",
-            3: "k_gather_tile<float,8>", 4: "k_fft2<float> (NFFT grid passes)", 5: "k_tv_fused<float>",
-            6: "k_zfft<float>", 7: "k_toep<float,32,24,0> (Toeplitz row forward + CG update)",
-            8: "k_toep<float,32,24,2> (Toeplitz row inverse + <p,q>)"}
-KC_CANDIDATES = (7, 1, 8, 3, 5, 4, 2)
+            3: "k_gather_chunk<float,8> (NFFT adjoint)", 4: "k_fft2<float> (NFFT grid passes)",
+            5: "k_tv_fused4<float>", 6: "k_zfft<float>",
+            7: "k_toep<float,32,24,0> (Toeplitz row forward + CG update)",
+            8: "k_toep<float,32,24,2> (Toeplitz row inverse + <p,q>)",
+            9: "k_toep<float,32,24,3> (two-input column pass, CG start)", 10: "k_cg_zupdate<float>"}
+KC_CANDIDATES = (7, 1, 8, 5, 10, 9, 4, 2, 6, 3)
 PROFILES = os.path.join(ROOT, "profiles")
 
 

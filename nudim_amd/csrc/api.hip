@@ -353,11 +353,20 @@ struct Job : JobBase {
     NUDIM_HIP(hipStreamSynchronize(st));
   }
 
-  void zfft_inverse(const cx<T> *g, T *out) {
+  // out = z IFFT of g (+ alpha p per plane when pv is non-null: the CG's last
+  // f += alpha p, folded into the loads)
+  void zfft_inverse(const cx<T> *g, T *out, const cx<T> *pv = nullptr, const double *alpha = nullptr) {
     int Gs = line_chunk_log2(P, 1 << 30, (int)sizeof(cx<T>), FftEpt<T>::value);
     long long nlines = (long long)N * N;
     size_t lds = (size_t)P * ((1 << Gs) + 1) * sizeof(cx<T>);
-    ProfScope ps(KC_ZFFT, st, 2.0 * V * sizeof(cx<T>));
+    ProfScope ps(KC_ZFFT, st, (double)V * sizeof(cx<T>) * (pv ? 2.0 : 1.0) + (double)V * sizeof(T));
+    if (zfft2_launch<T>(P, true, nlines, nullptr, g, pv, alpha, nullptr, nullptr, out, twz.as<cx<T>>(), st)) {
+      NUDIM_HIP(hipGetLastError());
+      return;
+    }
+    if (pv)
+      hipLaunchKernelGGL(k_axpy_planes<T>, dim3(cdiv(NN, 256)), dim3(256), 0, st, const_cast<cx<T> *>(g), pv, alpha,
+                         1.0, NN, P);
     hipLaunchKernelGGL(k_zfft_inv<T>, dim3(cdiv(nlines, 1 << Gs)), dim3(kFftThreads), lds, st, g, out, fdz,
                        twz.as<cx<T>>(), Gs, nlines);
     NUDIM_HIP(hipGetLastError());
@@ -367,7 +376,11 @@ struct Job : JobBase {
     int Gs = line_chunk_log2(P, 1 << 30, (int)sizeof(cx<T>), FftEpt<T>::value);
     long long nlines = (long long)N * N;
     size_t lds = (size_t)P * ((1 << Gs) + 1) * sizeof(cx<T>);
-    ProfScope ps(KC_ZFFT, st, 2.0 * V * sizeof(cx<T>));
+    ProfScope ps(KC_ZFFT, st, (double)V * sizeof(T) + (f0 ? 2.0 : 1.0) * V * sizeof(cx<T>));
+    if (zfft2_launch<T>(P, false, nlines, in, nullptr, nullptr, nullptr, fh, f0, nullptr, twz.as<cx<T>>(), st)) {
+      NUDIM_HIP(hipGetLastError());
+      return;
+    }
     hipLaunchKernelGGL(k_zfft_fwd<T>, dim3(cdiv(nlines, 1 << Gs)), dim3(kFftThreads), lds, st, in, fh, f0, fdz,
                        twz.as<cx<T>>(), Gs, nlines);
     NUDIM_HIP(hipGetLastError());
@@ -435,11 +448,14 @@ struct Job : JobBase {
     if (proj_start) {
       // z = b_K + T_U Fh - T f0, then the CG steps
       toep.apply_start(Fh.as<cx<T>>(), F0.as<cx<T>>(), bk.as<cx<T>>(), cg.z.template as<cx<T>>(), st);
-      cg.solve_from_residual(F0.as<cx<T>>(), 10, st);
+      // the last step's f += alpha p happens in the z transform's loads
+      // (f itself is not needed afterwards: the next pass rebuilds f0)
+      cg.solve_from_residual(F0.as<cx<T>>(), 10, st, true);
+      zfft_inverse(F0.as<cx<T>>(), img, cg.pv.template as<cx<T>>(), cg.alpha.template as<double>());
     } else {
       cg.solve(y.as<cx<T>>(), F0.as<cx<T>>(), 10, false, st);
+      zfft_inverse(F0.as<cx<T>>(), img);
     }
-    zfft_inverse(F0.as<cx<T>>(), img);
     if (prm.denoise) {
       if (nn == dump_nn && dump_before) copy_to_host_ref(img, dump_before);
       if (prm.method == NUDIM_DIFFUSION)
@@ -708,7 +724,7 @@ extern "C" int nudim_set_dump(nudim_ctx *c, int nn, double *before, double *afte
 }
 
 extern "C" int nudim_profile(nudim_ctx *c, int kernel_class) {
-  if (!c || kernel_class < 0 || kernel_class > KC_TOEPLITZ_ROWI) return fail(NUDIM_EINVAL, "invalid profile request");
+  if (!c || kernel_class < 0 || kernel_class > KC_CG_UPDATE) return fail(NUDIM_EINVAL, "invalid profile request");
   c->prof.reset(kernel_class);
   return NUDIM_OK;
 }

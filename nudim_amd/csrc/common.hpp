@@ -109,6 +109,8 @@ enum KernelClass {
   KC_ZFFT = 6,          // z-axis transforms
   KC_TOEPLITZ_ROWF = 7, // Toeplitz row forward pass with the CG prologue
   KC_TOEPLITZ_ROWI = 8, // Toeplitz row inverse pass with the <p, q> row sums
+  KC_TOEPLITZ_COL2 = 9, // two-input column pass of the projected CG start
+  KC_CG_UPDATE = 10,    // CG vector update z -= alpha q with |z|^2
 };
 
 // Records an event pair around every launch of one kernel class and

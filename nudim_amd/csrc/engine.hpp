@@ -24,6 +24,12 @@ void toep_launch(int kind, int n, int nlines, const cx<T> *in, cx<T> *out, long 
                  long long out_so, long long out_sk, int P, hipStream_t st, const cx<T> *tw, const T *mid,
                  const ToepCg<T> &cg, int pstride);
 
+// z-axis transforms of [line][z] volumes on the fixed-length kernels
+// (k_zfft2, fft2.hpp); false when n has no specialisation
+template <typename T>
+bool zfft2_launch(int n, bool inv, long long nlines, const T *img_in, const cx<T> *g_in, const cx<T> *pv,
+                  const double *alpha, cx<T> *fh, cx<T> *f0, T *img_out, const cx<T> *tw, hipStream_t st);
+
 // choose log2 of the plane chunk G of a line transform of length n over P planes
 int line_chunk_log2(int n, int P, int elem_bytes, int ept);
 
@@ -85,7 +91,7 @@ struct ToeplitzOp {
 // a [ny][nx][nz] volume held in u0; u1, g0, g1 are volume-sized scratch, and
 // g0, g1 must hold zeros outside rows ylo..yhi (clear them once when they are
 // allocated).  part_g and part_a hold nb >= tv_blocks(ny, nx, nz) partial
-// sums.  Returns the buffer (u0 or u1) that holds the result.
+// sums.  The result is left in u0 (returned); u1's rows are scratch.
 int tv_blocks(int ny, int nx, int nz);
 template <typename T>
 T *tv_sweeps(T *u0, T *u1, T *g0, T *g1, int ny, int nx, int nz, int ylo, int yhi, float lam, int niter,
@@ -121,8 +127,9 @@ struct Cgnr {
   // solver_before_loop_complex + nsteps loop_one_step (fhat in/out; f0_zero skips A*0)
   void solve(const cx<T> *y, cx<T> *fhat, int nsteps, bool f0_zero, hipStream_t st);
   // the same on the fused Toeplitz passes with the starting residual given
-  // in z (z = A^H W (y - A fhat))
-  void solve_from_residual(cx<T> *fhat, int nsteps, hipStream_t st);
+  // in z (z = A^H W (y - A fhat)).  With defer_last the last step's
+  // fhat += alpha p is left to the caller (p in pv, alpha per plane in alpha).
+  void solve_from_residual(cx<T> *fhat, int nsteps, hipStream_t st, bool defer_last = false);
 };
 
 }  // namespace nudim

@@ -108,6 +108,39 @@ template void toep_launch<double>(int, int, int, const cx<double> *, cx<double> 
                                   long long, int, hipStream_t, const cx<double> *, const double *,
                                   const ToepCg<double> &, int);
 
+// z-axis transforms (k_zfft2) for n in {64, 128, 256, 512}; false when no
+// fixed-length kernel exists (the caller then uses the generic kernels)
+template <typename T>
+bool zfft2_launch(int n, bool inv, long long nlines, const T *img_in, const cx<T> *g_in, const cx<T> *pv,
+                  const double *alpha, cx<T> *fh, cx<T> *f0, T *img_out, const cx<T> *tw, hipStream_t st) {
+#define NUDIM_ZFFT(NN, A, B)                                                                                     \
+  case NN: {                                                                                                     \
+    const dim3 grid((unsigned)((nlines + 256 / A - 1) / (256 / A)));                                            \
+    if (inv)                                                                                                     \
+      hipLaunchKernelGGL((k_zfft2<T, A, B, true>), grid, dim3(256), 0, st, img_in, g_in, pv, alpha, fh, f0,      \
+                         img_out, tw, nlines);                                                                   \
+    else                                                                                                         \
+      hipLaunchKernelGGL((k_zfft2<T, A, B, false>), grid, dim3(256), 0, st, img_in, g_in, pv, alpha, fh, f0,     \
+                         img_out, tw, nlines);                                                                   \
+    return true;                                                                                                 \
+  }
+  if (!g_use_fft2) return false;
+  switch (n) {
+    NUDIM_ZFFT(64, 8, 8)
+    NUDIM_ZFFT(128, 16, 8)
+    NUDIM_ZFFT(256, 16, 16)
+    NUDIM_ZFFT(512, 32, 16)
+    default: return false;
+  }
+#undef NUDIM_ZFFT
+}
+
+template bool zfft2_launch<float>(int, bool, long long, const float *, const cx<float> *, const cx<float> *,
+                                  const double *, cx<float> *, cx<float> *, float *, const cx<float> *, hipStream_t);
+template bool zfft2_launch<double>(int, bool, long long, const double *, const cx<double> *, const cx<double> *,
+                                   const double *, cx<double> *, cx<double> *, double *, const cx<double> *,
+                                   hipStream_t);
+
 // Returns false when no fixed-length kernel exists for n (the caller then
 // uses the generic Stockham kernel).
 template <typename T>

@@ -536,4 +536,84 @@ k_toep(const cx<T> *in, cx<T> *out, long long in_so, long long in_sk, long long 
   }
 }
 
+// z-axis transforms of a [line][z] volume (z contiguous, n = N1 N2) as the
+// four-step transform above with lanes along z: G = 256 / N1 lines per
+// block, thread (g, r) holds inputs r + N2 j1 (r < N2) in step A and outputs
+// r + N1 k2 (r < N1) in step B, so every load and store of a line is one
+// contiguous run.  S(z) = (z + n/2) mod n, the reference's z shift, maps to
+// a compile-time permutation of the register index: S(r + N2 j1) =
+// r + N2 ((j1 + N1/2) mod N1), S^-1(k1 + N1 k2) = k1 + N1 ((k2 + N2/2) mod N2).
+//   forward (INV false): d[z] = img[S(z)];  D = DFT_-(d);  fh[p] = D[S(p)];
+//                        f0[z] = d[z] (when f0 is non-null)
+//   inverse (INV true):  d[k] = g[S(k)] (+ alpha[S(k)] pv[S(k)] when pv is
+//                        non-null);  e = DFT_+(d);  img[z] = Re e[S(z)] / n
+// tw holds exp(-2 pi i e / n), e < n.
+template <typename T, int N1, int N2, bool INV>
+__global__ void __launch_bounds__(256)
+k_zfft2(const T *__restrict__ img_in, const cx<T> *__restrict__ g_in, const cx<T> *__restrict__ pv,
+        const double *__restrict__ alpha, cx<T> *__restrict__ fh, cx<T> *__restrict__ f0, T *__restrict__ img_out,
+        const cx<T> *__restrict__ tw, long long nlines) {
+  constexpr int n = N1 * N2, G = 256 / N1, RS = G * N2 + 1;
+  constexpr int S = INV ? +1 : -1;
+  static_assert(256 % N1 == 0 && N1 >= N2 && N2 % 2 == 0 && N1 % 2 == 0, "lane layout");
+  __shared__ cx<T> lx[N1 * RS];
+  __shared__ cx<T> stw[n];
+  const int tid = threadIdx.x, r = tid % N1, g = tid / N1;
+  for (int i = tid; i < n; i += 256) stw[i] = tw[i];
+  const long long line = (long long)xcd_block(blockIdx.x, gridDim.x) * G + g;
+  const bool act = line < nlines;
+  const bool actA = act && r < N2;
+  const long long base = line * n;
+  cx<T> a[N1];
+  if (actA) {
+#pragma unroll
+    for (int j1 = 0; j1 < N1; j1++) {
+      const int zs = r + N2 * ((j1 + N1 / 2) % N1);
+      if constexpr (!INV) {
+        const T v = img_in[base + zs];
+        a[j1] = mk(v, (T)0);
+      } else {
+        a[j1] = g_in[base + zs];
+      }
+    }
+    if constexpr (INV) {
+      if (pv) {
+#pragma unroll
+        for (int j1 = 0; j1 < N1; j1++) {
+          const int zs = r + N2 * ((j1 + N1 / 2) % N1);
+          a[j1] = a[j1] + (T)alpha[zs] * pv[base + zs];
+        }
+      }
+    } else {
+      if (f0) {
+#pragma unroll
+        for (int j1 = 0; j1 < N1; j1++) f0[base + r + N2 * j1] = a[j1];
+      }
+    }
+    dft_reg<T, N1, S>(a);
+  }
+  __syncthreads();  // twiddle table complete
+  if (actA) {
+#pragma unroll
+    for (int k1 = 0; k1 < N1; k1++) {
+      cx<T> w = stw[r * k1];
+      if (S > 0) w = conjx(w);
+      lx[k1 * RS + g * N2 + r] = a[k1] * w;
+    }
+  }
+  __syncthreads();
+  cx<T> b[N2];
+#pragma unroll
+  for (int j2 = 0; j2 < N2; j2++) b[j2] = lx[r * RS + g * N2 + j2];
+  dft_reg<T, N2, S>(b);
+  if (act) {
+#pragma unroll
+    for (int k2 = 0; k2 < N2; k2++) {
+      const int p = r + N1 * ((k2 + N2 / 2) % N2);
+      if constexpr (!INV) fh[base + p] = b[k2];
+      else img_out[base + p] = b[k2].x / (T)n;
+    }
+  }
+}
+
 }  // namespace nudim

@@ -609,9 +609,10 @@ k_tv_grad(const T *__restrict__ u, T *__restrict__ grad, int ny, int nx, int nz,
   }
 }
 
-// u -= lam * sqrt(dA) * (grad / sqrt(sum grad^2))  (denoise.c:125-142)
+// dst = u - lam * sqrt(dA) * (grad / sqrt(sum grad^2))  (denoise.c:125-142)
+// over the band rows (dst may be u)
 template <typename T>
-__global__ void k_tv_update(T *__restrict__ u, const T *__restrict__ grad, int nx, int nz, int ylo, int yhi,
+__global__ void k_tv_update(T *dst, const T *u, const T *__restrict__ grad, int nx, int nz, int ylo, int yhi,
                             const double *__restrict__ sums, double lam) {
   const long long rowlen = (long long)nx * nz;
   const long long total = (long long)(yhi - ylo + 1) * rowlen;
@@ -623,7 +624,7 @@ __global__ void k_tv_update(T *__restrict__ u, const T *__restrict__ grad, int n
   T gn;
   if constexpr (sizeof(T) == 4) gn = grad[o] * (T)(1.0 / s1d);
   else gn = grad[o] / s1d;
-  u[o] = u[o] - c * gn;
+  dst[o] = u[o] - c * gn;
 }
 
 __global__ void k_sum_scalars(const double *__restrict__ part_g, const double *__restrict__ part_a, int nb,

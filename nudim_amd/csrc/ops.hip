@@ -21,8 +21,18 @@ __global__ void k_sum_partials(const double *__restrict__ part, int nb, int P, d
   const int pl = threadIdx.x & 7, lane = threadIdx.x >> 3;
   const int p = blockIdx.x * 8 + pl;
   double s = 0;
+  // four strided partials in flight, added in index order
   if (p < P)
-    for (int b = lane; b < nb; b += 32) s += part[(size_t)b * P + p];
+    for (int b0 = lane; b0 < nb; b0 += 32 * 4) {
+      double v[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int b = b0 + 32 * k;
+        v[k] = b < nb ? part[(size_t)b * P + p] : 0.0;
+      }
+#pragma unroll
+      for (int k = 0; k < 4; k++) s += v[k];
+    }
   sh[threadIdx.x] = s;
   __syncthreads();
   for (int h = 16; h > 0; h >>= 1) {
@@ -52,9 +62,21 @@ __global__ void k_sum_scalars(const double *__restrict__ part_g, const double *_
                               double *__restrict__ sums) {
   __shared__ double sa[256], sb[256];
   double a = 0, b = 0;
-  for (int i = threadIdx.x; i < nb; i += 256) {
-    a += part_g[i];
-    b += part_a[i];
+  // eight strided values in flight, added in index order (the order of a
+  // plain strided loop: adding 0 past the end changes nothing)
+  for (int i0 = threadIdx.x; i0 < nb; i0 += 256 * 8) {
+    double va[8], vb[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const int i = i0 + k * 256;
+      va[k] = i < nb ? part_g[i] : 0.0;
+      vb[k] = i < nb ? part_a[i] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      a += va[k];
+      b += vb[k];
+    }
   }
   sa[threadIdx.x] = a;
   sb[threadIdx.x] = b;
@@ -390,15 +412,19 @@ void ToeplitzOp<T>::apply_start(const cx<T> *a, const cx<T> *b, const cx<T> *add
   for (int p0 = 0; p0 < P; p0 += pc) {
     cx<T> *xa = X2.as<cx<T>>() + (long long)p0 * N * L, *xb = X.as<cx<T>>() + (long long)p0 * N * L;
     ToepCg<T> c{};
+    // algorithmic bytes of a plain row pass: input read (N x N), X written (N x L)
+    const double rowf_bytes = (double)(N + L) * N * pc * sizeof(cx<T>);
     {
-      // algorithmic bytes: input read (N x N), X written (N x L), twice
-      ProfScope ps(KC_TOEPLITZ_ROWF, st, 2.0 * (N + L) * N * pc * sizeof(cx<T>));
+      ProfScope ps(KC_TOEPLITZ_ROWF, st, rowf_bytes);
       toep_launch<T>(TK_ROWF, L, N, a + p0, xa, NP, P, LC, pc, pc, st, tw, nullptr, c, P);
+    }
+    {
+      ProfScope ps(KC_TOEPLITZ_ROWF, st, rowf_bytes);
       toep_launch<T>(TK_ROWF, L, N, b + p0, xb, NP, P, LC, pc, pc, st, tw, nullptr, c, P);
     }
     {
       // X2 and X read, X written (N x L each), both kernels read
-      ProfScope ps(KC_TOEPLITZ_COL, st, 3.0 * N * L * pc * sizeof(cx<T>) + 2.0 * L * L * sizeof(T));
+      ProfScope ps(KC_TOEPLITZ_COL2, st, 3.0 * N * L * pc * sizeof(cx<T>) + 2.0 * L * L * sizeof(T));
       c.in2 = xb;
       c.mid2 = khat_re.template as<T>();
       toep_launch<T>(TK_COL2, L, L, xa, xb, pc, LC, pc, LC, pc, st, tw, khat2_re.template as<T>(), c, P);
@@ -557,7 +583,7 @@ void Cgnr<T>::init(int N_, int M_, int P_, int form_, NfftOp<T> *op_, ToeplitzOp
 }
 
 template <typename T>
-void Cgnr<T>::solve_from_residual(cx<T> *fhat, int nsteps, hipStream_t st) {
+void Cgnr<T>::solve_from_residual(cx<T> *fhat, int nsteps, hipStream_t st, bool defer_last) {
   NUDIM_CHECK(form == NUDIM_CG_TOEPLITZ && toep->fused(), "the fused Toeplitz passes are not available");
   const long long NN = (long long)N * N * P;
   const unsigned bN = cdiv(NN, 256);
@@ -578,14 +604,18 @@ void Cgnr<T>::solve_from_residual(cx<T> *fhat, int nsteps, hipStream_t st) {
     ToepCg<T> cg{zp, pp, fhat, zz, it ? zz_old : nullptr, it ? al : nullptr, rowpart.as<double>()};
     toep->apply_cg(cg, qp, st);
     hipLaunchKernelGGL(k_sum_partials, dim3(cdiv(P, 8)), dim3(256), 0, st, rowpart.as<double>(), N, P, pq);
-    hipLaunchKernelGGL(k_cg_zupdate<T>, rg, dim3(kThreads), 0, st, zp, qp, zz, pq, al, (long long)N * N, P, PCs,
-                       red.part.as<double>());
+    {
+      // algorithmic bytes: z, q read, z written
+      ProfScope ps(KC_CG_UPDATE, st, 3.0 * NN * sizeof(cx<T>));
+      hipLaunchKernelGGL(k_cg_zupdate<T>, rg, dim3(kThreads), 0, st, zp, qp, zz, pq, al, (long long)N * N, P, PCs,
+                         red.part.as<double>());
+    }
     hipLaunchKernelGGL(k_sum_partials, dim3(cdiv(P, 8)), dim3(256), 0, st, red.part.as<double>(), kRedBlocks, P,
                        zz_old);
     NUDIM_HIP(hipGetLastError());
     std::swap(zz, zz_old);
   }
-  if (nsteps > 0) {
+  if (nsteps > 0 && !defer_last) {
     hipLaunchKernelGGL(k_axpy_planes<T>, dim3(bN), dim3(256), 0, st, fhat, pp, al, 1.0, NN, P);
     NUDIM_HIP(hipGetLastError());
   }
@@ -687,8 +717,14 @@ T *tv_sweeps(T *u0, T *u1, T *g0, T *g1, int ny, int nx, int nz, int ylo, int yh
   NUDIM_CHECK(np_ <= nb && nrows * gx4 <= nb, "TV partial-sum buffer too small");
   NUDIM_CHECK((double)V * sizeof(T) < 4294967296.0, "TV kernels address volumes of fewer than 2^32 bytes");
   const dim3 grid(gx, nrows);
-  // rows outside the band keep their values in both buffers
-  hipLaunchKernelGGL(k_copy_real<T>, dim3(cdiv(V, 256)), dim3(256), 0, st, u1, u0, V);
+  // the sweeps write band rows only and read one row on each side of the
+  // band, so u1 needs those two rows of u0; the last update writes into u0
+  if (ylo > 0)
+    hipLaunchKernelGGL(k_copy_real<T>, dim3(cdiv(rowlen, 256)), dim3(256), 0, st, u1 + (ylo - 1) * rowlen,
+                       u0 + (ylo - 1) * rowlen, rowlen);
+  if (yhi < ny - 1)
+    hipLaunchKernelGGL(k_copy_real<T>, dim3(cdiv(rowlen, 256)), dim3(256), 0, st, u1 + (yhi + 1) * rowlen,
+                       u0 + (yhi + 1) * rowlen, rowlen);
   {
     ProfScope ps(KC_TV, st, vb);
     hipLaunchKernelGGL(k_tv_grad<T>, grid, dim3(kThreads), 0, st, u0, g0, ny, nx, nz, ylo, yhi, part_g, part_a);
@@ -711,10 +747,11 @@ T *tv_sweeps(T *u0, T *u1, T *g0, T *g1, int ny, int nx, int nz, int ylo, int yh
   }
   {
     ProfScope ps(KC_TV, st, vb);
-    hipLaunchKernelGGL(k_tv_update<T>, dim3(cdiv(band, 256)), dim3(256), 0, st, uc, gc, nx, nz, ylo, yhi, sums, lamd);
+    hipLaunchKernelGGL(k_tv_update<T>, dim3(cdiv(band, 256)), dim3(256), 0, st, u0, uc, gc, nx, nz, ylo, yhi, sums,
+                       lamd);
   }
   NUDIM_HIP(hipGetLastError());
-  return uc;
+  return u0;
 }
 
 template float *tv_sweeps<float>(float *, float *, float *, float *, int, int, int, int, int, float, int, double *,

@@ -403,3 +403,35 @@ def test_projected_cg_start(hip, orc, monkeypatch, np_, nz, weight, nhio):
         assert dr < (1e-7 if nhio == 0 else 1e-5) and dro < 1e-4, (dr, dro)
     else:
         assert eo < 2 * ea + 1e-4, (eo, ea)
+
+
+@pytest.mark.parametrize("np_,nz", [(64, 128), (64, 256), (32, 512)])
+def test_z_transform_lengths(hip, orc, np_, nz):
+    """The z transforms at the lengths the fixed-length z kernels cover
+    (with the z shift of reconstruction.c:536-621 folded into their index
+    maps, and in the FP32 form the CG's last f += alpha p folded into the
+    inverse's loads): FP64 initial volume within 1e-11 of the oracle and one
+    ER pass within 1e-10; FP32 within 1e-4 after two ER passes (np = 64
+    runs the FP32 form on the fused Toeplitz passes, where the fold applies)."""
+    rng = np.random.default_rng(nz)
+    dens = blob_phantom((nz, np_, np_), 12, rng, sig=(1.5, 3.0), slab=np_ // 2)
+    proj = orc.project_2d1d(dens, np_, -60, 60, 5)
+    kw = dict(padangle=1.0, weight=1, nhio=0, ner=2)
+    op = orc.params(np_, nz, np_, -60, 60, 5, **kw)
+    slices, kind = orc.assemble(op, proj)
+    img0 = orc.initial(op, slices)
+    img1, r1 = orc.iterate(op, img0, slices, kind, 0)
+    img2, r2 = orc.iterate(op, img1, slices, kind, 1)
+    with hip.Reconstruction(hip.make_params(np_, nz, np_, -60, 60, 5, precision=hip.FP64_PLAIN, **kw)) as job:
+        job.load_projections(proj)
+        job.initial()
+        assert rel(job.volume(), img0) < 1e-11
+        r = job.iterate(1)
+        assert rel(job.volume(), img1) < 1e-10
+        assert abs(r[0] - r1) < 1e-12
+    with hip.Reconstruction(hip.make_params(np_, nz, np_, -60, 60, 5, precision=hip.FP32, **kw)) as job:
+        job.load_projections(proj)
+        job.initial()
+        r = job.iterate(2)
+        assert rel(job.volume(), img2) < 1e-4
+        assert abs(r[1] - r2) < 1e-5
