@@ -50,11 +50,21 @@ def rel(a, b):
     return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
 
 
+NFFT_CHOICE = {}
+
+
 def make_job(prm, proj, precision, **kw):
     p = nd.make_params(prm["nx"], prm["nz"], prm["np_"], prm["gam_min"], prm["gam_max"], prm["interval"],
                        padangle=prm["padangle"], weight=prm["weight"], nhio=prm["nhio"], ner=prm["ner"],
                        denoise=prm.get("denoise", False), method=nd.TV, height=prm.get("height"),
                        center=prm.get("center"), precision=precision, **kw)
+    t0 = time.perf_counter()
+    n, w = nd.autotune(p)
+    key = "f32" if precision == nd.FP32 else "f64"
+    if key not in NFFT_CHOICE:
+        NFFT_CHOICE[key] = {"n": n, "W": w, "choice_s": round(time.perf_counter() - t0, 3)}
+        log("  NFFT n=%d, W=%d (%s, %.2f s)" % (n, w, "measured" if precision == nd.FP32 else "reference sizes",
+                                                time.perf_counter() - t0))
     t0 = time.perf_counter()
     job = nd.Reconstruction(p)
     job.load_projections(proj)
@@ -188,6 +198,16 @@ def serve(args, precision):
     log("  serving: %d jobs (%s), inputs generated in %.1f s" % (
         args.serve_jobs, ", ".join("%s x%d" % (names[c], int((cls == c).sum())) for c in sorted(set(cls))),
         time.perf_counter() - t0))
+    # start-up calibration of each job class's NFFT sizing (nudim_autotune),
+    # as a server does before it takes requests
+    t0 = time.perf_counter()
+    for c in sorted(set(cls)):
+        prm = inputs[c][0][0]
+        nd.autotune(nd.make_params(prm["nx"], prm["nz"], prm["np_"], prm["gam_min"], prm["gam_max"],
+                                   prm["interval"], padangle=prm["padangle"], weight=prm["weight"],
+                                   precision=precision))
+    calib_s = time.perf_counter() - t0
+    log("  serving: NFFT sizing measured for %d job classes in %.1f s" % (len(set(cls)), calib_s))
     reps = {}
     policy = nd.SERVE_FIFO if args.serve_fifo else nd.SERVE_SHORTEST
     with nd.Server(max_active=args.serve_slots, policy=policy) as srv:
@@ -226,6 +246,7 @@ def serve(args, precision):
            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
            "mean_latency_s": float(lat.mean()), "makespan_s": span,
            "p50_latency_s_by_class": {k: float(np.percentile(v, 50)) for k, v in sorted(by.items())},
+           "calibration_s": calib_s,
            "precision": "f32" if precision == nd.FP32 else "f64",
            "policy": "fifo" if args.serve_fifo else "shortest work first, 30 s aging",
            "mix": "%s; seed %d" % (", ".join("%s %.0f%%" % (n, 100 * f) for n, f in phantom.SERVE_MIX),
@@ -414,7 +435,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "c3: 256^3 blob phantom, np=384, nz=256, 71 tilts -70..70 step 2, SNR 2, "
                                "padangle 1, weight, TV height 128 centre 192; one HIO/ER pass",
-                   "grid": [prm["nz"], prm["np_"], prm["np_"]]},
+                   "grid": [prm["nz"], prm["np_"], prm["np_"]], "nfft": NFFT_CHOICE or None},
         "fp64_plain_passes_per_s": rate64,
         "speedup_vs_fp64_plain": (rate32 / rate64) if rate64 else None,
         "accuracy": acc or None,

@@ -99,6 +99,15 @@ int nudim_set_dump(nudim_ctx *ctx, int nn, double *before, double *after);
 size_t nudim_device_bytes(const nudim_ctx *ctx);
 /* bytes a context with these parameters would hold (admission control) */
 int nudim_estimate_bytes(const nudim_params *p, size_t *bytes);
+/* NFFT grid size n and window width W a job with these parameters uses.  For
+ * an FP32 job with nfft_n = nfft_w = 0 (and NUDIM_AUTOTUNE not "0") they are
+ * measured on the device for its node set, once per process and node set: the
+ * fastest pair among the fixed-length FFT sizes up to the default n whose
+ * trafo and adjoint stay within 2e-6 relative L2 of the FP64 NFFT at the
+ * reference's sizes.  nudim_create does the same; calling this first moves
+ * the measurement out of the first job.  (No reference counterpart: the
+ * reference fixes n = 2 nextpow2(np), 14 taps, src/reconstruction.c:393-405.) */
+int nudim_autotune(const nudim_params *p, int *n, int *W);
 /* kernel timing with HIP events on the job's stream during nudim_iterate:
  * class 0 off, 1 Toeplitz column pass, 2 NFFT interpolation, 3 NFFT gather,
  * 4 other line FFT passes, 5 TV sweeps, 6 z-axis transforms, 7 Toeplitz row

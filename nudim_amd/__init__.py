@@ -47,7 +47,7 @@ EXPORTS = [
     "nudim_last_error", "nudim_default_params", "nudim_create", "nudim_destroy", "nudim_sizes",
     "nudim_load_projections", "nudim_load_samples", "nudim_initial", "nudim_iterate",
     "nudim_get_volume", "nudim_set_volume", "nudim_set_dump", "nudim_device_bytes",
-    "nudim_estimate_bytes", "nudim_synchronize", "nudim_profile", "nudim_profile_read", "nudim_nfft_trafo", "nudim_nfft_adjoint",
+    "nudim_estimate_bytes", "nudim_autotune", "nudim_synchronize", "nudim_profile", "nudim_profile_read", "nudim_nfft_trafo", "nudim_nfft_adjoint",
     "nudim_cgnr", "nudim_zfft", "nudim_tv", "nudim_tv_height", "nudim_diffusion2d",
     "nudim_constraint", "nudim_projection_ft", "nudim_project_2d1d",
     "nudim_server_create", "nudim_server_submit", "nudim_server_next", "nudim_server_stats",
@@ -101,6 +101,7 @@ def _declare(L):
         "nudim_set_dump": (i, [v, i, dp, dp]),
         "nudim_device_bytes": (C.c_size_t, [v]),
         "nudim_estimate_bytes": (i, [C.POINTER(Params), C.POINTER(C.c_size_t)]),
+        "nudim_autotune": (i, [C.POINTER(Params), C.POINTER(i), C.POINTER(i)]),
         "nudim_synchronize": (i, [v]),
         "nudim_profile": (i, [v, i]),
         "nudim_profile_read": (i, [v, C.POINTER(d), C.POINTER(C.c_long), C.POINTER(d)]),
@@ -169,6 +170,14 @@ def estimate_bytes(p):
     b = C.c_size_t()
     _check(lib().nudim_estimate_bytes(C.byref(p), C.byref(b)))
     return b.value
+
+
+def autotune(p):
+    """(n, W) of the job's NFFT: measured on the device for FP32 jobs without
+    explicit sizes (once per process and node set), else the fixed choice."""
+    n, w = C.c_int(), C.c_int()
+    _check(lib().nudim_autotune(C.byref(p), C.byref(n), C.byref(w)))
+    return n.value, w.value
 
 
 class Reconstruction:

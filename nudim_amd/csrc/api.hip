@@ -4,7 +4,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -192,6 +194,31 @@ static void make_geometry(const nudim_params *p, Geometry *g) {
                  &g->order);
 }
 
+// Tuned FP32 NFFT sizes per node set (device, np and the angular grid fix the
+// nodes), measured once per process.
+static std::mutex g_tune_mu;
+static std::map<std::vector<double>, NfftTuning> g_tune;
+
+static std::vector<double> tune_key(const nudim_params *p) {
+  return {(double)p->device, (double)p->np, p->gam_min, p->gam_max, p->interval, p->padangle};
+}
+
+static bool autotune_enabled(const nudim_params *p) {
+  const char *e = getenv("NUDIM_AUTOTUNE");
+  return p->precision == NUDIM_FP32 && p->nfft_n <= 0 && p->nfft_w <= 0 && !(e && std::string(e) == "0");
+}
+
+// the tuned (n, W) of the job's node set, measuring it on first use
+static NfftTuning tuned_nfft(const nudim_params *p, const Geometry &g, hipStream_t st) {
+  const auto key = tune_key(p);
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune.find(key);
+  if (it != g_tune.end()) return it->second;
+  NfftTuning t = tune_nfft_fp32(p->np, g.x.data(), g.M, st, nullptr);
+  g_tune[key] = t;
+  return t;
+}
+
 static void resolve_knobs(const nudim_params *p, int *n, int *W, int *form) {
   if (p->precision == NUDIM_FP64_PLAIN) {
     *n = p->nfft_n > 0 ? p->nfft_n : 2 * next_pow2(p->np);
@@ -245,6 +272,11 @@ struct Job : JobBase {
     P = p.nz;
     M = g.M;
     resolve_knobs(&p, &n, &W, &form);
+    if (autotune_enabled(&p)) {
+      const NfftTuning t = tuned_nfft(&p, g, s);
+      n = t.n;
+      W = t.W;
+    }
     V = (long long)N * N * P;
     NN = V;
     MM = (long long)M * P;
@@ -785,6 +817,26 @@ struct TmpStream {
     }
   }
 };
+
+extern "C" int nudim_autotune(const nudim_params *p, int *n, int *W) {
+  API_BEGIN
+  validate(p);
+  NUDIM_CHECK(n && W, "null output");
+  int form;
+  resolve_knobs(p, n, W, &form);
+  if (autotune_enabled(p)) {
+    NUDIM_HIP(hipSetDevice(p->device));
+    Geometry g;
+    make_geometry(p, &g);
+    TmpStream ts;
+    const NfftTuning t = tuned_nfft(p, g, ts.s);
+    *n = t.n;
+    *W = t.W;
+  }
+  return NUDIM_OK;
+  API_END
+}
+
 
 template <typename T>
 static void to_dev_planes(const double *h, long long K, int P, DevBuf &out, hipStream_t st) {

@@ -1,6 +1,8 @@
 // engine.hpp -- device-side operators of one reconstruction job.
 #pragma once
 
+#include <vector>
+
 #include "common.hpp"
 #include "nudim.h"
 #include "fft.hpp"
@@ -29,6 +31,16 @@ void toep_launch(int kind, int n, int nlines, const cx<T> *in, cx<T> *out, long 
 template <typename T>
 bool zfft2_launch(int n, bool inv, long long nlines, const T *img_in, const cx<T> *g_in, const cx<T> *pv,
                   const double *alpha, cx<T> *fh, cx<T> *f0, T *img_out, const cx<T> *tw, hipStream_t st);
+
+// FP32 NFFT sizing for a node set (tune.hip): the fastest measured (n, W)
+// whose trafo and adjoint stay within 2e-6 relative L2 of the FP64 NFFT at
+// the reference's sizes; n <= nice_fft_size(2N).  err / ms < 0: not measured
+// or above the tolerance.  trace (may be null) gets every candidate tried.
+struct NfftTuning {
+  int n, W;
+  double err, ms;
+};
+NfftTuning tune_nfft_fp32(int N, const double *x, int M, hipStream_t st, std::vector<NfftTuning> *trace);
 
 // choose log2 of the plane chunk G of a line transform of length n over P planes
 int line_chunk_log2(int n, int P, int elem_bytes, int ept);
