@@ -210,7 +210,9 @@ def serve(args, precision):
     log("  serving: NFFT sizing measured for %d job classes in %.1f s" % (len(set(cls)), calib_s))
     reps = {}
     policy = nd.SERVE_FIFO if args.serve_fifo else nd.SERVE_SHORTEST
-    with nd.Server(max_active=args.serve_slots, policy=policy) as srv:
+    # small jobs: the 128^3 class (np 192, 60 passes: 2.8e8 work units)
+    with nd.Server(max_active=args.serve_slots, policy=policy, small_cus=args.serve_small_cus,
+                   small_work=5e8) as srv:
         meta = {}
         start = time.perf_counter()
         k = 0
@@ -246,7 +248,7 @@ def serve(args, precision):
            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
            "mean_latency_s": float(lat.mean()), "makespan_s": span,
            "p50_latency_s_by_class": {k: float(np.percentile(v, 50)) for k, v in sorted(by.items())},
-           "calibration_s": calib_s,
+           "calibration_s": calib_s, "small_job_cus": args.serve_small_cus,
            "precision": "f32" if precision == nd.FP32 else "f64",
            "policy": "fifo" if args.serve_fifo else "shortest work first, 30 s aging",
            "mix": "%s; seed %d" % (", ".join("%s %.0f%%" % (n, 100 * f) for n, f in phantom.SERVE_MIX),
@@ -300,7 +302,7 @@ def batch(args):
     return {"volumes": len(inst), "volume_iters": passes, "slots": args.batch_slots,
             "seconds": res[args.batch_slots], "volume_iters_per_s": passes / res[args.batch_slots],
             "seconds_one_at_a_time": res[1], "speedup_over_one_at_a_time": res[1] / res[args.batch_slots],
-            "calibration_s": calib_s, "sizes": sizes}
+            "calibration_s": calib_s, "small_job_cus": args.serve_small_cus, "sizes": sizes}
 
 
 def main():
@@ -319,6 +321,8 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=4, help="jobs in flight at once")
     ap.add_argument("--serve-seed", type=int, default=5)
     ap.add_argument("--serve-fifo", action="store_true", help="admit in arrival order (default: least work first)")
+    ap.add_argument("--serve-small-cus", type=int, default=0,
+                    help="CUs reserved for small jobs (128^3 class) in the serving leg (0: none)")
     ap.add_argument("--serve-fp64", action="store_true", help="also serve the trace on the plain FP64 path")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving leg")
     ap.add_argument("--batch", action="store_true", help="run only the uneven-batch leg (48 volumes)")

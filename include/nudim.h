@@ -63,6 +63,8 @@ typedef struct nudim_params {
   /* denoising variants of src/reconstruction-parallel.c (0 / 0 for the serial program) */
   int tv_full;                        /* TV over the whole volume even with a height band (:766) */
   float diffusion_lambda;             /* aniso_diffusion_2d lambda; 0: 0.25f (:764 passes 0.4) */
+  unsigned int cu_mask[8];            /* CUs the job's kernels may use, bit i = CU i (as
+                                         hipExtStreamCreateWithCUMask); all zero: every CU */
 } nudim_params;
 
 typedef struct nudim_ctx nudim_ctx;
@@ -175,6 +177,11 @@ typedef struct {
   int device;
   int policy;           /* NUDIM_SERVE_* */
   double max_wait_s;    /* aging bound of NUDIM_SERVE_SHORTEST; <= 0: 30 s */
+  int small_cus;        /* > 0: CUs reserved for small jobs; a job of at most small_work runs on
+                           the first small_cus CUs, every other job on the rest (jobs that set
+                           their own cu_mask keep it), and small jobs have max_active slots of
+                           their own */
+  double small_work;    /* np^2 nz passes of the largest small job */
 } nudim_server_opts;
 typedef struct {
   long long job_id;

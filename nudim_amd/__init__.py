@@ -40,7 +40,8 @@ class Params(C.Structure):
                 ("ncenter", C.c_int), ("denoise", C.c_int), ("method", C.c_int),
                 ("nhio", C.c_int), ("ner", C.c_int), ("precision", C.c_int), ("device", C.c_int),
                 ("nfft_n", C.c_int), ("nfft_w", C.c_int), ("cg_form", C.c_int),
-                ("use_graph", C.c_int), ("tv_full", C.c_int), ("diffusion_lambda", C.c_float)]
+                ("use_graph", C.c_int), ("tv_full", C.c_int), ("diffusion_lambda", C.c_float),
+                ("cu_mask", C.c_uint * 8)]
 
 
 EXPORTS = [
@@ -62,7 +63,8 @@ SERVE_FIFO, SERVE_SHORTEST = 0, 1
 class ServerOpts(C.Structure):
     """nudim_server_opts (include/nudim.h)"""
     _fields_ = [("hbm_budget", C.c_longlong), ("max_active", C.c_int), ("device", C.c_int),
-                ("policy", C.c_int), ("max_wait_s", C.c_double)]
+                ("policy", C.c_int), ("max_wait_s", C.c_double), ("small_cus", C.c_int),
+                ("small_work", C.c_double)]
 
 
 class JobReport(C.Structure):
@@ -147,7 +149,7 @@ def _f64(a):
 def make_params(nx, nz, np_, gam_min, gam_max, interval, padangle=1.0, weight=True, random=False,
                 seed=0, height=None, center=None, denoise=False, method=TV, nhio=0, ner=20,
                 precision=FP64_PLAIN, device=0, nfft_n=0, nfft_w=0, cg_form=CG_AUTO,
-                use_graph=False, tv_full=False, diffusion_lambda=0.0):
+                use_graph=False, tv_full=False, diffusion_lambda=0.0, cu_mask=None):
     """Parameters with the meaning of reconstruction's flags (src/reconstruction.c:91-112)."""
     p = Params()
     lib().nudim_default_params(C.byref(p))
@@ -163,6 +165,9 @@ def make_params(nx, nz, np_, gam_min, gam_max, interval, padangle=1.0, weight=Tr
     p.precision, p.device = precision, device
     p.nfft_n, p.nfft_w, p.cg_form, p.use_graph = nfft_n, nfft_w, cg_form, int(use_graph)
     p.tv_full, p.diffusion_lambda = int(tv_full), diffusion_lambda
+    if cu_mask:
+        for i, w in enumerate(cu_mask):
+            p.cu_mask[i] = w
     return p
 
 
@@ -368,8 +373,9 @@ class Server:
     arrival order) and run concurrently, one stream and host slot thread
     each."""
 
-    def __init__(self, max_active=4, hbm_budget=0, device=0, policy=SERVE_SHORTEST, max_wait_s=30.0):
-        o = ServerOpts(hbm_budget, max_active, device, policy, max_wait_s)
+    def __init__(self, max_active=4, hbm_budget=0, device=0, policy=SERVE_SHORTEST, max_wait_s=30.0,
+                 small_cus=0, small_work=0.0):
+        o = ServerOpts(hbm_budget, max_active, device, policy, max_wait_s, small_cus, small_work)
         h = C.c_void_p()
         _check(lib().nudim_server_create(C.byref(o), C.byref(h)))
         self._h = h

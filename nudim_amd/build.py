@@ -19,7 +19,9 @@ HIP_SOURCES = ["ops.hip", "api.hip", "fft2.hip", "serve.hip", "tune.hip"]
 CPP_SOURCES = ["geom.cpp"]
 ARCH = os.environ.get("NUDIM_OFFLOAD_ARCH", "gfx950")
 # Packed FP32 instructions are switched off for all device code.
-HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+# Per-thread default streams: the synchronous copies of one job's set-up do
+# not wait for other jobs' blocking (CU-masked) streams.
+HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-fgpu-default-stream=per-thread",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-Wall", "-Wshadow", "-Wno-unused-function", "-Wno-unused-variable"]
 

@@ -618,7 +618,15 @@ extern "C" int nudim_create(const nudim_params *p, nudim_ctx **out) {
   std::unique_ptr<nudim_ctx> c(new nudim_ctx());
   c->p = *p;
   NUDIM_HIP(hipSetDevice(p->device));
-  NUDIM_HIP(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+  bool masked = false;
+  for (unsigned w : p->cu_mask) masked |= w != 0;
+  // a CU-masked stream cannot be created non-blocking; the library's
+  // synchronous copies use the per-thread default stream, so blocking
+  // streams do not serialise with them
+  if (masked)
+    NUDIM_HIP(hipExtStreamCreateWithCUMask(&c->st, 8, p->cu_mask));
+  else
+    NUDIM_HIP(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
   make_geometry(p, &c->geo);
   if (p->precision == NUDIM_FP64_PLAIN)
     c->job.reset(new Job<double>(c->p, c->geo, c->st));

@@ -53,20 +53,25 @@ struct nudim_server {
   std::vector<std::thread> slots;
   size_t used = 0;
   long long next_id = 1;
-  int active = 0;
+  int active = 0, active_small = 0;
   bool stop = false;
 
   double now() const { return std::chrono::duration<double>(clk::now() - epoch).count(); }
 
-  // the next job to admit among those that fit the free budget (caller
-  // holds mu): FIFO takes the first in arrival order; SHORTEST takes the
-  // least work, except that a job waiting longer than max_wait_s (oldest
-  // first) goes before it
+  bool small(const ServerJob *j) const { return partition && j->work <= o.small_work; }
+
+  // the next job to admit among those that fit the free budget and a free
+  // slot of their kind (caller holds mu): FIFO takes the first in arrival
+  // order; SHORTEST takes the least work, except that a job waiting longer
+  // than max_wait_s (oldest first) goes before it.  With the small-job
+  // partition, small and other jobs have max_active slots each.
   ServerJob *take() {
     auto best = pending.end();
     const double t = now();
+    const bool room_small = active_small < o.max_active, room_large = active - active_small < o.max_active;
     for (auto it = pending.begin(); it != pending.end(); ++it) {
       if (used + (*it)->bytes > (size_t)o.hbm_budget) continue;
+      if (!(small(*it) ? room_small : room_large)) continue;
       if (o.policy != NUDIM_SERVE_SHORTEST || t - (*it)->t_submit > o.max_wait_s) {
         best = it;
         break;
@@ -78,10 +83,20 @@ struct nudim_server {
     pending.erase(best);
     used += j->bytes;
     active++;
+    if (small(j)) active_small++;
     return j;
   }
 
+  // CU masks of the small-job partition: the first small_cus CUs, and the rest
+  unsigned small_mask[8] = {}, large_mask[8] = {};
+  bool partition = false;
+
   void run(ServerJob *j) {
+    if (partition) {
+      bool own = false;
+      for (unsigned w : j->p.cu_mask) own |= w != 0;
+      if (!own) std::memcpy(j->p.cu_mask, j->work <= o.small_work ? small_mask : large_mask, sizeof(small_mask));
+    }
     nudim_ctx *c = nullptr;
     auto check = [&](int rc) {
       if (rc != NUDIM_OK && j->status == NUDIM_OK) {
@@ -117,6 +132,7 @@ struct nudim_server {
         std::lock_guard<std::mutex> lk(mu);
         used -= j->bytes;
         active--;
+        if (small(j)) active_small--;
         done.push_back(j);
       }
       cv_done.notify_all();
@@ -151,8 +167,22 @@ extern "C" int nudim_server_create(const nudim_server_opts *o, nudim_server **ou
       }
       s->o.hbm_budget = (long long)(0.9 * (double)fr);
     }
+    if (s->o.small_cus > 0) {
+      int ncu = 0;
+      if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, o->device) != hipSuccess) {
+        nudim::set_last_error("hipDeviceGetAttribute failed");
+        return NUDIM_EHIP;
+      }
+      if (s->o.small_cus >= ncu || ncu > 256) {
+        nudim::set_last_error("small_cus must be below the device's " + std::to_string(ncu) + " CUs");
+        return NUDIM_EINVAL;
+      }
+      for (int i = 0; i < ncu; i++) (i < s->o.small_cus ? s->small_mask : s->large_mask)[i / 32] |= 1u << (i % 32);
+      s->partition = true;
+    }
     s->epoch = clk::now();
-    for (int i = 0; i < o->max_active; i++) s->slots.emplace_back([p = s.get()] { p->slot_main(); });
+    const int nslots = o->max_active * (s->partition ? 2 : 1);
+    for (int i = 0; i < nslots; i++) s->slots.emplace_back([p = s.get()] { p->slot_main(); });
     *out = s.release();
     return NUDIM_OK;
   } catch (const std::exception &e) {

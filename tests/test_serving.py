@@ -100,3 +100,30 @@ def test_admission_policy_order(hip, policy):
         assert reps[later_small]["t_start"] < reps[later_big]["t_start"]
     else:
         assert reps[later_big]["t_start"] < reps[later_small]["t_start"]
+
+
+def test_cu_masked_jobs_identical(hip):
+    """A job restricted to part of the device (cu_mask), and jobs placed by
+    the server's small-job partition, compute exactly what an unrestricted
+    job computes: every sum has a fixed order whichever CUs run the blocks."""
+    p = job_params(hip, 48, 48, 1, True, 1)
+    proj = job_inputs(hip, 48, 48, 11)
+    v0, r0 = alone(hip, p, proj)
+    pm = job_params(hip, 48, 48, 1, True, 1)
+    for i in range(2):
+        pm.cu_mask[i] = 0xFFFFFFFF  # the first 64 CUs
+    v1, r1 = alone(hip, pm, proj)
+    assert np.array_equal(v0, v1) and np.array_equal(r0, r1)
+    big = job_params(hip, 64, 64, 1, False, 1)
+    pb = job_inputs(hip, 64, 64, 12)
+    vb, rb = alone(hip, big, pb)
+    small_work = 48 * 48 * 48 * (p.nhio + p.ner)
+    with hip.Server(max_active=3, small_cus=32, small_work=small_work) as srv:
+        ids = {srv.submit(p, proj): "small", srv.submit(big, pb): "big", srv.submit(p, proj): "small"}
+        for _ in range(3):
+            rep, vol, rf = srv.next(timeout_ms=120000)
+            assert rep["status"] == 0, rep["error"]
+            ref = (v0, r0) if ids[rep["job_id"]] == "small" else (vb, rb)
+            assert np.array_equal(vol, ref[0]) and np.array_equal(rf, ref[1])
+    with pytest.raises(hip.NudimError):
+        hip.Server(max_active=1, small_cus=100000)
