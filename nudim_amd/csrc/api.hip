@@ -89,16 +89,27 @@ __global__ void k_assemble(const cx<double> *__restrict__ tomo, const int *__res
   meas[i] = v;
 }
 
+// R = sqrt(sum_p d1 / sum_p d2) (reconstruction.c:603-604); one block of 256
+// threads, fixed-order sums
 __global__ void k_rfactor_final(const double *__restrict__ d1, const double *__restrict__ d2, int P,
                                 double *__restrict__ out) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    double a = 0, b = 0;
-    for (int p = 0; p < P; p++) {
-      a += d1[p];
-      b += d2[p];
-    }
-    *out = sqrt(a / b);
+  __shared__ double sa[256], sb[256];
+  double a = 0, b = 0;
+  for (int p = threadIdx.x; p < P; p += 256) {
+    a += d1[p];
+    b += d2[p];
   }
+  sa[threadIdx.x] = a;
+  sb[threadIdx.x] = b;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) {
+      sa[threadIdx.x] += sa[threadIdx.x + h];
+      sb[threadIdx.x] += sb[threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sqrt(sa[0] / sb[0]);
 }
 
 // ---------------------------------------------------------------------------
@@ -264,6 +275,8 @@ struct Job : JobBase {
   DevBuf bk;
   DevBuf dknown;  // with proj_start: the known nodes (R needs A fh only there)
   int nknown = 0;
+  static constexpr int kRBlocks = 1024;  // node strides of the fused R interpolation
+  DevBuf rpart;                          // its block sums [2][kRBlocks][P]
   T *img = nullptr, *img_last = nullptr, *spare = nullptr;
   static constexpr int kTvBlocks = 1024;
 
@@ -295,6 +308,7 @@ struct Job : JobBase {
       for (int m = 0; m < M; m++) wu[m] = g.kind[m] == 1 ? 0.0 : (g.w.empty() ? 1.0 : g.w[m]);
       toep.init_second(g.x.data(), wu.data(), M, st);
       bk.alloc(NN * sizeof(cx<T>));
+      rpart.alloc((size_t)2 * kRBlocks * P * sizeof(double));
       std::vector<int> kn;
       for (int m = 0; m < M; m++)
         if (g.kind[m] == 1) kn.push_back(m);
@@ -352,7 +366,7 @@ struct Job : JobBase {
 
   size_t bytes() const override {
     return op.bytes() + toep.bytes() + cg.bytes() + dw.bytes + dkind.bytes + dorder.bytes + meas.bytes + y.bytes +
-           bk.bytes + dknown.bytes +
+           bk.bytes + dknown.bytes + rpart.bytes +
            Fh.bytes + F0.bytes + img_a.bytes + img_b.bytes + img_c.bytes + grad.bytes + grad2.bytes + d1.bytes +
            d2.bytes + tvpart_g.bytes +
            tvpart_a.bytes + tvsums.bytes + twz.bytes;
@@ -466,23 +480,28 @@ struct Job : JobBase {
     std::swap(img, img_last);  // img_last = img (reconstruction.c:551)
     zfft_forward(img_last, Fh.as<cx<T>>(), F0.as<cx<T>>());
     cx<T> *v = cg.v.template as<cx<T>>();
-    // A fh: everywhere for the projector, or only at the known nodes for R
-    if (proj_start && nknown > 0)
-      op.trafo(Fh.as<cx<T>>(), v, st, dknown.as<int>(), nknown);
-    else
+    if (proj_start) {
+      // R needs A fh only at the known nodes: the interpolation sums the R
+      // terms itself (no A fh stored)
+      double *p1 = rpart.as<double>(), *p2 = p1 + (size_t)kRBlocks * P;
+      op.trafo_rsums(Fh.as<cx<T>>(), meas.as<cx<T>>(), dknown.as<int>(), nknown, p1, p2, kRBlocks, st);
+      hipLaunchKernelGGL(k_sum_partials, dim3(cdiv(P, 8)), dim3(256), 0, st, p1, kRBlocks, P, d1.as<double>());
+      hipLaunchKernelGGL(k_sum_partials, dim3(cdiv(P, 8)), dim3(256), 0, st, p2, kRBlocks, P, d2.as<double>());
+    } else {
       op.trafo(Fh.as<cx<T>>(), v, st);
-    cg.red.template rfactor<T>(v, meas.as<cx<T>>(), dkind.as<int>(), M, d1.as<double>(), d2.as<double>(), st);
-    if (!proj_start)
+      cg.red.template rfactor<T>(v, meas.as<cx<T>>(), dkind.as<int>(), M, d1.as<double>(), d2.as<double>(), st);
       hipLaunchKernelGGL(k_projector<T>, dim3(cdiv(MM, 256)), dim3(256), 0, st, v, meas.as<cx<T>>(),
                          dkind.as<int>(), y.as<cx<T>>(), MM, P);
-    hipLaunchKernelGGL(k_rfactor_final, dim3(1), dim3(1), 0, st, d1.as<double>(), d2.as<double>(), P, rslot);
+    }
+    hipLaunchKernelGGL(k_rfactor_final, dim3(1), dim3(256), 0, st, d1.as<double>(), d2.as<double>(), P, rslot);
     NUDIM_HIP(hipGetLastError());
     if (proj_start) {
       // z = b_K + T_U Fh - T f0, then the CG steps
-      toep.apply_start(Fh.as<cx<T>>(), F0.as<cx<T>>(), bk.as<cx<T>>(), cg.z.template as<cx<T>>(), st);
+      toep.apply_start(Fh.as<cx<T>>(), F0.as<cx<T>>(), bk.as<cx<T>>(), cg.z.template as<cx<T>>(),
+                       cg.rowpart.template as<double>(), st);
       // the last step's f += alpha p happens in the z transform's loads
       // (f itself is not needed afterwards: the next pass rebuilds f0)
-      cg.solve_from_residual(F0.as<cx<T>>(), 10, st, true);
+      cg.solve_from_residual(F0.as<cx<T>>(), 10, st, true, true);
       zfft_inverse(F0.as<cx<T>>(), img, cg.pv.template as<cx<T>>(), cg.alpha.template as<double>());
     } else {
       cg.solve(y.as<cx<T>>(), F0.as<cx<T>>(), 10, false, st);
@@ -802,7 +821,8 @@ extern "C" int nudim_estimate_bytes(const nudim_params *p, size_t *bytes) {
   b += (size_t)N * n * P * cb + (size_t)n * n * P * cb;           // NFFT X, grid
   b += (size_t)M * W * W * (4 + rb) + (size_t)n * n * 4 + M * W * 16;  // gather maps, window tables
   if (form == NUDIM_CG_TOEPLITZ) b += (size_t)4 * N * N * (cb + rb) + (size_t)N * 2 * N * P * cb + NN * cb;
-  if (projected_start(form, (int)N)) b += (size_t)4 * N * N * rb + (size_t)N * 2 * N * P * cb + NN * cb;
+  if (projected_start(form, (int)N))
+    b += (size_t)4 * N * N * rb + (size_t)N * 2 * N * P * cb + NN * cb + (size_t)2 * 1024 * P * 8 + M * 4;
   b += 2 * NN * cb + 3 * MM * cb;                                   // CG z, p; r, v, t
   b += 2 * MM * cb + 2 * NN * cb;                                   // meas, y, Fh, F0
   b += 2 * NN * rb + ((p->denoise && p->method == NUDIM_TV) ? 3 * NN * rb : 0);  // volumes, TV gradients

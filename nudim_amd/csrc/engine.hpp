@@ -71,6 +71,11 @@ struct NfftOp {
   void init(int N, int n, int W, const double *x, int M, int P);
   // f = A fhat; with sel only at the msel nodes sel[] (other rows untouched)
   void trafo(const cx<T> *fhat, cx<T> *f, hipStream_t st, const int *sel = nullptr, int msel = 0);
+  // the R sums of A fhat against meas over the msel measured nodes sel[]:
+  // part1/part2[nblocks][P] block sums of |A fhat - m|^2 and |A fhat + m|^2
+  void trafo_rsums(const cx<T> *fhat, const cx<T> *meas, const int *sel, int msel, double *part1, double *part2,
+                   int nblocks, hipStream_t st);
+  void grid_fft(const cx<T> *fhat, hipStream_t st);  // deconvolve, pad, 2D FFT into grid
   void adjoint(const cx<T> *f, cx<T> *fhat, hipStream_t st);
 };
 
@@ -90,8 +95,9 @@ struct ToeplitzOp {
   void init(int N, const double *x, const double *w, int M, int P, hipStream_t st);
   // the kernel of the weights w2[M] (host) as a second operator T2
   void init_second(const double *x, const double *w2, int M, hipStream_t st);
-  // z = add + T2 a - T b on the fused passes (fused() and init_second done)
-  void apply_start(const cx<T> *a, const cx<T> *b, const cx<T> *add, cx<T> *z, hipStream_t st);
+  // z = add + T2 a - T b on the fused passes (fused() and init_second done);
+  // zrows (may be null): [N][P] row sums of |z|^2
+  void apply_start(const cx<T> *a, const cx<T> *b, const cx<T> *add, cx<T> *z, double *zrows, hipStream_t st);
   void apply(const cx<T> *p, cx<T> *q, hipStream_t st);
   // fused CGNR step on the fixed-length passes (when fused()): p from z and
   // the old p (see ToepCg), q = T p, part[N][P] = row sums of Re <p, q>
@@ -141,7 +147,8 @@ struct Cgnr {
   // the same on the fused Toeplitz passes with the starting residual given
   // in z (z = A^H W (y - A fhat)).  With defer_last the last step's
   // fhat += alpha p is left to the caller (p in pv, alpha per plane in alpha).
-  void solve_from_residual(cx<T> *fhat, int nsteps, hipStream_t st, bool defer_last = false);
+  // With z_rows, rowpart holds the [N][P] row sums of |z|^2 (apply_start).
+  void solve_from_residual(cx<T> *fhat, int nsteps, hipStream_t st, bool defer_last = false, bool z_rows = false);
 };
 
 }  // namespace nudim

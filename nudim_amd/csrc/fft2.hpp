@@ -466,17 +466,22 @@ k_toep(const cx<T> *in, cx<T> *out, long long in_so, long long in_sk, long long 
 #pragma unroll
       for (int k2 = 0; k2 < H2; k2++) b[k2] = av[k2] + b[k2];
     }
+    // row sum: Re <p, q> with p, |q|^2 without (when part is given)
+    double s = 0;
     if (cg.p) {
       const cx<T> *pp = cg.p + ob;
       cx<T> pv[H2];
 #pragma unroll
       for (int k2 = 0; k2 < H2; k2++) pv[k2] = pp[(long long)(r + N1 * k2) * out_sk];
-      double s = 0;
 #pragma unroll
-      for (int k2 = 0; k2 < H2; k2++) {
-        dst[(long long)(r + N1 * k2) * out_sk] = b[k2];
-        s += (double)pv[k2].x * b[k2].x + (double)pv[k2].y * b[k2].y;
-      }
+      for (int k2 = 0; k2 < H2; k2++) s += (double)pv[k2].x * b[k2].x + (double)pv[k2].y * b[k2].y;
+    } else if (cg.part) {
+#pragma unroll
+      for (int k2 = 0; k2 < H2; k2++) s += (double)b[k2].x * b[k2].x + (double)b[k2].y * b[k2].y;
+    }
+#pragma unroll
+    for (int k2 = 0; k2 < H2; k2++) dst[(long long)(r + N1 * k2) * out_sk] = b[k2];
+    if (cg.part) {
       // fixed-order sum over the N1 threads of each plane
       __syncthreads();  // exchange buffer free
       double *sh = reinterpret_cast<double *>(smem_raw);
@@ -487,9 +492,6 @@ k_toep(const cx<T> *in, cx<T> *out, long long in_so, long long in_sk, long long 
         for (int rr = 0; rr < N1; rr++) t += sh[rr * G + g];
         cg.part[(long long)o * pstride + plane] = t;
       }
-    } else {
-#pragma unroll
-      for (int k2 = 0; k2 < H2; k2++) dst[(long long)(r + N1 * k2) * out_sk] = b[k2];
     }
   } else {
     // forward N2-point DFT, real product, inverse N2-point DFT, twiddle, exchange

@@ -57,20 +57,13 @@ k_interp(const cx<T> *__restrict__ g, cx<T> *__restrict__ out, const int *__rest
   out[(size_t)node * P + p] = acc;
 }
 
-// k_interp with the window width W fixed at compile time: the node's 2W
-// weights and W column offsets are kept in registers and the W grid reads of
-// a row are issued together.  Same summation order as k_interp.
+// one node's interpolated value at plane p with the window width W fixed at
+// compile time: the node's 2W weights and W column offsets in registers, the
+// W grid reads of a row issued together; same summation order as k_interp
 template <typename T, int W>
-__global__ void __launch_bounds__(kThreads)
-k_interp_w(const cx<T> *__restrict__ g, cx<T> *__restrict__ out, const int *__restrict__ c0,
-           const int *__restrict__ c1, const double *__restrict__ psi0, const double *__restrict__ psi1, int M, int n,
-           int P, int PCs, const int *__restrict__ sel) {
-  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
-  const int PC = 1 << PCs;
-  const int i = lb * (kThreads >> PCs) + (threadIdx.x >> PCs);
-  const int p = blockIdx.y * PC + (threadIdx.x & (PC - 1));
-  if (i >= M) return;
-  const int node = sel ? sel[i] : i;
+__device__ __forceinline__ cx<T> interp_node(const cx<T> *__restrict__ g, const int *__restrict__ c0,
+                                             const int *__restrict__ c1, const double *__restrict__ psi0,
+                                             const double *__restrict__ psi1, int node, int n, int P, int p) {
   const int b0 = c0[node], b1 = c1[node];
   T a0[W], a1[W];
   long long col[W];
@@ -99,7 +92,61 @@ k_interp_w(const cx<T> *__restrict__ g, cx<T> *__restrict__ out, const int *__re
     for (int t1 = 0; t1 < W; t1++) racc = racc + a1[t1] * v[t1];
     acc = acc + a0[t0] * racc;
   }
-  out[(size_t)node * P + p] = acc;
+  return acc;
+}
+
+// k_interp with W fixed at compile time (interp_node)
+template <typename T, int W>
+__global__ void __launch_bounds__(kThreads)
+k_interp_w(const cx<T> *__restrict__ g, cx<T> *__restrict__ out, const int *__restrict__ c0,
+           const int *__restrict__ c1, const double *__restrict__ psi0, const double *__restrict__ psi1, int M, int n,
+           int P, int PCs, const int *__restrict__ sel) {
+  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  const int PC = 1 << PCs;
+  const int i = lb * (kThreads >> PCs) + (threadIdx.x >> PCs);
+  const int p = blockIdx.y * PC + (threadIdx.x & (PC - 1));
+  if (i >= M) return;
+  const int node = sel ? sel[i] : i;
+  out[(size_t)node * P + p] = interp_node<T, W>(g, c0, c1, psi0, psi1, node, n, P, p);
+}
+
+// The R sums of reconstruction.c:574-582 straight from the interpolation:
+// over the M nodes sel[] (all measured), per plane, sum |v - m|^2 and
+// |v + m|^2 with v the interpolated value and m the measured one; no v is
+// stored.  Blocks stride over the nodes; part1/part2[blockIdx.x][p] get the
+// block's FP64 sums (fixed order: node stride, then the node groups).
+template <typename T, int W>
+__global__ void __launch_bounds__(kThreads)
+k_interp_rsums(const cx<T> *__restrict__ g, const cx<T> *__restrict__ meas, const int *__restrict__ c0,
+               const int *__restrict__ c1, const double *__restrict__ psi0, const double *__restrict__ psi1, int M,
+               int n, int P, int PCs, const int *__restrict__ sel, double *__restrict__ part1,
+               double *__restrict__ part2) {
+  __shared__ double s1[kThreads], s2[kThreads];
+  const int PC = 1 << PCs, NG = kThreads >> PCs;
+  const int grp = threadIdx.x >> PCs, lane = threadIdx.x & (PC - 1);
+  const int p = blockIdx.y * PC + lane;
+  double a1 = 0, a2 = 0;
+  for (int i = blockIdx.x * NG + grp; i < M; i += gridDim.x * NG) {
+    const int node = sel[i];
+    const cx<T> v = interp_node<T, W>(g, c0, c1, psi0, psi1, node, n, P, p);
+    const cx<T> m = meas[(size_t)node * P + p];
+    const double dx = (double)v.x - (double)m.x, dy = (double)v.y - (double)m.y;
+    const double sx = (double)v.x + (double)m.x, sy = (double)v.y + (double)m.y;
+    a1 += dx * dx + dy * dy;
+    a2 += sx * sx + sy * sy;
+  }
+  s1[threadIdx.x] = a1;
+  s2[threadIdx.x] = a2;
+  __syncthreads();
+  if (grp == 0) {
+    double t1 = 0, t2 = 0;
+    for (int k = 0; k < NG; k++) {
+      t1 += s1[k * PC + lane];
+      t2 += s2[k * PC + lane];
+    }
+    part1[(size_t)blockIdx.x * P + p] = t1;
+    part2[(size_t)blockIdx.x * P + p] = t2;
+  }
 }
 
 // k_gather with entries decoded eight at a time so that the weight and

@@ -239,7 +239,7 @@ static int plane_group_log2(int P, int cap = 64) {
 static const int g_gather_lanes = getenv("NUDIM_GATHER_LANES") ? atoi(getenv("NUDIM_GATHER_LANES")) : 64;
 
 template <typename T>
-void NfftOp<T>::trafo(const cx<T> *fhat, cx<T> *f, hipStream_t st, const int *sel, int msel) {
+void NfftOp<T>::grid_fft(const cx<T> *fhat, hipStream_t st) {
   const long long NP = (long long)N * P, nP = (long long)n * P;
   const double *d = dec.as<double>();
   // rows: deconvolve, place at (k - N/2) mod n, FFT along x
@@ -250,6 +250,42 @@ void NfftOp<T>::trafo(const cx<T> *fhat, cx<T> *f, hipStream_t st, const int *se
   LineIO ci{P, nP, N, -N / 2, nullptr, nullptr};
   LineIO co{P, nP, n, 0, nullptr, nullptr};
   fft.run(X.as<cx<T>>(), grid.as<cx<T>>(), n, ci, co, -1, P, st);
+}
+
+template <typename T>
+void NfftOp<T>::trafo_rsums(const cx<T> *fhat, const cx<T> *meas, const int *sel, int msel, double *part1,
+                            double *part2, int nblocks, hipStream_t st) {
+  grid_fft(fhat, st);
+  const int PCs = plane_group_log2(P);
+  const dim3 gr(nblocks, P >> PCs);
+  // algorithmic bytes: grid read once, measured values and node coordinates
+  ProfScope ps(KC_INTERP, st, ((double)n * n * sizeof(cx<T>) + (double)msel * sizeof(cx<T>) + 2.0 * msel * sizeof(T)) * P);
+  const cx<T> *gp = grid.as<cx<T>>();
+  const int *q0 = c0.as<int>(), *q1 = c1.as<int>();
+  const double *p0 = psi0.as<double>(), *p1 = psi1.as<double>();
+#define NUDIM_INTERP_R(WW)                                                                                        \
+  case WW:                                                                                                        \
+    hipLaunchKernelGGL((k_interp_rsums<T, WW>), gr, dim3(kThreads), 0, st, gp, meas, q0, q1, p0, p1, msel, n, P,  \
+                       PCs, sel, part1, part2);                                                                   \
+    break;
+  switch (W) {
+    NUDIM_INTERP_R(4)
+    NUDIM_INTERP_R(6)
+    NUDIM_INTERP_R(8)
+    NUDIM_INTERP_R(10)
+    NUDIM_INTERP_R(12)
+    NUDIM_INTERP_R(14)
+    NUDIM_INTERP_R(16)
+    default:
+      throw Error(NUDIM_EINVAL, "no fused R interpolation for window width " + std::to_string(W));
+  }
+#undef NUDIM_INTERP_R
+  NUDIM_HIP(hipGetLastError());
+}
+
+template <typename T>
+void NfftOp<T>::trafo(const cx<T> *fhat, cx<T> *f, hipStream_t st, const int *sel, int msel) {
+  grid_fft(fhat, st);
   int PCs = plane_group_log2(P);
   const int Mc = sel ? msel : M;
   dim3 gr(cdiv(Mc, kThreads >> PCs), P >> PCs);
@@ -405,7 +441,8 @@ void ToeplitzOp<T>::init_second(const double *x, const double *w2, int M, hipStr
 }
 
 template <typename T>
-void ToeplitzOp<T>::apply_start(const cx<T> *a, const cx<T> *b, const cx<T> *add, cx<T> *z, hipStream_t st) {
+void ToeplitzOp<T>::apply_start(const cx<T> *a, const cx<T> *b, const cx<T> *add, cx<T> *z, double *zrows,
+                                 hipStream_t st) {
   NUDIM_CHECK(fused() && khat2_re.p, "apply_start needs the fused passes and a second kernel");
   const long long NP = (long long)N * P, LC = (long long)L * pc;
   const cx<T> *tw = fft.tw.template as<cx<T>>();
@@ -434,6 +471,7 @@ void ToeplitzOp<T>::apply_start(const cx<T> *a, const cx<T> *b, const cx<T> *add
       ProfScope ps(KC_TOEPLITZ_ROWI, st, (2.0 * N + L) * N * pc * sizeof(cx<T>));
       ToepCg<T> ci{};
       ci.add = add + p0;
+      if (zrows) ci.part = zrows + p0;
       toep_launch<T>(TK_ROWI, L, N, xb, z + p0, LC, pc, NP, P, pc, st, tw, nullptr, ci, P);
     }
   }
@@ -583,13 +621,17 @@ void Cgnr<T>::init(int N_, int M_, int P_, int form_, NfftOp<T> *op_, ToeplitzOp
 }
 
 template <typename T>
-void Cgnr<T>::solve_from_residual(cx<T> *fhat, int nsteps, hipStream_t st, bool defer_last) {
+void Cgnr<T>::solve_from_residual(cx<T> *fhat, int nsteps, hipStream_t st, bool defer_last, bool z_rows) {
   NUDIM_CHECK(form == NUDIM_CG_TOEPLITZ && toep->fused(), "the fused Toeplitz passes are not available");
   const long long NN = (long long)N * N * P;
   const unsigned bN = cdiv(NN, 256);
   cx<T> *zp = z.as<cx<T>>(), *pp = pv.as<cx<T>>();
   double *dz = dot_z.as<double>(), *cf = coef.as<double>(), *s = sc.as<double>();
-  red.dot<T>(zp, nullptr, (long long)N * N, s, st);
+  // |z|^2 per plane: from the row sums the start's row pass left, or a read of z
+  if (z_rows)
+    hipLaunchKernelGGL(k_sum_partials, dim3(cdiv(P, 8)), dim3(256), 0, st, rowpart.as<double>(), N, P, s);
+  else
+    red.dot<T>(zp, nullptr, (long long)N * N, s, st);
   hipLaunchKernelGGL(k_cg_scalars, dim3(cdiv(P, 64)), dim3(64), 0, st, dz, cf, s, P, 0);
   NUDIM_HIP(hipGetLastError());
   // step k: row pass forms p_k = z_k + beta_k p_{k-1} (p_0 = z_0) and
