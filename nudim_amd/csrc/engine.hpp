@@ -1,6 +1,7 @@
 // engine.hpp -- device-side operators of one reconstruction job.
 #pragma once
 
+#include <memory>
 #include <vector>
 
 #include "common.hpp"
@@ -56,17 +57,33 @@ struct LineFft {
            hipStream_t st, const cx<T> *mid = nullptr, long long mid_so = 0) const;
 };
 
+// The node tables of one NFFT geometry (N, n, W and the nodes), which every
+// operator and job with that geometry shares: window and deconvolution
+// tables, line-FFT twiddles and the gather maps.  nfft_geometry() builds them
+// once per device and geometry and keeps the most recent ones.
+template <typename T>
+struct NfftGeom {
+  int N = 0, n = 0, W = 0, M = 0;
+  LineFft<T> fft;
+  DevBuf c0, c1, psi0, psi1, dec;
+  DevBuf cell_ptr, ent;  // per-cell gather map (any plane count)
+  // tiled gather map (built when n % kGatherTile == 0; used when P % 64 == 0)
+  bool has_tiles = false;
+  int nchunk = 0, ncomb = 0, npart = 0;
+  DevBuf chunk_tile, chunk_part, chunk_node, tnode, chunk_cell, tent, comb;
+  size_t bytes() const;
+};
+template <typename T>
+std::shared_ptr<const NfftGeom<T>> nfft_geometry(int N, int n, int W, const double *x, int M);
+
 // Square N x N NFFT with plane batch P (all planes share the nodes).
 template <typename T>
 struct NfftOp {
   int N = 0, n = 0, W = 0, M = 0, P = 0;
-  LineFft<T> fft;
-  DevBuf c0, c1, psi0, psi1, dec, cell_ptr, ent;
-  // tiled gather map (used when n % kGatherTile == 0 and P % 64 == 0)
-  bool tiled = false;
-  int nchunk = 0, ncomb = 0;
-  DevBuf chunk_tile, chunk_part, chunk_node, tnode, chunk_cell, tent, comb, part;
-  DevBuf X, grid;  // [N][n][P], [n][n][P]
+  std::shared_ptr<const NfftGeom<T>> g;  // shared node tables
+  bool tiled = false;                    // tiled gather map in use (P % 64 == 0)
+  DevBuf part;                           // its partial sums [npart][tile cells][P]
+  DevBuf X, grid;                        // [N][n][P], [n][n][P]
   size_t bytes() const;
   void init(int N, int n, int W, const double *x, int M, int P);
   // f = A fhat; with sel only at the msel nodes sel[] (other rows untouched)
@@ -79,18 +96,31 @@ struct NfftOp {
   void adjoint(const cx<T> *f, cx<T> *fhat, hipStream_t st);
 };
 
+// Spectrum of a Toeplitz kernel embedded in the 2N x 2N circulant: khat
+// cx<T>[L][L] stored [column][row], scaled by 1/L^2, and its real part khat_re
+// (the kernel is Hermitian, so the spectrum is real).  toeplitz_spectrum()
+// builds it once per device, N, nodes and weights and keeps the most recent.
+template <typename T>
+struct ToepSpectrum {
+  DevBuf khat, khat_re;
+};
+template <typename T>
+std::shared_ptr<const ToepSpectrum<T>> toeplitz_spectrum(int N, const double *x, const double *w, int M,
+                                                         hipStream_t st);
+
 // Normal operator A^H W A as a 2N x 2N circulant embedding of its Toeplitz
 // kernel T(d) = sum_m w_m exp(2 pi i d.x_m).
 template <typename T>
 struct ToeplitzOp {
   int N = 0, L = 0, P = 0;
   LineFft<T> fft;
-  DevBuf khat;  // cx<T>[L][L], stored [column][row], scaled by 1/L^2
-  DevBuf khat_re;  // T[L][L]: its real part (the kernel is Hermitian, so khat is real)
+  // the kernel spectrum (shared by every job with the same nodes and weights)
+  std::shared_ptr<const ToepSpectrum<T>> spec;
   DevBuf X;     // [N][L][P]; for the fused passes [P/pc][N][L][pc]
   int pc = 0;   // planes per chunk of X in the fused passes
   // second kernel (init_second) and intermediate for apply_start
-  DevBuf khat2_re, X2;
+  std::shared_ptr<const ToepSpectrum<T>> spec2;
+  DevBuf X2;
   size_t bytes() const;
   void init(int N, const double *x, const double *w, int M, int P, hipStream_t st);
   // the kernel of the weights w2[M] (host) as a second operator T2

@@ -3,6 +3,9 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <cstring>
+#include <list>
+#include <mutex>
 #include <cmath>
 
 namespace nudim {
@@ -180,48 +183,113 @@ void LineFft<T>::run(const cx<T> *in, cx<T> *out, int nlines, const LineIO &li, 
 // ---------------------------------------------------------------------------
 
 template <typename T>
-size_t NfftOp<T>::bytes() const {
+size_t NfftGeom<T>::bytes() const {
   return fft.tw.bytes + c0.bytes + c1.bytes + psi0.bytes + psi1.bytes + dec.bytes + cell_ptr.bytes + ent.bytes +
          chunk_tile.bytes + chunk_part.bytes + chunk_node.bytes + tnode.bytes + chunk_cell.bytes + tent.bytes +
-         comb.bytes + part.bytes +
-         X.bytes + grid.bytes;
+         comb.bytes;
+}
+
+template <typename T>
+static std::shared_ptr<NfftGeom<T>> build_nfft_geometry(int N, int n, int W, const double *x, int M) {
+  NUDIM_CHECK(N % 2 == 0 && n >= N && n % 2 == 0, "NFFT sizes must be even with n >= N");
+  NUDIM_CHECK(W >= 2 && W <= 63 && W <= n, "window width out of range");
+  NUDIM_CHECK(M < (1 << 20), "too many nodes for the gather map");
+  auto g = std::make_shared<NfftGeom<T>>();
+  g->N = N;
+  g->n = n;
+  g->W = W;
+  g->M = M;
+  NfftTables t;
+  build_nfft_tables(N, n, W, x, M, true, &t);
+  g->fft.init(n);
+  upload(g->c0, t.c0);
+  upload(g->c1, t.c1);
+  upload(g->psi0, t.psi0);
+  upload(g->psi1, t.psi1);
+  upload(g->dec, t.dec);
+  upload(g->cell_ptr, t.cell_ptr);
+  upload(g->ent, t.ent);
+  g->has_tiles = n % kGatherTile == 0;
+  if (g->has_tiles) {
+    // node values of a chunk staged as [slot][64 planes]: 64 nodes per chunk
+    GatherTiles gt;
+    build_gather_tiles(t, kGatherTile, 64, &gt);
+    g->nchunk = (int)gt.chunk_tile.size();
+    g->ncomb = (int)gt.comb.size() / 3;
+    g->npart = gt.npart;
+    upload(g->chunk_tile, gt.chunk_tile);
+    upload(g->chunk_part, gt.chunk_part);
+    upload(g->chunk_node, gt.chunk_node);
+    upload(g->tnode, gt.node);
+    upload(g->chunk_cell, gt.chunk_cell);
+    upload(g->comb, gt.comb);
+    std::vector<GEnt<T>> ev(gt.slot.size());
+    for (size_t e = 0; e < ev.size(); e++) ev[e] = GEnt<T>{(int)gt.slot[e], (T)gt.weight[e]};
+    upload(g->tent, ev);
+  }
+  return g;
+}
+
+namespace {
+// most recently used geometries per precision (device ordinal, sizes, nodes)
+template <typename T>
+struct GeomCache {
+  struct Entry {
+    int dev, N, n, W, M;
+    std::vector<double> x;
+    std::shared_ptr<NfftGeom<T>> g;
+  };
+  std::mutex mu;
+  std::list<Entry> lru;
+};
+// the caches keep up to kCacheEntries entries and kCacheBytes of device memory
+constexpr size_t kCacheEntries = 64;
+constexpr size_t kCacheBytes = (size_t)16 << 30;
+}  // namespace
+
+template <typename T>
+std::shared_ptr<const NfftGeom<T>> nfft_geometry(int N, int n, int W, const double *x, int M) {
+  // never destroyed: its device buffers must not be freed after the runtime
+  // is torn down at exit
+  static GeomCache<T> *cache = new GeomCache<T>();
+  int dev = 0;
+  NUDIM_HIP(hipGetDevice(&dev));
+  auto same = [&](const typename GeomCache<T>::Entry &e) {
+    return e.dev == dev && e.N == N && e.n == n && e.W == W && e.M == M &&
+           std::memcmp(e.x.data(), x, sizeof(double) * 2 * (size_t)M) == 0;
+  };
+  {
+    std::lock_guard<std::mutex> lk(cache->mu);
+    for (auto it = cache->lru.begin(); it != cache->lru.end(); ++it)
+      if (same(*it)) {
+        cache->lru.splice(cache->lru.begin(), cache->lru, it);
+        return cache->lru.front().g;
+      }
+  }
+  // built outside the lock: two jobs of a new geometry may build it twice
+  auto g = build_nfft_geometry<T>(N, n, W, x, M);
+  std::lock_guard<std::mutex> lk(cache->mu);
+  cache->lru.push_front({dev, N, n, W, M, std::vector<double>(x, x + 2 * (size_t)M), g});
+  size_t total = 0;
+  for (const auto &e : cache->lru) total += e.g->bytes();
+  while (cache->lru.size() > 1 && (cache->lru.size() > kCacheEntries || total > kCacheBytes)) {
+    total -= cache->lru.back().g->bytes();
+    cache->lru.pop_back();
+  }
+  return g;
+}
+
+template <typename T>
+size_t NfftOp<T>::bytes() const {
+  return (g ? g->bytes() : 0) + part.bytes + X.bytes + grid.bytes;
 }
 
 template <typename T>
 void NfftOp<T>::init(int N_, int n_, int W_, const double *x, int M_, int P_) {
   N = N_; n = n_; W = W_; M = M_; P = P_;
-  NUDIM_CHECK(N % 2 == 0 && n >= N && n % 2 == 0, "NFFT sizes must be even with n >= N");
-  NUDIM_CHECK(W >= 2 && W <= 63 && W <= n, "window width out of range");
-  NUDIM_CHECK(M < (1 << 20), "too many nodes for the gather map");
-  NfftTables t;
-  build_nfft_tables(N, n, W, x, M, true, &t);
-  fft.init(n);
-  upload(c0, t.c0);
-  upload(c1, t.c1);
-  upload(psi0, t.psi0);
-  upload(psi1, t.psi1);
-  upload(dec, t.dec);
-  tiled = (n % kGatherTile == 0) && (P % 64 == 0);
-  if (tiled) {
-    // node values of a chunk staged as [slot][64 planes]: 64 nodes per chunk
-    GatherTiles gt;
-    build_gather_tiles(t, kGatherTile, 64, &gt);
-    nchunk = (int)gt.chunk_tile.size();
-    ncomb = (int)gt.comb.size() / 3;
-    upload(chunk_tile, gt.chunk_tile);
-    upload(chunk_part, gt.chunk_part);
-    upload(chunk_node, gt.chunk_node);
-    upload(tnode, gt.node);
-    upload(chunk_cell, gt.chunk_cell);
-    upload(comb, gt.comb);
-    std::vector<GEnt<T>> ev(gt.slot.size());
-    for (size_t e = 0; e < ev.size(); e++) ev[e] = GEnt<T>{(int)gt.slot[e], (T)gt.weight[e]};
-    upload(tent, ev);
-    part.alloc((size_t)std::max(gt.npart, 1) * kGatherTile * kGatherTile * P * sizeof(cx<T>));
-  } else {
-    upload(cell_ptr, t.cell_ptr);
-    upload(ent, t.ent);
-  }
+  g = nfft_geometry<T>(N, n, W, x, M);
+  tiled = g->has_tiles && (P % 64 == 0);
+  if (tiled) part.alloc((size_t)std::max(g->npart, 1) * kGatherTile * kGatherTile * P * sizeof(cx<T>));
   X.alloc((size_t)N * n * P * sizeof(cx<T>));
   grid.alloc((size_t)n * n * P * sizeof(cx<T>));
 }
@@ -241,15 +309,15 @@ static const int g_gather_lanes = getenv("NUDIM_GATHER_LANES") ? atoi(getenv("NU
 template <typename T>
 void NfftOp<T>::grid_fft(const cx<T> *fhat, hipStream_t st) {
   const long long NP = (long long)N * P, nP = (long long)n * P;
-  const double *d = dec.as<double>();
+  const double *d = g->dec.template as<double>();
   // rows: deconvolve, place at (k - N/2) mod n, FFT along x
   LineIO li{NP, P, N, -N / 2, d, d};
   LineIO lo{nP, P, n, 0, nullptr, nullptr};
-  fft.run(fhat, X.as<cx<T>>(), N, li, lo, -1, P, st);
+  g->fft.run(fhat, X.as<cx<T>>(), N, li, lo, -1, P, st);
   // columns: N non-zero rows at (k - N/2) mod n, FFT along y
   LineIO ci{P, nP, N, -N / 2, nullptr, nullptr};
   LineIO co{P, nP, n, 0, nullptr, nullptr};
-  fft.run(X.as<cx<T>>(), grid.as<cx<T>>(), n, ci, co, -1, P, st);
+  g->fft.run(X.as<cx<T>>(), grid.as<cx<T>>(), n, ci, co, -1, P, st);
 }
 
 template <typename T>
@@ -261,8 +329,8 @@ void NfftOp<T>::trafo_rsums(const cx<T> *fhat, const cx<T> *meas, const int *sel
   // algorithmic bytes: grid read once, measured values and node coordinates
   ProfScope ps(KC_INTERP, st, ((double)n * n * sizeof(cx<T>) + (double)msel * sizeof(cx<T>) + 2.0 * msel * sizeof(T)) * P);
   const cx<T> *gp = grid.as<cx<T>>();
-  const int *q0 = c0.as<int>(), *q1 = c1.as<int>();
-  const double *p0 = psi0.as<double>(), *p1 = psi1.as<double>();
+  const int *q0 = g->c0.template as<int>(), *q1 = g->c1.template as<int>();
+  const double *p0 = g->psi0.template as<double>(), *p1 = g->psi1.template as<double>();
 #define NUDIM_INTERP_R(WW)                                                                                        \
   case WW:                                                                                                        \
     hipLaunchKernelGGL((k_interp_rsums<T, WW>), gr, dim3(kThreads), 0, st, gp, meas, q0, q1, p0, p1, msel, n, P,  \
@@ -292,8 +360,8 @@ void NfftOp<T>::trafo(const cx<T> *fhat, cx<T> *f, hipStream_t st, const int *se
   // algorithmic bytes: grid read once, node values written, node coordinates
   ProfScope ps(KC_INTERP, st, ((double)n * n * sizeof(cx<T>) + (double)Mc * sizeof(cx<T>) + 2.0 * Mc * sizeof(T)) * P);
   const cx<T> *gp = grid.as<cx<T>>();
-  const int *q0 = c0.as<int>(), *q1 = c1.as<int>();
-  const double *p0 = psi0.as<double>(), *p1 = psi1.as<double>();
+  const int *q0 = g->c0.template as<int>(), *q1 = g->c1.template as<int>();
+  const double *p0 = g->psi0.template as<double>(), *p1 = g->psi1.template as<double>();
 #define NUDIM_INTERP_W(WW)                                                                                      \
   case WW:                                                                                                      \
     hipLaunchKernelGGL((k_interp_w<T, WW>), gr, dim3(kThreads), 0, st, gp, f, q0, q1, p0, p1, Mc, n, P, PCs, sel); \
@@ -322,20 +390,21 @@ void NfftOp<T>::adjoint(const cx<T> *f, cx<T> *fhat, hipStream_t st) {
   if (tiled) {
     ProfScope ps(KC_GATHER, st, ((double)M * sizeof(cx<T>) + 4.0 * M + (double)ncell * sizeof(cx<T>)) * P);
     cx<T> *gp = grid.as<cx<T>>(), *pp = part.as<cx<T>>();
-    if (nchunk > 0)
-      hipLaunchKernelGGL((k_gather_chunk<T, kGatherTile>), dim3(nchunk, P / 64), dim3(kThreads),
-                         65 * 64 * sizeof(cx<T>), st, f, gp, pp, chunk_tile.as<int>(), chunk_part.as<int>(),
-                         chunk_node.as<int>(), tnode.as<int>(), chunk_cell.as<int>(), tent.as<GEnt<T>>(), n, P);
-    if (ncomb > 0)
-      hipLaunchKernelGGL((k_gather_combine<T, kGatherTile>), dim3(ncomb, P / 64), dim3(kThreads), 0, st, pp, gp,
-                         comb.as<int>(), n, P);
+    if (g->nchunk > 0)
+      hipLaunchKernelGGL((k_gather_chunk<T, kGatherTile>), dim3(g->nchunk, P / 64), dim3(kThreads),
+                         65 * 64 * sizeof(cx<T>), st, f, gp, pp, g->chunk_tile.template as<int>(), g->chunk_part.template as<int>(),
+                         g->chunk_node.template as<int>(), g->tnode.template as<int>(), g->chunk_cell.template as<int>(),
+                         g->tent.template as<GEnt<T>>(), n, P);
+    if (g->ncomb > 0)
+      hipLaunchKernelGGL((k_gather_combine<T, kGatherTile>), dim3(g->ncomb, P / 64), dim3(kThreads), 0, st, pp, gp,
+                         g->comb.template as<int>(), n, P);
   } else {
   // algorithmic bytes: node values read, node ids of the inverse map, grid written
   ProfScope ps(KC_GATHER, st, ((double)M * sizeof(cx<T>) + 4.0 * M + (double)ncell * sizeof(cx<T>)) * P);
   cx<T> *gp = grid.as<cx<T>>();
-  const int *cp = cell_ptr.as<int>();
-  const uint32_t *ep = ent.as<uint32_t>();
-  const double *p0 = psi0.as<double>(), *p1 = psi1.as<double>();
+  const int *cp = g->cell_ptr.template as<int>();
+  const uint32_t *ep = g->ent.template as<uint32_t>();
+  const double *p0 = g->psi0.template as<double>(), *p1 = g->psi1.template as<double>();
 #define NUDIM_GATHER_W(WW)                                                                                 \
   case WW:                                                                                                 \
     hipLaunchKernelGGL((k_gather_w<T, WW>), gr, dim3(kThreads), 0, st, f, gp, cp, ep, p0, p1, ncell, P, PCs); \
@@ -354,13 +423,13 @@ void NfftOp<T>::adjoint(const cx<T> *f, cx<T> *fhat, hipStream_t st) {
 #undef NUDIM_GATHER_W
   }
   NUDIM_HIP(hipGetLastError());
-  const double *d = dec.as<double>();
+  const double *d = g->dec.template as<double>();
   LineIO ci{P, nP, n, 0, nullptr, nullptr};
   LineIO co{P, nP, N, -N / 2, nullptr, nullptr};
-  fft.run(grid.as<cx<T>>(), X.as<cx<T>>(), n, ci, co, +1, P, st);
+  g->fft.run(grid.as<cx<T>>(), X.as<cx<T>>(), n, ci, co, +1, P, st);
   LineIO ri{nP, P, n, 0, nullptr, nullptr};
   LineIO ro{NP, P, N, -N / 2, d, d};
-  fft.run(X.as<cx<T>>(), fhat, N, ri, ro, +1, P, st);
+  g->fft.run(X.as<cx<T>>(), fhat, N, ri, ro, +1, P, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -429,21 +498,70 @@ static void toeplitz_kernel(int N, const double *x, const double *w, int M, DevB
   NUDIM_HIP(hipStreamSynchronize(st));
 }
 
+namespace {
+template <typename T>
+struct SpecCache {
+  struct Entry {
+    int dev, N, M;
+    std::vector<double> x, w;  // w empty: unit weights
+    std::shared_ptr<ToepSpectrum<T>> s;
+  };
+  std::mutex mu;
+  std::list<Entry> lru;
+};
+}  // namespace
+
+template <typename T>
+std::shared_ptr<const ToepSpectrum<T>> toeplitz_spectrum(int N, const double *x, const double *w, int M,
+                                                         hipStream_t st) {
+  // never destroyed (device buffers must outlive any use at exit)
+  static SpecCache<T> *cache = new SpecCache<T>();
+  int dev = 0;
+  NUDIM_HIP(hipGetDevice(&dev));
+  auto same = [&](const typename SpecCache<T>::Entry &e) {
+    return e.dev == dev && e.N == N && e.M == M && e.w.empty() == (w == nullptr) &&
+           std::memcmp(e.x.data(), x, sizeof(double) * 2 * (size_t)M) == 0 &&
+           (!w || std::memcmp(e.w.data(), w, sizeof(double) * (size_t)M) == 0);
+  };
+  {
+    std::lock_guard<std::mutex> lk(cache->mu);
+    for (auto it = cache->lru.begin(); it != cache->lru.end(); ++it)
+      if (same(*it)) {
+        cache->lru.splice(cache->lru.begin(), cache->lru, it);
+        return cache->lru.front().s;
+      }
+  }
+  auto sp = std::make_shared<ToepSpectrum<T>>();
+  toeplitz_kernel<T>(N, x, w, M, &sp->khat, sp->khat_re, st);
+  std::lock_guard<std::mutex> lk(cache->mu);
+  cache->lru.push_front({dev, N, M, std::vector<double>(x, x + 2 * (size_t)M),
+                         w ? std::vector<double>(w, w + M) : std::vector<double>(), sp});
+  auto bytes = [](const ToepSpectrum<T> &t) { return t.khat.bytes + t.khat_re.bytes; };
+  size_t total = 0;
+  for (const auto &e : cache->lru) total += bytes(*e.s);
+  while (cache->lru.size() > 1 && (cache->lru.size() > kCacheEntries || total > kCacheBytes)) {
+    total -= bytes(*cache->lru.back().s);
+    cache->lru.pop_back();
+  }
+  return sp;
+}
+
 template <typename T>
 size_t ToeplitzOp<T>::bytes() const {
-  return fft.tw.bytes + khat.bytes + khat_re.bytes + X.bytes + khat2_re.bytes + X2.bytes;
+  return fft.tw.bytes + (spec ? spec->khat.bytes + spec->khat_re.bytes : 0) + X.bytes +
+         (spec2 ? spec2->khat.bytes + spec2->khat_re.bytes : 0) + X2.bytes;
 }
 
 template <typename T>
 void ToeplitzOp<T>::init_second(const double *x, const double *w2, int M, hipStream_t st) {
-  toeplitz_kernel<T>(N, x, w2, M, nullptr, khat2_re, st);
+  spec2 = toeplitz_spectrum<T>(N, x, w2, M, st);
   X2.alloc((size_t)N * L * P * sizeof(cx<T>));
 }
 
 template <typename T>
 void ToeplitzOp<T>::apply_start(const cx<T> *a, const cx<T> *b, const cx<T> *add, cx<T> *z, double *zrows,
                                  hipStream_t st) {
-  NUDIM_CHECK(fused() && khat2_re.p, "apply_start needs the fused passes and a second kernel");
+  NUDIM_CHECK(fused() && spec2, "apply_start needs the fused passes and a second kernel");
   const long long NP = (long long)N * P, LC = (long long)L * pc;
   const cx<T> *tw = fft.tw.template as<cx<T>>();
   for (int p0 = 0; p0 < P; p0 += pc) {
@@ -463,8 +581,8 @@ void ToeplitzOp<T>::apply_start(const cx<T> *a, const cx<T> *b, const cx<T> *add
       // X2 and X read, X written (N x L each), both kernels read
       ProfScope ps(KC_TOEPLITZ_COL2, st, 3.0 * N * L * pc * sizeof(cx<T>) + 2.0 * L * L * sizeof(T));
       c.in2 = xb;
-      c.mid2 = khat_re.template as<T>();
-      toep_launch<T>(TK_COL2, L, L, xa, xb, pc, LC, pc, LC, pc, st, tw, khat2_re.template as<T>(), c, P);
+      c.mid2 = spec->khat_re.template as<T>();
+      toep_launch<T>(TK_COL2, L, L, xa, xb, pc, LC, pc, LC, pc, st, tw, spec2->khat_re.template as<T>(), c, P);
     }
     {
       // X read (N x L), add read and z written (N x N each)
@@ -484,7 +602,7 @@ void ToeplitzOp<T>::init(int N_, const double *x, const double *w, int M, int P_
   L = 2 * N;
   P = P_;
   fft.init(L);
-  toeplitz_kernel<T>(N, x, w, M, &khat, khat_re, st);
+  spec = toeplitz_spectrum<T>(N, x, w, M, st);
   X.alloc((size_t)N * L * P * sizeof(cx<T>));
   // The column pass walks X's rows with stride L * pc: chunks of pc planes
   // (P halved while even and at least 64) keep a chunk's row within
@@ -503,7 +621,7 @@ void ToeplitzOp<T>::apply(const cx<T> *p, cx<T> *q, hipStream_t st) {
   LineIO ro{LP, P, L, 0, nullptr, nullptr};
   fft.run(p, X.as<cx<T>>(), N, ri, ro, -1, P, st);
   LineIO ci{P, LP, N, 0, nullptr, nullptr};
-  fft.run(X.as<cx<T>>(), X.as<cx<T>>(), L, ci, ci, -1, P, st, khat.as<cx<T>>(), L);
+  fft.run(X.as<cx<T>>(), X.as<cx<T>>(), L, ci, ci, -1, P, st, spec->khat.template as<cx<T>>(), L);
   LineIO bi{LP, P, L, 0, nullptr, nullptr};
   LineIO bo{NP, P, N, 0, nullptr, nullptr};
   fft.run(X.as<cx<T>>(), q, N, bi, bo, +1, P, st);
@@ -513,7 +631,7 @@ template <typename T>
 void ToeplitzOp<T>::apply_cg(const ToepCg<T> &cg, cx<T> *q, hipStream_t st) {
   const long long NP = (long long)N * P, LC = (long long)L * pc;
   const cx<T> *tw = fft.tw.template as<cx<T>>();
-  const T *kr = khat_re.template as<T>();
+  const T *kr = spec->khat_re.template as<T>();
   // one chunk of pc planes at a time; X of the chunk is [N][L][pc]
   for (int p0 = 0; p0 < P; p0 += pc) {
     cx<T> *x = X.as<cx<T>>() + (long long)p0 * N * L;
@@ -803,6 +921,8 @@ template double *tv_sweeps<double>(double *, double *, double *, double *, int, 
 
 template struct LineFft<float>;
 template struct LineFft<double>;
+template struct NfftGeom<float>;
+template struct NfftGeom<double>;
 template struct NfftOp<float>;
 template struct NfftOp<double>;
 template struct ToeplitzOp<float>;
