@@ -31,13 +31,31 @@ import nudim_amd as nd  # noqa: E402
 from nudim_amd import phantom  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# profiler classes of the library (nudim_profile) and the kernels they time
-KC_NAMES = {1: "k_toep<float,32,24,1> (Toeplitz column pass)", 2: "k_interp_w<float,8>",
-            3: "k_gather_chunk<float,8> (NFFT adjoint)", 4: "k_fft2<float> (NFFT grid passes)",
-            5: "k_tv_fused4<float>", 6: "k_zfft<float>",
-            7: "k_toep<float,32,24,0> (Toeplitz row forward + CG update)",
-            8: "k_toep<float,32,24,2> (Toeplitz row inverse + <p,q>)",
-            9: "k_toep<float,32,24,3> (two-input column pass, CG start)", 10: "k_cg_zupdate<float>"}
+# profiler classes of the library (nudim_profile) and the kernels they time;
+# the template arguments follow the workload: the register FFT split of each
+# length (fft2.hip), the NFFT window W, the z length
+FFT2_SPLIT = {128: (16, 8), 256: (16, 16), 384: (24, 16), 512: (32, 16), 768: (32, 24), 1024: (32, 32),
+              1536: (48, 32)}
+ZFFT2_SPLIT = {64: (8, 8), 128: (16, 8), 256: (16, 16), 512: (32, 16)}
+
+
+def kc_names(np_, nz, n, W):
+    a, b = FFT2_SPLIT.get(2 * np_, ("N1", "N2"))
+
+    def toep(k, what):
+        return "k_toep<float,%s,%s,%d> (%s)" % (a, b, k, what)
+
+    grid = ("k_fft2<float,%d,%d> (NFFT grid passes, n=%d)" % (FFT2_SPLIT[n] + (n,)) if n in FFT2_SPLIT
+            else "k_fft_lines<float> (NFFT grid passes, n=%d)" % n)
+    zf = "k_zfft2<float,%d,%d> (z transforms)" % ZFFT2_SPLIT[nz] if nz in ZFFT2_SPLIT else "k_zfft_fwd/inv<float>"
+    gather = ("k_gather_chunk<float,%d> (NFFT adjoint)" % W if nz % 64 == 0
+              else "k_gather_w<float,%d> (NFFT adjoint)" % W)
+    return {1: toep(1, "Toeplitz column pass"), 2: "k_interp_w<float,%d>" % W, 3: gather, 4: grid,
+            5: "k_tv_fused_q<float,4>" if nz % 4 == 0 else "k_tv_fused<float>", 6: zf,
+            7: toep(0, "Toeplitz row forward + CG update"), 8: toep(2, "Toeplitz row inverse + <p,q>"),
+            9: toep(3, "two-input column pass, CG start"), 10: "k_cg_zupdate<float>"}
+
+
 KC_CANDIDATES = (7, 1, 8, 5, 10, 9, 4, 2, 6, 3)
 PROFILES = os.path.join(ROOT, "profiles")
 
@@ -137,6 +155,15 @@ def cpu_baseline(prm, proj):
                       "projector + 10-step NFFT CGNR of %d of %d planes (scaled x%.0f), %d TV sweeps (scaled x%.0f), "
                       "one z FFT (x2); %.1f s of host work" % (threads, cpu_model(), nplanes, nz, nz / nplanes,
                                                                sweeps, 40 / sweeps, sample)}
+
+
+def workload_name(cfg, prm):
+    if cfg == "c3":
+        return ("c3: 256^3 blob phantom, np=384, nz=256, 71 tilts -70..70 step 2, SNR 2, padangle 1, weight, "
+                "TV height 128 centre 192; one HIO/ER pass")
+    return ("%s: np=%d, nz=%d, tilts %g..%g step %g, padangle %g, TV height %s centre %s; one HIO/ER pass"
+            % (cfg, prm["np_"], prm["nz"], prm["gam_min"], prm["gam_max"], prm["interval"], prm["padangle"],
+               prm.get("height"), prm.get("center")))
 
 
 def traffic_from_profiles(kclass):
@@ -360,6 +387,7 @@ def main():
     # per-class HIP-event timing over separate timed runs; the dominant
     # kernel is the class with the most device time per pass
     classes = [args.kclass] if args.kclass else list(KC_CANDIDATES)
+    KC_NAMES = kc_names(prm["np_"], prm["nz"], NFFT_CHOICE["f32"]["n"], NFFT_CHOICE["f32"]["W"])
     kern = {}
     for kc in classes:
         nsteps = max(2, args.steps // 2)
@@ -444,15 +472,15 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "c3: 256^3 blob phantom, np=384, nz=256, 71 tilts -70..70 step 2, SNR 2, "
-                               "padangle 1, weight, TV height 128 centre 192; one HIO/ER pass",
+        "config": {"workload": workload_name(args.config, prm),
                    "grid": [prm["nz"], prm["np_"], prm["np_"]], "nfft": NFFT_CHOICE or None},
         "fp64_plain_passes_per_s": rate64,
         "speedup_vs_fp64_plain": (rate32 / rate64) if rate64 else None,
         "accuracy": acc or None,
         "roofline": {"kernel": kd["name"], "bound": "hbm", "achieved": kd["achieved_GBs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": kd["achieved_GBs"] / HBM_PEAK_GBS,
-                     "traffic": traffic_from_profiles(dom), "algorithmic_bytes": kd["bytes_per_launch"],
+                     "traffic": traffic_from_profiles(dom) if args.config == "c3" else None,
+                     "algorithmic_bytes": kd["bytes_per_launch"],
                      "avg_us": kd["avg_us"]},
         "kernels": {KC_NAMES.get(k): {kk: round(vv, 4) for kk, vv in v.items() if kk != "name"}
                     for k, v in kern.items()},

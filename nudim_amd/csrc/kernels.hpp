@@ -754,24 +754,25 @@ k_tv_fused(const T *__restrict__ u, const T *__restrict__ g, T *__restrict__ un,
   }
 }
 
-// k_tv_fused on four consecutive z per lane (nz % 4 == 0).  A lane reads the
-// seven stencil columns of its quad z0..z0+3 with one 16-byte load each; the
-// values one step outside the quad (z0-1 of the centre, y+1 and x+1 columns,
-// z0+4 of the centre, y-1 and x-1 columns) come from the neighbouring lanes,
-// which hold the adjacent quads of the same (y, x) line, and are read directly
-// only by a wave's first or last lane when its quad does not start or end a
-// line.  Reads per voxel: 14/4 full-line loads instead of 26 single ones;
-// each voxel's arithmetic is that of k_tv_fused.
-template <typename T>
-struct alignas(4 * sizeof(T)) Quad {
-  T v[4];
+// k_tv_fused on Q = 4 or 8 consecutive z per lane (nz % Q == 0).  A lane
+// reads the seven stencil columns of its run z0..z0+Q-1 with 16-byte loads;
+// the values one step outside the run (z0-1 of the centre, y+1 and x+1
+// columns, z0+Q of the centre, y-1 and x-1 columns) come from the
+// neighbouring lanes, which hold the adjacent runs of the same (y, x) line,
+// and are read directly only by a wave's first or last lane when its run
+// does not start or end a line (Q = 8 makes a wave cover a whole 512-long
+// line).  Each voxel's arithmetic is that of k_tv_fused.
+template <typename T, int Q>
+struct alignas(16) Run {
+  T v[Q];
 };
 
-template <typename T>
+template <typename T, int Q>
 __global__ void __launch_bounds__(kThreads)
-k_tv_fused4(const T *__restrict__ u, const T *__restrict__ g, T *__restrict__ un, T *__restrict__ gn, int ny, int nx,
-            int nz, int ylo, int yhi, const double *__restrict__ sums, double lam, double *__restrict__ part_g,
-            double *__restrict__ part_a) {
+k_tv_fused_q(const T *__restrict__ u, const T *__restrict__ g, T *__restrict__ un, T *__restrict__ gn, int ny,
+             int nx, int nz, int ylo, int yhi, const double *__restrict__ sums, double lam,
+             double *__restrict__ part_g, double *__restrict__ part_a) {
+  static_assert(Q == 4 || Q == 8, "runs of 4 or 8");
   __shared__ double sg[kThreads], sa[kThreads];
   const T eps = (T)1e-8;
   const double s1d = sqrt(sums[0]);
@@ -781,11 +782,11 @@ k_tv_fused4(const T *__restrict__ u, const T *__restrict__ g, T *__restrict__ un
     if constexpr (sizeof(T) == 4) return v - c * (gg * inv_s1);
     else return v - c * (gg / s1);
   };
-  using Q = Quad<T>;
-  auto nxt4 = [&](unsigned b, T(&r)[4]) {
-    const Q v = ld_at(reinterpret_cast<const Q *>(u), b), gg = ld_at(reinterpret_cast<const Q *>(g), b);
+  using R = Run<T, Q>;
+  auto nxtq = [&](unsigned b, T(&r)[Q]) {
+    const R v = ld_at(reinterpret_cast<const R *>(u), b), gg = ld_at(reinterpret_cast<const R *>(g), b);
 #pragma unroll
-    for (int k = 0; k < 4; k++) r[k] = upd(v.v[k], gg.v[k]);
+    for (int k = 0; k < Q; k++) r[k] = upd(v.v[k], gg.v[k]);
   };
   auto nxt1 = [&](unsigned b) -> T { return upd(ld_at(u, b), ld_at(g, b)); };
   // the stencil of k_tv_fused at one voxel, neighbour values named as there
@@ -815,25 +816,25 @@ k_tv_fused4(const T *__restrict__ u, const T *__restrict__ g, T *__restrict__ un
   const unsigned bxk = lb % gridDim.x;
   const int y = ylo + (int)(lb / gridDim.x);
   constexpr unsigned E = sizeof(T);
-  const unsigned hz = (unsigned)nz / 4, rl = (unsigned)nx * hz, sx = (unsigned)nz * E, sy = (unsigned)nx * sx;
+  const unsigned hz = (unsigned)nz / Q, rl = (unsigned)nx * hz, sx = (unsigned)nz * E, sy = (unsigned)nx * sx;
   const unsigned ybase = (unsigned)y * sy;
   const unsigned dym = y > 0 ? sy : 0u, dyp = y < ny - 1 ? sy : 0u;
   for (TvWalk w(bxk, gridDim.x, hz); w.rem < rl; w.next(hz)) {
-    const unsigned o = ybase + w.x * sx + w.z * 4 * E;
+    const unsigned o = ybase + w.x * sx + w.z * Q * E;
     const unsigned dxm = w.x > 0 ? sx : 0u, dxp = w.x + 1 < (unsigned)nx ? sx : 0u;
-    T cc[4], ym[4], xm[4], yp[4], ypxm[4], xp[4], ymxp[4];
-    nxt4(o, cc);
-    nxt4(o - dym, ym);
-    nxt4(o - dxm, xm);
-    nxt4(o + dyp, yp);
-    nxt4(o + dyp - dxm, ypxm);
-    nxt4(o + dxp, xp);
-    nxt4(o + dxp - dym, ymxp);
-    // one step below / above the quad: the neighbouring lane's quad, a
-    // direct read at a wave edge inside the line, the quad's own end value
-    // at the line's ends (the stencil's clamp)
+    T cc[Q], ym[Q], xm[Q], yp[Q], ypxm[Q], xp[Q], ymxp[Q];
+    nxtq(o, cc);
+    nxtq(o - dym, ym);
+    nxtq(o - dxm, xm);
+    nxtq(o + dyp, yp);
+    nxtq(o + dyp - dxm, ypxm);
+    nxtq(o + dxp, xp);
+    nxtq(o + dxp - dym, ymxp);
+    // one step below / above the run: the neighbouring lane's run, a direct
+    // read at a wave edge inside the line, the run's own end value at the
+    // line's ends (the stencil's clamp)
     const bool hm = w.z > 0, hp = w.z + 1 < hz;
-    T c_m = __shfl_up(cc[3], 1), yp_m = __shfl_up(yp[3], 1), xp_m = __shfl_up(xp[3], 1);
+    T c_m = __shfl_up(cc[Q - 1], 1), yp_m = __shfl_up(yp[Q - 1], 1), xp_m = __shfl_up(xp[Q - 1], 1);
     T c_p = __shfl_down(cc[0], 1), ym_p = __shfl_down(ym[0], 1), xm_p = __shfl_down(xm[0], 1);
     if (hm && lane == 0) {
       c_m = nxt1(o - E);
@@ -841,9 +842,9 @@ k_tv_fused4(const T *__restrict__ u, const T *__restrict__ g, T *__restrict__ un
       xp_m = nxt1(o + dxp - E);
     }
     if (hp && lane == 63) {
-      c_p = nxt1(o + 4 * E);
-      ym_p = nxt1(o - dym + 4 * E);
-      xm_p = nxt1(o - dxm + 4 * E);
+      c_p = nxt1(o + Q * E);
+      ym_p = nxt1(o - dym + Q * E);
+      xm_p = nxt1(o - dxm + Q * E);
     }
     if (!hm) {
       c_m = cc[0];
@@ -851,23 +852,22 @@ k_tv_fused4(const T *__restrict__ u, const T *__restrict__ g, T *__restrict__ un
       xp_m = xp[0];
     }
     if (!hp) {
-      c_p = cc[3];
-      ym_p = ym[3];
-      xm_p = xm[3];
+      c_p = cc[Q - 1];
+      ym_p = ym[Q - 1];
+      xm_p = xm[Q - 1];
     }
-    Q gq;
+    R gq, cq;
 #pragma unroll
-    for (int k = 0; k < 4; k++)
-      gq.v[k] = voxel(cc[k], k > 0 ? cc[k - 1] : c_m, k < 3 ? cc[k + 1] : c_p, ym[k], k < 3 ? ym[k + 1] : ym_p, xm[k],
-                      k < 3 ? xm[k + 1] : xm_p, yp[k], k > 0 ? yp[k - 1] : yp_m, ypxm[k], xp[k],
-                      k > 0 ? xp[k - 1] : xp_m, ymxp[k]);
-    Q cq;
+    for (int k = 0; k < Q; k++) {
+      gq.v[k] = voxel(cc[k], k > 0 ? cc[k - 1] : c_m, k < Q - 1 ? cc[k + 1] : c_p, ym[k],
+                      k < Q - 1 ? ym[k + 1] : ym_p, xm[k], k < Q - 1 ? xm[k + 1] : xm_p, yp[k],
+                      k > 0 ? yp[k - 1] : yp_m, ypxm[k], xp[k], k > 0 ? xp[k - 1] : xp_m, ymxp[k]);
+      cq.v[k] = cc[k];
+    }
+    st_at(reinterpret_cast<R *>(un), o, cq);
+    st_at(reinterpret_cast<R *>(gn), o, gq);
 #pragma unroll
-    for (int k = 0; k < 4; k++) cq.v[k] = cc[k];
-    st_at(reinterpret_cast<Q *>(un), o, cq);
-    st_at(reinterpret_cast<Q *>(gn), o, gq);
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < Q; k++) {
       accg += (double)gq.v[k] * (double)gq.v[k];
       if (cc[k] < (T)0) acca += (double)cc[k] * (double)cc[k];
     }

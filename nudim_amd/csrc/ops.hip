@@ -871,9 +871,11 @@ T *tv_sweeps(T *u0, T *u1, T *g0, T *g1, int ny, int nx, int nz, int ylo, int yh
   const int nrows = yhi - ylo + 1;
   const int gx = (int)std::min<long long>(cdiv(rowlen, kThreads), 64);
   const int np_ = nrows * gx;
-  // sweeps on z quads when nz % 4 == 0 (k_tv_fused4): one quad per thread
+  // sweeps on runs of 4 z per lane (k_tv_fused_q) when nz % 4 == 0.  (Runs
+  // of 8, which make a wave cover a whole 512-long line, measured slower at
+  // 512^3 in FP32: 583 against 534 us a sweep, at 3 waves per SIMD.)
   const bool quads = nz % 4 == 0;
-  const int gx4 = (int)std::min<long long>(cdiv(rowlen / 4, kThreads), 128);
+  const int gx4 = quads ? (int)std::min<long long>(cdiv(rowlen / 4, kThreads), 128) : 0;
   NUDIM_CHECK(np_ <= nb && nrows * gx4 <= nb, "TV partial-sum buffer too small");
   NUDIM_CHECK((double)V * sizeof(T) < 4294967296.0, "TV kernels address volumes of fewer than 2^32 bytes");
   const dim3 grid(gx, nrows);
@@ -895,8 +897,8 @@ T *tv_sweeps(T *u0, T *u1, T *g0, T *g1, int ny, int nx, int nz, int ylo, int yh
     {
       ProfScope ps(KC_TV, st, 2.0 * vb);
       if (quads)
-        hipLaunchKernelGGL(k_tv_fused4<T>, dim3(gx4, nrows), dim3(kThreads), 0, st, uc, gc, un, gn, ny, nx, nz, ylo,
-                           yhi, sums, lamd, part_g, part_a);
+        hipLaunchKernelGGL((k_tv_fused_q<T, 4>), dim3(gx4, nrows), dim3(kThreads), 0, st, uc, gc, un, gn, ny, nx, nz,
+                           ylo, yhi, sums, lamd, part_g, part_a);
       else
         hipLaunchKernelGGL(k_tv_fused<T>, grid, dim3(kThreads), 0, st, uc, gc, un, gn, ny, nx, nz, ylo, yhi, sums,
                            lamd, part_g, part_a);

@@ -226,6 +226,41 @@ def test_tv_fp32(hip, orc):
     assert rel(hip.tv(u, niter=1, precision=1), orc.tv(u, niter=1)) < 1e-5
 
 
+# The sweep kernels by contiguous extent nz (the first axis here): nz % 4 == 0
+# takes runs of 4 z per lane, FP32 with nz >= 512 and nz % 8 == 0 runs of 8;
+# a wave of runs may cover part of a line (1040: edge lanes read their
+# neighbours directly), exactly one line (256 / 512) or several (36, 16).
+@pytest.mark.parametrize("shape", [(16, 7, 9), (36, 5, 7), (256, 4, 5), (512, 5, 6), (1040, 3, 4)])
+def test_tv_runs(hip, orc, shape):
+    u = smooth_volume(shape, 11)
+    # three sweeps (two through the run kernel) move u by 8-16% and stay well
+    # conditioned: the oracle's own response to FP32-sized input noise is
+    # below 5e-7 on these shapes
+    ref3 = orc.tv(u, niter=3)
+    assert rel(hip.tv(u, niter=3), ref3) < 1e-13
+    assert rel(hip.tv(u, niter=3, precision=1), ref3) < 1e-5
+    assert rel(hip.tv(u, niter=1), orc.tv(u, niter=1)) < 1e-14
+    # 20 sweeps: FP32 within a small multiple of the oracle's response to an
+    # input perturbation of FP32 rounding size.  Past ~20 sweeps that response
+    # varies 10-40x between perturbation samples (t / sqrt(1e-8 + |t|^2) near
+    # flat regions), so one sample does not bound an FP32 result there.
+    ref20 = orc.tv(u, niter=20)
+    floor20 = rel(orc.tv(perturbed(u, 6e-8, 3), niter=20), ref20)
+    assert rel(hip.tv(u, niter=20, precision=1), ref20) < 4 * floor20 + 1e-6
+    # 40 sweeps in FP64: the same amplification (floor / 6e-8, up to ~1e5)
+    # applied to FP64 rounding, 4e-7 * floor ~ 200 ulp
+    ref = orc.tv(u)
+    floor = rel(orc.tv(perturbed(u, 6e-8, 3)), ref)
+    assert rel(hip.tv(u), ref) < 1e-11 + 4e-7 * floor
+    # a band of rows (the other rows untouched)
+    ny = shape[1]
+    ref_h = orc.tv_height(u, 2, ny // 2)
+    floor_h = rel(orc.tv_height(perturbed(u, 6e-8, 4), 2, ny // 2), ref_h)
+    assert rel(hip.tv_height(u, 2, ny // 2), ref_h) < 1e-11 + 4e-7 * floor_h
+    ref_h3 = orc.tv_height(u, 2, ny // 2, niter=3)
+    assert rel(hip.tv_height(u, 2, ny // 2, niter=3, precision=1), ref_h3) < 1e-5
+
+
 @pytest.mark.parametrize("prec,tol", [(0, 1e-11), (1, 2e-5)])
 def test_diffusion(hip, orc, prec, tol):
     u = smooth_volume((12, 10, 12), 6)
