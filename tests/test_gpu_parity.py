@@ -146,6 +146,33 @@ def test_cgnr_register_fft_sizes(hip, orc, N):
     assert rel(got32, ref) < 1e-4
 
 
+@pytest.mark.parametrize("N", [192, 256, 384, 512, 768])
+def test_cgnr_toeplitz_lengths(hip, orc, N):
+    """The fused Toeplitz passes at the register-FFT lengths of the full
+    workloads, L = 2N = 384 (24 x 16), 512 (32 x 16), 768 (32 x 24, the
+    256^3 workload at np = 384), 1024 (32 x 32) and 1536 (48 x 32, np = 768),
+    against the FP64 NFFT CGNR at the reference's sizes (n = 2 nextpow2(N),
+    W = 14), whose parity with the oracle is pinned at N = 32-128 above and
+    on one plane here for N <= 256: FP64 Toeplitz to 1e-9, FP32 to 1e-4."""
+    rng = np.random.default_rng(N + 1)
+    pmin, pmax = orc.pad_range(-70, 2)
+    x = orc.polar_nodes(pmin, pmax, 1.0, N)
+    M, P = x.shape[0], 16
+    w = orc.weights(N, M // N, 1.0)
+    y = rng.standard_normal((P, M)) + 1j * rng.standard_normal((P, M))
+    f0 = rng.standard_normal((P, N, N))
+    ref = hip.cgnr(x, y, f0, w)
+    if N <= 256:
+        plan = orc.NFFT(x, N)
+        assert rel(ref[P - 1], plan.cgnr(y[P - 1], f0[P - 1], w, 10)) < 1e-10
+    got_t = hip.cgnr(x, y, f0, w, form=hip.CG_TOEPLITZ)
+    assert rel(got_t, ref) < 1e-9
+    got32 = hip.cgnr(x, y, f0, w, n=2 * N, W=8, precision=hip.FP32)
+    assert rel(got32, ref) < 1e-4
+    for p in (0, 7, P - 1):
+        assert rel(got32[p], ref[p]) < 1e-4
+
+
 def test_toeplitz_plane_chunks(hip, orc, monkeypatch):
     """The fused Toeplitz passes over plane chunks (long rows of the 2N x 2N
     intermediate, forced here by a small row budget) equal the single-chunk
