@@ -50,7 +50,7 @@ def kc_names(np_, nz, n, W):
     zf = "k_zfft2<float,%d,%d> (z transforms)" % ZFFT2_SPLIT[nz] if nz in ZFFT2_SPLIT else "k_zfft_fwd/inv<float>"
     gather = ("k_gather_chunk<float,%d> (NFFT adjoint)" % W if nz % 64 == 0
               else "k_gather_w<float,%d> (NFFT adjoint)" % W)
-    return {1: toep(1, "Toeplitz column pass"), 2: "k_interp_w<float,%d>" % W, 3: gather, 4: grid,
+    return {1: toep(1, "Toeplitz column pass"), 2: "k_interp_rsums<float,%d> (R sums at the known nodes)" % W, 3: gather, 4: grid,
             5: "k_tv_fused_q<float,4>" if nz % 4 == 0 else "k_tv_fused<float>", 6: zf,
             7: toep(0, "Toeplitz row forward + CG update"), 8: toep(2, "Toeplitz row inverse + <p,q>"),
             9: toep(3, "two-input column pass, CG start"), 10: "k_cg_zupdate<float>"}
@@ -424,12 +424,24 @@ def main():
             v32 = j32.volume()
             j64.initial()
             v0_64 = j64.volume()
-            acc = {"rel_l2_initial": rel(v0, v0_64), "rel_l2_after_passes": rel(v32, v64),
-                   "passes_compared": npass,
-                   "max_abs_rfactor_drift": float(np.max(np.abs(r32a - r64))),
-                   "rfactor_fp64": [float(v) for v in r64], "rfactor_fp32": [float(v) for v in r32a]}
-            log("  rel L2 FP32 vs FP64: initial %.2e, after %d passes %.2e" % (acc["rel_l2_initial"], npass,
-                                                                              acc["rel_l2_after_passes"]))
+            # the FP64 path's own distance after the same passes from its
+            # initial volume times (1 + 1e-7 noise): the chaotic floor of
+            # the HIO + TV schedule
+            rng = np.random.default_rng(1)
+            j64.set_volume(v0_64 * (1 + 1e-7 * rng.standard_normal(v0_64.shape)))
+            rp64 = j64.iterate(npass)
+            floor = rel(j64.volume(), v64)
+            acc = {"criterion": "rel L2 <= 1e-4 of the volume on ER passes without TV (er_no_tv); the configured "
+                                "HIO + TV passes are chaotic in the reference's own iteration, so there FP32 is "
+                                "compared with the FP64 path's distance from itself after a 1e-7 perturbation",
+                   "rel_l2_initial": rel(v0, v0_64),
+                   "hio_tv": {"passes": npass, "rel_l2_fp32_vs_fp64": rel(v32, v64),
+                              "rel_l2_fp64_vs_fp64_perturbed_1e-7": floor,
+                              "max_abs_rfactor_drift": float(np.max(np.abs(r32a - r64))),
+                              "max_abs_rfactor_drift_fp64_perturbed": float(np.max(np.abs(rp64 - r64))),
+                              "rfactor_fp64": [float(v) for v in r64], "rfactor_fp32": [float(v) for v in r32a]}}
+            log("  rel L2 FP32 vs FP64: initial %.2e; after %d HIO+TV passes %.2e (FP64 vs itself after a 1e-7 "
+                "perturbation: %.2e)" % (acc["rel_l2_initial"], npass, acc["hio_tv"]["rel_l2_fp32_vs_fp64"], floor))
         j64.close()
     j32.close()
     if not args.no_plain and not args.no_accuracy:
@@ -445,10 +457,11 @@ def main():
             rs.append(j.iterate(npass))
             vols.append(j.volume())
             j.close()
-        acc["er_no_denoise"] = {"passes": npass, "rel_l2": rel(vols[0], vols[1]),
-                                "max_abs_rfactor_drift": float(np.max(np.abs(rs[0] - rs[1])))}
+        e = rel(vols[0], vols[1])
+        acc["er_no_tv"] = {"passes": npass, "rel_l2": e, "within_1e-4": bool(e <= 1e-4),
+                           "max_abs_rfactor_drift": float(np.max(np.abs(rs[0] - rs[1])))}
         log("  ER without TV, %d passes: rel L2 FP32 vs FP64 %.2e, R drift %.2e"
-            % (npass, acc["er_no_denoise"]["rel_l2"], acc["er_no_denoise"]["max_abs_rfactor_drift"]))
+            % (npass, e, acc["er_no_tv"]["max_abs_rfactor_drift"]))
 
     serving = serve(args, nd.FP32) if args.serve_jobs > 0 else None
 
