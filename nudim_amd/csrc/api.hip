@@ -564,12 +564,16 @@ struct nudim_ctx {
   DevBuf rcur;  // R of a replayed pass, copied to its trace slot after the graph
   std::vector<PassGraph> graphs;
   KernelProfiler prof;
+  // also the clean-up of a context whose creation failed part way
+  ~nudim_ctx() {
+    (void)hipSetDevice(p.device);
+    if (st) (void)hipStreamSynchronize(st);
+    for (auto &g : graphs) (void)hipGraphExecDestroy(g.exec);
+    graphs.clear();
+    job.reset();
+    if (st) (void)hipStreamDestroy(st);
+  }
 };
-
-static void destroy_graphs(nudim_ctx *c) {
-  for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.exec);
-  c->graphs.clear();
-}
 
 // pass nn through the graph captured for the job's current state
 static void iterate_graph(nudim_ctx *c, int nn, double *rslot) {
@@ -657,19 +661,7 @@ extern "C" int nudim_create(const nudim_params *p, nudim_ctx **out) {
   API_END
 }
 
-extern "C" void nudim_destroy(nudim_ctx *c) {
-  if (!c) return;
-  try {
-    (void)hipSetDevice(c->p.device);
-    if (c->st) (void)hipStreamSynchronize(c->st);
-    destroy_graphs(c);
-    c->job.reset();
-    c->rtrace.release();
-    if (c->st) (void)hipStreamDestroy(c->st);
-  } catch (...) {
-  }
-  delete c;
-}
+extern "C" void nudim_destroy(nudim_ctx *c) { delete c; }
 
 extern "C" int nudim_sizes(const nudim_ctx *c, int *nmeas, int *M, int *nang) {
   if (!c) return fail(NUDIM_EINVAL, "null context");

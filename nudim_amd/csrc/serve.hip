@@ -182,7 +182,18 @@ extern "C" int nudim_server_create(const nudim_server_opts *o, nudim_server **ou
     }
     s->epoch = clk::now();
     const int nslots = o->max_active * (s->partition ? 2 : 1);
-    for (int i = 0; i < nslots; i++) s->slots.emplace_back([p = s.get()] { p->slot_main(); });
+    try {
+      for (int i = 0; i < nslots; i++) s->slots.emplace_back([p = s.get()] { p->slot_main(); });
+    } catch (...) {
+      // the slots already started hold the server: stop and join them first
+      {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+      }
+      s->cv_work.notify_all();
+      for (auto &t : s->slots) t.join();
+      throw;
+    }
     *out = s.release();
     return NUDIM_OK;
   } catch (const std::exception &e) {
