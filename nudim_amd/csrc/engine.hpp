@@ -20,12 +20,13 @@ bool fft2_launch(int n, const cx<T> *in, cx<T> *out, int nlines, const LineIO &l
 extern bool g_use_fft2;
 // Toeplitz passes with the CGNR vector work (fft2.hip) for n = 2N in
 // toep_supported(n); kind is TK_ROWF, TK_COL or TK_ROWI.  P planes from the
-// given (offset) pointers; pstride = row stride of cg.part.
+// given (offset) pointers; pstride = row stride of cg.part.  pairs: two lanes
+// per line in the 48-point stage of n = 1536 (same results to rounding).
 bool toep_supported(int n);
 template <typename T>
 void toep_launch(int kind, int n, int nlines, const cx<T> *in, cx<T> *out, long long in_so, long long in_sk,
                  long long out_so, long long out_sk, int P, hipStream_t st, const cx<T> *tw, const T *mid,
-                 const ToepCg<T> &cg, int pstride);
+                 const ToepCg<T> &cg, int pstride, bool pairs);
 
 // z-axis transforms of [line][z] volumes on the fixed-length kernels
 // (k_zfft2, fft2.hpp); false when n has no specialisation
@@ -118,6 +119,7 @@ struct ToeplitzOp {
   std::shared_ptr<const ToepSpectrum<T>> spec;
   DevBuf X;     // [N][L][P]; for the fused passes [P/pc][N][L][pc]
   int pc = 0;   // planes per chunk of X in the fused passes
+  bool pairs = true;  // lane pairs in the 48-point stage (NUDIM_TOEP_PAIRS=0: off)
   // second kernel (init_second) and intermediate for apply_start
   std::shared_ptr<const ToepSpectrum<T>> spec2;
   DevBuf X2;

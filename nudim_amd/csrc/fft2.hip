@@ -41,35 +41,49 @@ bool launch_shape(const cx<T> *in, cx<T> *out, int nlines, const LineIO &li, con
   return true;
 }
 
-template <typename T, int N1, int N2>
-void launch_toep(int kind, int nlines, const cx<T> *in, cx<T> *out, long long in_so, long long in_sk,
-                 long long out_so, long long out_sk, int P, hipStream_t st, const cx<T> *tw, const T *mid,
-                 const ToepCg<T> &cg, int pstride) {
-  constexpr int n = N1 * N2;
-  int cap = std::max(1, std::min(256, g_fft2_threads) / N1);
+template <typename T, int N1, int N2, int H>
+void launch_toep_h(int kind, int nlines, const cx<T> *in, cx<T> *out, long long in_so, long long in_sk,
+                   long long out_so, long long out_sk, int P, hipStream_t st, const cx<T> *tw, const T *mid,
+                   const ToepCg<T> &cg, int pstride) {
+  constexpr int n = N1 * N2, TL = (N1 > H * N2 ? N1 : H * N2);  // threads per plane
+  int cap = std::max(1, std::min(256, g_fft2_threads) / TL);
   int Gs = 0;
   while ((2 << Gs) <= cap && P % (2 << Gs) == 0) Gs++;
   const int G = 1 << Gs;
   const size_t lds = (size_t)N2 * (N1 * G + G) * (toep_split(kind) ? 1 : 2) * sizeof(T) + (size_t)n * sizeof(cx<T>);
   const int nchunk = P >> Gs;
-  dim3 grid((unsigned)((long long)nlines * nchunk)), block(N1 * G);
+  dim3 grid((unsigned)((long long)nlines * nchunk)), block(TL * G);
   switch (kind) {
     case TK_ROWF:
-      hipLaunchKernelGGL((k_toep<T, N1, N2, TK_ROWF>), grid, block, lds, st, in, out, in_so, in_sk, out_so, out_sk,
-                         tw, Gs, nchunk, mid, cg, pstride);
+      hipLaunchKernelGGL((k_toep<T, N1, N2, TK_ROWF, H>), grid, block, lds, st, in, out, in_so, in_sk, out_so,
+                         out_sk, tw, Gs, nchunk, mid, cg, pstride);
       break;
     case TK_COL:
-      hipLaunchKernelGGL((k_toep<T, N1, N2, TK_COL>), grid, block, lds, st, in, out, in_so, in_sk, out_so, out_sk,
-                         tw, Gs, nchunk, mid, cg, pstride);
+      hipLaunchKernelGGL((k_toep<T, N1, N2, TK_COL, H>), grid, block, lds, st, in, out, in_so, in_sk, out_so,
+                         out_sk, tw, Gs, nchunk, mid, cg, pstride);
       break;
     case TK_COL2:
-      hipLaunchKernelGGL((k_toep<T, N1, N2, TK_COL2>), grid, block, lds, st, in, out, in_so, in_sk, out_so, out_sk,
-                         tw, Gs, nchunk, mid, cg, pstride);
+      hipLaunchKernelGGL((k_toep<T, N1, N2, TK_COL2, H>), grid, block, lds, st, in, out, in_so, in_sk, out_so,
+                         out_sk, tw, Gs, nchunk, mid, cg, pstride);
       break;
     default:
-      hipLaunchKernelGGL((k_toep<T, N1, N2, TK_ROWI>), grid, block, lds, st, in, out, in_so, in_sk, out_so, out_sk,
-                         tw, Gs, nchunk, mid, cg, pstride);
+      hipLaunchKernelGGL((k_toep<T, N1, N2, TK_ROWI, H>), grid, block, lds, st, in, out, in_so, in_sk, out_so,
+                         out_sk, tw, Gs, nchunk, mid, cg, pstride);
   }
+}
+
+// lane pairs (H = 2) for the 48-point stage of n = 1536 unless pairs is false
+template <typename T, int N1, int N2>
+void launch_toep(int kind, int nlines, const cx<T> *in, cx<T> *out, long long in_so, long long in_sk,
+                 long long out_so, long long out_sk, int P, hipStream_t st, const cx<T> *tw, const T *mid,
+                 const ToepCg<T> &cg, int pstride, bool pairs) {
+  if constexpr (N1 == 48) {
+    if (pairs) {
+      launch_toep_h<T, N1, N2, 2>(kind, nlines, in, out, in_so, in_sk, out_so, out_sk, P, st, tw, mid, cg, pstride);
+      return;
+    }
+  }
+  launch_toep_h<T, N1, N2, 1>(kind, nlines, in, out, in_so, in_sk, out_so, out_sk, P, st, tw, mid, cg, pstride);
 }
 
 }  // namespace
@@ -84,9 +98,10 @@ bool toep_supported(int n) {
 template <typename T>
 void toep_launch(int kind, int n, int nlines, const cx<T> *in, cx<T> *out, long long in_so, long long in_sk,
                  long long out_so, long long out_sk, int P, hipStream_t st, const cx<T> *tw, const T *mid,
-                 const ToepCg<T> &cg, int pstride) {
-#define NUDIM_TOEP(NN, A, B) \
-  case NN: launch_toep<T, A, B>(kind, nlines, in, out, in_so, in_sk, out_so, out_sk, P, st, tw, mid, cg, pstride); \
+                 const ToepCg<T> &cg, int pstride, bool pairs) {
+#define NUDIM_TOEP(NN, A, B)                                                                                   \
+  case NN:                                                                                                     \
+    launch_toep<T, A, B>(kind, nlines, in, out, in_so, in_sk, out_so, out_sk, P, st, tw, mid, cg, pstride, pairs); \
     return;
   switch (n) {
     NUDIM_TOEP(128, 16, 8)
@@ -103,10 +118,10 @@ void toep_launch(int kind, int n, int nlines, const cx<T> *in, cx<T> *out, long 
 
 template void toep_launch<float>(int, int, int, const cx<float> *, cx<float> *, long long, long long, long long,
                                  long long, int, hipStream_t, const cx<float> *, const float *,
-                                 const ToepCg<float> &, int);
+                                 const ToepCg<float> &, int, bool);
 template void toep_launch<double>(int, int, int, const cx<double> *, cx<double> *, long long, long long, long long,
                                   long long, int, hipStream_t, const cx<double> *, const double *,
-                                  const ToepCg<double> &, int);
+                                  const ToepCg<double> &, int, bool);
 
 // z-axis transforms (k_zfft2) for n in {64, 128, 256, 512}; false when no
 // fixed-length kernel exists (the caller then uses the generic kernels)

@@ -213,6 +213,43 @@ __device__ __forceinline__ void xchg(T *lx, const cx<T> *wv, bool wact, int wbas
   }
 }
 
+// xchg with the LDS index of each write k and read j given by wi(k), ri(j)
+template <typename T, int NW, int NR, bool SPLIT, typename WF, typename RF>
+__device__ __forceinline__ void xchg_at(T *lx, const cx<T> *wv, bool wact, WF wi, cx<T> *rv, bool ract, RF ri) {
+  if constexpr (!SPLIT) {
+    cx<T> *lc = reinterpret_cast<cx<T> *>(lx);
+    if (wact) {
+#pragma unroll
+      for (int k = 0; k < NW; k++) lc[wi(k)] = wv[k];
+    }
+    __syncthreads();
+    if (ract) {
+#pragma unroll
+      for (int j = 0; j < NR; j++) rv[j] = lc[ri(j)];
+    }
+    return;
+  }
+  if (wact) {
+#pragma unroll
+    for (int k = 0; k < NW; k++) lx[wi(k)] = wv[k].x;
+  }
+  __syncthreads();
+  if (ract) {
+#pragma unroll
+    for (int j = 0; j < NR; j++) rv[j].x = lx[ri(j)];
+  }
+  __syncthreads();
+  if (wact) {
+#pragma unroll
+    for (int k = 0; k < NW; k++) lx[wi(k)] = wv[k].y;
+  }
+  __syncthreads();
+  if (ract) {
+#pragma unroll
+    for (int j = 0; j < NR; j++) rv[j].y = lx[ri(j)];
+  }
+}
+
 // v * exp(-S 2 pi i q / 4): a quarter-turn rotation (no flops)
 template <typename T, int S>
 __device__ __forceinline__ cx<T> mul_quarter(cx<T> v, int q) {
@@ -370,15 +407,52 @@ k_fft2(const cx<T> *in, cx<T> *out, LineIO li, LineIO lo, const cx<T> *__restric
 // (n positions) -> out = q (N positions).  LDS: the exchange buffer
 // [N2][N1*G + G] (real with toep_split(KIND), else complex) followed by the
 // n twiddles.
+//
+// H lanes per step-A line.  With H = 2 the two adjacent lanes h = 0, 1 of a
+// pair share line position r and each holds N1/2 of its values, which halves
+// the registers of the N1-point stage (N1 = 48 at n = 1536):
+//   - half-input forward DFT (ROWF, COL): even outputs = DFT_{N1/2}(v), odd
+//     outputs = DFT_{N1/2}(v[j] w_N1^{-j}); lane h computes parity h, k1 =
+//     2s + h, both lanes loading the N1/2 inputs;
+//   - full-input inverse DFT (ROWI): lane h transforms the inputs of parity h
+//     (E, O), then X[k] = E[k] + w^k O[k] and X[k + N1/2] = E[k] - w^k O[k]
+//     with the lanes swapping N1/4 values: lane h keeps k in [h N1/4,
+//     (h+1) N1/4);
+//   - half-output last DFT of the column pass: Y[m] = E[m] + w^m O[m] for
+//     m < N1/2, E from the even k1, O from the odd, the same swap.
+// Step B keeps one thread per (k1, plane): blocks of max(N1, H N2) * G.
 constexpr bool toep_split(int kind) { return kind != TK_ROWF; }
 
-template <typename T, int N1, int N2, int KIND>
+// the pair swap of the H = 2 forms: a[0..2Q) holds E (lane 0) or w^k O
+// (lane 1) for k < 2Q; on return a[i] = X[k], a[Q + i] = X[k + 2Q] (first
+// half-output form: a[i] = Y[k] only) for this lane's k = Q h + i
+template <typename T, int Q, bool BOTH>
+__device__ __forceinline__ void pair_swap(cx<T> *a, int h) {
+  cx<T> c[2 * Q];
+#pragma unroll
+  for (int i = 0; i < Q; i++) {
+    const cx<T> send = h ? a[i] : a[Q + i];
+    cx<T> recv;
+    recv.x = __shfl_xor(send.x, 1);
+    recv.y = __shfl_xor(send.y, 1);
+    const cx<T> e = h ? recv : a[i], od = h ? a[Q + i] : recv;
+    c[i] = e + od;
+    if (BOTH) c[Q + i] = e - od;
+  }
+#pragma unroll
+  for (int i = 0; i < (BOTH ? 2 * Q : Q); i++) a[i] = c[i];
+}
+
+template <typename T, int N1, int N2, int KIND, int H>
 __global__ void __launch_bounds__(256)
 k_toep(const cx<T> *in, cx<T> *out, long long in_so, long long in_sk, long long out_so, long long out_sk,
        const cx<T> *__restrict__ tw, int Gs, int nchunk, const T *__restrict__ mid, ToepCg<T> cg, int pstride) {
   constexpr int n = N1 * N2, H1 = N1 / 2, H2 = N2 / 2;
+  constexpr int A = N1 / H;   // step-A values per lane
+  constexpr int Q4 = N1 / 4;  // values a pair swaps (H = 2)
   constexpr bool SPLIT = toep_split(KIND);
   static_assert(N1 % 2 == 0 && N2 % 2 == 0 && N1 >= N2, "even factors, N1 >= N2");
+  static_assert(H == 1 || (H == 2 && N1 % 4 == 0), "lane pairs need N1 % 4 == 0");
   extern __shared__ unsigned char smem_raw[];
   T *lx = reinterpret_cast<T *>(smem_raw);
   const int G = 1 << Gs, G1 = G - 1;
@@ -388,120 +462,177 @@ k_toep(const cx<T> *in, cx<T> *out, long long in_so, long long in_sk, long long 
   const int o = lb / nchunk, chunk = lb - o * nchunk;
   const int tid = threadIdx.x;
   for (int i = tid; i < n; i += blockDim.x) stw[i] = tw[i];
-  const int g = tid & G1, r = tid >> Gs;
+  // step A: lane h of line position r < N2, plane g
+  const int h = H == 2 ? (tid & 1) : 0;
+  const int g = (tid >> (H - 1)) & G1, r = tid >> (Gs + H - 1);
   const bool actA = r < N2;
   const int plane = chunk * G + g;
+  // step B: column k1 = kb < N1, plane gb
+  const int gb = tid & G1, kb = tid >> Gs;
+  const bool actB = H == 1 ? true : kb < N1;
+  const int planeB = chunk * G + gb;
   constexpr int SA = (KIND == TK_ROWI) ? +1 : -1;
+  // k1 of step-A value s of this lane
+  auto k1of = [&](int s) -> int {
+    if constexpr (H == 1) return s;
+    else if constexpr (KIND == TK_ROWI) return s < Q4 ? Q4 * h + s : H1 + Q4 * h + (s - Q4);
+    else return 2 * s + h;
+  };
 
   // ---- step A ----
-  cx<T> a[N1];
+  cx<T> a[A];
   if (actA) {
     if constexpr (KIND == TK_ROWI) {
       const cx<T> *src = in + (long long)o * in_so + plane;
+      if constexpr (H == 1) {
 #pragma unroll
-      for (int j1 = 0; j1 < N1; j1++) a[j1] = src[(long long)(r + N2 * j1) * in_sk];
-      dft_reg<T, N1, +1>(a);
+        for (int j1 = 0; j1 < N1; j1++) a[j1] = src[(long long)(r + N2 * j1) * in_sk];
+        dft_reg<T, N1, +1>(a);
+      } else {
+#pragma unroll
+        for (int j = 0; j < A; j++) a[j] = src[(long long)(r + N2 * (2 * j + h)) * in_sk];
+        dft_reg<T, A, +1>(a);
+        if (h) {
+#pragma unroll
+          for (int k = 1; k < A; k++) a[k] = a[k] * wconst<T, +1, N1>(k);
+        }
+        pair_swap<T, Q4, true>(a, h);
+      }
     } else {
       // positions r + N2 j1 < N for j1 < N1/2; the rest are zero
       const long long base = (long long)o * in_so + plane;
       if (KIND == TK_ROWF && cg.z) {
-        cx<T> zv[H1];
+        // p = z + beta p_old (and f += alpha p_old) at positions j1 < N1/2;
+        // with lane pairs lane h updates j1 = Q4 h + i and the pair swaps
+        // the new p values
+        constexpr int NV = H1 / H;
+        auto pos = [&](int i) { return base + (long long)(r + N2 * (NV * h + i)) * in_sk; };
+        cx<T> zv[NV];
 #pragma unroll
-        for (int j1 = 0; j1 < H1; j1++) zv[j1] = cg.z[base + (long long)(r + N2 * j1) * in_sk];
+        for (int i = 0; i < NV; i++) zv[i] = cg.z[pos(i)];
         if (cg.zz_old) {
-          cx<T> pv[H1];
+          cx<T> pv[NV];
 #pragma unroll
-          for (int j1 = 0; j1 < H1; j1++) pv[j1] = cg.p[base + (long long)(r + N2 * j1) * in_sk];
+          for (int i = 0; i < NV; i++) pv[i] = cg.p[pos(i)];
           if (cg.alpha) {
             const T al = (T)cg.alpha[plane];
-            cx<T> fv[H1];
+            cx<T> fv[NV];
 #pragma unroll
-            for (int j1 = 0; j1 < H1; j1++) fv[j1] = cg.f[base + (long long)(r + N2 * j1) * in_sk];
+            for (int i = 0; i < NV; i++) fv[i] = cg.f[pos(i)];
 #pragma unroll
-            for (int j1 = 0; j1 < H1; j1++) cg.f[base + (long long)(r + N2 * j1) * in_sk] = fv[j1] + al * pv[j1];
+            for (int i = 0; i < NV; i++) cg.f[pos(i)] = fv[i] + al * pv[i];
           }
           const T b = (T)(cg.zz[plane] / cg.zz_old[plane]);
 #pragma unroll
-          for (int j1 = 0; j1 < H1; j1++) a[j1] = b * pv[j1] + zv[j1];
-        } else {
-#pragma unroll
-          for (int j1 = 0; j1 < H1; j1++) a[j1] = zv[j1];
+          for (int i = 0; i < NV; i++) zv[i] = b * pv[i] + zv[i];
         }
 #pragma unroll
-        for (int j1 = 0; j1 < H1; j1++) cg.p[base + (long long)(r + N2 * j1) * in_sk] = a[j1];
+        for (int i = 0; i < NV; i++) cg.p[pos(i)] = zv[i];
+        if constexpr (H == 1) {
+#pragma unroll
+          for (int i = 0; i < NV; i++) a[i] = zv[i];
+        } else {
+#pragma unroll
+          for (int i = 0; i < NV; i++) {
+            cx<T> other;
+            other.x = __shfl_xor(zv[i].x, 1);
+            other.y = __shfl_xor(zv[i].y, 1);
+            a[i] = h ? other : zv[i];
+            a[NV + i] = h ? zv[i] : other;
+          }
+        }
       } else {
         const cx<T> *src = in + base;
 #pragma unroll
         for (int j1 = 0; j1 < H1; j1++) a[j1] = src[(long long)(r + N2 * j1) * in_sk];
       }
-      dft_half_in<T, N1, -1>(a);
+      if constexpr (H == 1) {
+        dft_half_in<T, N1, -1>(a);
+      } else {
+        if (h) {
+#pragma unroll
+          for (int j = 1; j < A; j++) a[j] = a[j] * wconst<T, -1, N1>(j);
+        }
+        dft_reg<T, A, -1>(a);
+      }
     }
   }
   __syncthreads();  // twiddle table complete
   if (actA) {
 #pragma unroll
-    for (int k1 = 0; k1 < N1; k1++) {
-      cx<T> w = stw[r * k1];
+    for (int s = 0; s < A; s++) {
+      cx<T> w = stw[r * k1of(s)];
       if (SA > 0) w = conjx(w);
-      a[k1] = a[k1] * w;
+      a[s] = a[s] * w;
     }
   }
-  // ---- step B (thread r = k1 < N1: every thread) ----
+  // ---- step B (thread kb = k1 < N1) ----
   cx<T> b[N2];
-  xchg<T, N1, N2, SPLIT>(lx, a, actA, r * rowA + g, G, b, true, r * G + g, rowA);
+  auto wA = [&](int s) { return r * rowA + k1of(s) * G + g; };
+  auto rB = [&](int j2) { return j2 * rowA + kb * G + gb; };
+  if constexpr (H == 1)
+    xchg<T, N1, N2, SPLIT>(lx, a, actA, r * rowA + g, G, b, true, r * G + g, rowA);
+  else
+    xchg_at<T, A, N2, SPLIT>(lx, a, actA, wA, b, actB, rB);
   if constexpr (KIND == TK_ROWF) {
-    dft_reg<T, N2, -1>(b);
-    cx<T> *dst = out + (long long)o * out_so + plane;
+    if (actB) {
+      dft_reg<T, N2, -1>(b);
+      cx<T> *dst = out + (long long)o * out_so + planeB;
 #pragma unroll
-    for (int k2 = 0; k2 < N2; k2++) dst[(long long)(r + N1 * k2) * out_sk] = b[k2];
+      for (int k2 = 0; k2 < N2; k2++) dst[(long long)(kb + N1 * k2) * out_sk] = b[k2];
+    }
   } else if constexpr (KIND == TK_ROWI) {
-    // outputs r + N1 k2 < N for k2 < N2/2
-    dft_half_out<T, N2, +1>(b);
-    const long long ob = (long long)o * out_so + plane;
-    cx<T> *dst = out + ob;
-    if (cg.add) {
-      cx<T> av[H2];
-#pragma unroll
-      for (int k2 = 0; k2 < H2; k2++) av[k2] = cg.add[ob + (long long)(r + N1 * k2) * out_sk];
-#pragma unroll
-      for (int k2 = 0; k2 < H2; k2++) b[k2] = av[k2] + b[k2];
-    }
-    // row sum: Re <p, q> with p, |q|^2 without (when part is given)
+    // outputs kb + N1 k2 < N for k2 < N2/2
     double s = 0;
-    if (cg.p) {
-      const cx<T> *pp = cg.p + ob;
-      cx<T> pv[H2];
+    if (actB) {
+      dft_half_out<T, N2, +1>(b);
+      const long long ob = (long long)o * out_so + planeB;
+      cx<T> *dst = out + ob;
+      if (cg.add) {
+        cx<T> av[H2];
 #pragma unroll
-      for (int k2 = 0; k2 < H2; k2++) pv[k2] = pp[(long long)(r + N1 * k2) * out_sk];
+        for (int k2 = 0; k2 < H2; k2++) av[k2] = cg.add[ob + (long long)(kb + N1 * k2) * out_sk];
 #pragma unroll
-      for (int k2 = 0; k2 < H2; k2++) s += (double)pv[k2].x * b[k2].x + (double)pv[k2].y * b[k2].y;
-    } else if (cg.part) {
+        for (int k2 = 0; k2 < H2; k2++) b[k2] = av[k2] + b[k2];
+      }
+      // row sum: Re <p, q> with p, |q|^2 without (when part is given)
+      if (cg.p) {
+        const cx<T> *pp = cg.p + ob;
+        cx<T> pv[H2];
 #pragma unroll
-      for (int k2 = 0; k2 < H2; k2++) s += (double)b[k2].x * b[k2].x + (double)b[k2].y * b[k2].y;
+        for (int k2 = 0; k2 < H2; k2++) pv[k2] = pp[(long long)(kb + N1 * k2) * out_sk];
+#pragma unroll
+        for (int k2 = 0; k2 < H2; k2++) s += (double)pv[k2].x * b[k2].x + (double)pv[k2].y * b[k2].y;
+      } else if (cg.part) {
+#pragma unroll
+        for (int k2 = 0; k2 < H2; k2++) s += (double)b[k2].x * b[k2].x + (double)b[k2].y * b[k2].y;
+      }
+#pragma unroll
+      for (int k2 = 0; k2 < H2; k2++) dst[(long long)(kb + N1 * k2) * out_sk] = b[k2];
     }
-#pragma unroll
-    for (int k2 = 0; k2 < H2; k2++) dst[(long long)(r + N1 * k2) * out_sk] = b[k2];
     if (cg.part) {
-      // fixed-order sum over the N1 threads of each plane
+      // fixed-order sum over the N1 step-B threads of each plane
       __syncthreads();  // exchange buffer free
       double *sh = reinterpret_cast<double *>(smem_raw);
-      sh[r * G + g] = s;
+      if (actB) sh[kb * G + gb] = s;
       __syncthreads();
-      if (r == 0) {
+      if (kb == 0) {
         double t = 0;
-        for (int rr = 0; rr < N1; rr++) t += sh[rr * G + g];
-        cg.part[(long long)o * pstride + plane] = t;
+        for (int rr = 0; rr < N1; rr++) t += sh[rr * G + gb];
+        cg.part[(long long)o * pstride + planeB] = t;
       }
     }
   } else {
     // forward N2-point DFT, real product, inverse N2-point DFT, twiddle, exchange
-    dft_reg<T, N2, -1>(b);
     const T *mrow = mid + (long long)o * n;
     T mv[N2];
+    if (actB) {
+      dft_reg<T, N2, -1>(b);
 #pragma unroll
-    for (int k2 = 0; k2 < N2; k2++) mv[k2] = mrow[r + N1 * k2];
+      for (int k2 = 0; k2 < N2; k2++) mv[k2] = mrow[kb + N1 * k2];
 #pragma unroll
-    for (int k2 = 0; k2 < N2; k2++) b[k2] = mv[k2] * b[k2];
+      for (int k2 = 0; k2 < N2; k2++) b[k2] = mv[k2] * b[k2];
+    }
     if constexpr (KIND == TK_COL2) {
       // the second input's column through the same forward steps, times
       // mid2, subtracted
@@ -510,30 +641,63 @@ k_toep(const cx<T> *in, cx<T> *out, long long in_so, long long in_sk, long long 
         const cx<T> *src = cg.in2 + (long long)o * in_so + plane;
 #pragma unroll
         for (int j1 = 0; j1 < H1; j1++) a[j1] = src[(long long)(r + N2 * j1) * in_sk];
-        dft_half_in<T, N1, -1>(a);
+        if constexpr (H == 1) {
+          dft_half_in<T, N1, -1>(a);
+        } else {
+          if (h) {
 #pragma unroll
-        for (int k1 = 0; k1 < N1; k1++) a[k1] = a[k1] * stw[r * k1];
+            for (int j = 1; j < A; j++) a[j] = a[j] * wconst<T, -1, N1>(j);
+          }
+          dft_reg<T, A, -1>(a);
+        }
+#pragma unroll
+        for (int s = 0; s < A; s++) a[s] = a[s] * stw[r * k1of(s)];
       }
       cx<T> b2[N2];
-      xchg<T, N1, N2, SPLIT>(lx, a, actA, r * rowA + g, G, b2, true, r * G + g, rowA);
-      dft_reg<T, N2, -1>(b2);
-      const T *mrow2 = cg.mid2 + (long long)o * n;
+      if constexpr (H == 1)
+        xchg<T, N1, N2, SPLIT>(lx, a, actA, r * rowA + g, G, b2, true, r * G + g, rowA);
+      else
+        xchg_at<T, A, N2, SPLIT>(lx, a, actA, wA, b2, actB, rB);
+      if (actB) {
+        dft_reg<T, N2, -1>(b2);
+        const T *mrow2 = cg.mid2 + (long long)o * n;
 #pragma unroll
-      for (int k2 = 0; k2 < N2; k2++) mv[k2] = mrow2[r + N1 * k2];
+        for (int k2 = 0; k2 < N2; k2++) mv[k2] = mrow2[kb + N1 * k2];
 #pragma unroll
-      for (int k2 = 0; k2 < N2; k2++) b[k2] = b[k2] - mv[k2] * b2[k2];
+        for (int k2 = 0; k2 < N2; k2++) b[k2] = b[k2] - mv[k2] * b2[k2];
+      }
     }
-    dft_reg<T, N2, +1>(b);
+    if (actB) {
+      dft_reg<T, N2, +1>(b);
 #pragma unroll
-    for (int m2 = 0; m2 < N2; m2++) b[m2] = b[m2] * conjx(stw[r * m2]);
+      for (int m2 = 0; m2 < N2; m2++) b[m2] = b[m2] * conjx(stw[kb * m2]);
+    }
     __syncthreads();  // every thread has read the exchange buffer
-    // thread r = m2 gathers k1 = 0..N1-1; outputs r + N2 m1 < N for m1 < N1/2
-    xchg<T, N2, N1, SPLIT>(lx, b, true, r * G + g, rowA, a, actA, r * rowA + g, G);
-    if (actA) {
-      dft_half_out<T, N1, +1>(a);
-      cx<T> *dst = out + (long long)o * out_so + plane;
+    // thread r = m2 gathers k1 = 0..N1-1 (lane pairs: k1 of parity h);
+    // outputs r + N2 m1 < N for m1 < N1/2
+    if constexpr (H == 1) {
+      xchg<T, N2, N1, SPLIT>(lx, b, true, r * G + g, rowA, a, actA, r * rowA + g, G);
+      if (actA) {
+        dft_half_out<T, N1, +1>(a);
+        cx<T> *dst = out + (long long)o * out_so + plane;
 #pragma unroll
-      for (int m1 = 0; m1 < H1; m1++) dst[(long long)(r + N2 * m1) * out_sk] = a[m1];
+        for (int m1 = 0; m1 < H1; m1++) dst[(long long)(r + N2 * m1) * out_sk] = a[m1];
+      }
+    } else {
+      xchg_at<T, N2, A, SPLIT>(
+          lx, b, actB, [&](int m2) { return m2 * rowA + kb * G + gb; }, a, actA,
+          [&](int s) { return r * rowA + (2 * s + h) * G + g; });
+      if (actA) {
+        dft_reg<T, A, +1>(a);
+        if (h) {
+#pragma unroll
+          for (int m = 1; m < A; m++) a[m] = a[m] * wconst<T, +1, N1>(m);
+        }
+        pair_swap<T, Q4, false>(a, h);
+        cx<T> *dst = out + (long long)o * out_so + plane;
+#pragma unroll
+        for (int i = 0; i < Q4; i++) dst[(long long)(r + N2 * (Q4 * h + i)) * out_sk] = a[i];
+      }
     }
   }
 }

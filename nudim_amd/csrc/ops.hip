@@ -571,18 +571,18 @@ void ToeplitzOp<T>::apply_start(const cx<T> *a, const cx<T> *b, const cx<T> *add
     const double rowf_bytes = (double)(N + L) * N * pc * sizeof(cx<T>);
     {
       ProfScope ps(KC_TOEPLITZ_ROWF, st, rowf_bytes);
-      toep_launch<T>(TK_ROWF, L, N, a + p0, xa, NP, P, LC, pc, pc, st, tw, nullptr, c, P);
+      toep_launch<T>(TK_ROWF, L, N, a + p0, xa, NP, P, LC, pc, pc, st, tw, nullptr, c, P, pairs);
     }
     {
       ProfScope ps(KC_TOEPLITZ_ROWF, st, rowf_bytes);
-      toep_launch<T>(TK_ROWF, L, N, b + p0, xb, NP, P, LC, pc, pc, st, tw, nullptr, c, P);
+      toep_launch<T>(TK_ROWF, L, N, b + p0, xb, NP, P, LC, pc, pc, st, tw, nullptr, c, P, pairs);
     }
     {
       // X2 and X read, X written (N x L each), both kernels read
       ProfScope ps(KC_TOEPLITZ_COL2, st, 3.0 * N * L * pc * sizeof(cx<T>) + 2.0 * L * L * sizeof(T));
       c.in2 = xb;
       c.mid2 = spec->khat_re.template as<T>();
-      toep_launch<T>(TK_COL2, L, L, xa, xb, pc, LC, pc, LC, pc, st, tw, spec2->khat_re.template as<T>(), c, P);
+      toep_launch<T>(TK_COL2, L, L, xa, xb, pc, LC, pc, LC, pc, st, tw, spec2->khat_re.template as<T>(), c, P, pairs);
     }
     {
       // X read (N x L), add read and z written (N x N each)
@@ -590,7 +590,7 @@ void ToeplitzOp<T>::apply_start(const cx<T> *a, const cx<T> *b, const cx<T> *add
       ToepCg<T> ci{};
       ci.add = add + p0;
       if (zrows) ci.part = zrows + p0;
-      toep_launch<T>(TK_ROWI, L, N, xb, z + p0, LC, pc, NP, P, pc, st, tw, nullptr, ci, P);
+      toep_launch<T>(TK_ROWI, L, N, xb, z + p0, LC, pc, NP, P, pc, st, tw, nullptr, ci, P, pairs);
     }
   }
   NUDIM_HIP(hipGetLastError());
@@ -612,6 +612,8 @@ void ToeplitzOp<T>::init(int N_, const double *x, const double *w, int M, int P_
   const size_t row_cap = rb ? (size_t)strtoull(rb, nullptr, 10) : ((size_t)2 << 20);
   pc = P;
   while ((size_t)L * pc * sizeof(cx<T>) > row_cap && pc % 2 == 0 && pc / 2 >= 64) pc /= 2;
+  const char *tp = getenv("NUDIM_TOEP_PAIRS");
+  pairs = !(tp && std::string(tp) == "0");
 }
 
 template <typename T>
@@ -648,16 +650,16 @@ void ToeplitzOp<T>::apply_cg(const ToepCg<T> &cg, cx<T> *q, hipStream_t st) {
       // f read and written (N x N each); X written (N x L)
       const double nvec = 2.0 + (cg.zz_old ? 1.0 : 0.0) + (cg.alpha ? 2.0 : 0.0);
       ProfScope ps(KC_TOEPLITZ_ROWF, st, (nvec * N + L) * N * pc * sizeof(cx<T>));
-      toep_launch<T>(TK_ROWF, L, N, nullptr, x, NP, P, LC, pc, pc, st, tw, nullptr, c, P);
+      toep_launch<T>(TK_ROWF, L, N, nullptr, x, NP, P, LC, pc, pc, st, tw, nullptr, c, P, pairs);
     }
     {
       ProfScope ps(KC_TOEPLITZ_COL, st, 2.0 * N * L * pc * sizeof(cx<T>) + (double)L * L * sizeof(T));
-      toep_launch<T>(TK_COL, L, L, x, x, pc, LC, pc, LC, pc, st, tw, kr, c, P);
+      toep_launch<T>(TK_COL, L, L, x, x, pc, LC, pc, LC, pc, st, tw, kr, c, P, pairs);
     }
     {
       // algorithmic bytes: X read (N x L), p read, q written (N x N each)
       ProfScope ps(KC_TOEPLITZ_ROWI, st, (2.0 * N + L) * N * pc * sizeof(cx<T>));
-      toep_launch<T>(TK_ROWI, L, N, x, q + p0, LC, pc, NP, P, pc, st, tw, nullptr, c, P);
+      toep_launch<T>(TK_ROWI, L, N, x, q + p0, LC, pc, NP, P, pc, st, tw, nullptr, c, P, pairs);
     }
   }
   NUDIM_HIP(hipGetLastError());

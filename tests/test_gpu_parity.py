@@ -173,6 +173,35 @@ def test_cgnr_toeplitz_lengths(hip, orc, N):
         assert rel(got32[p], ref[p]) < 1e-4
 
 
+def test_toeplitz_lane_pairs_job(hip, monkeypatch):
+    """A whole job at np = 768 (n = 1536: the 48 x 32 passes, the projected
+    CG start's two-input column pass and row inverse with the added vector):
+    the passes with two lanes per 48-point line equal the single-lane passes
+    to FP32 rounding and the FP64 plain path to the FP32 bar (ER without TV,
+    random projections: the parity does not depend on the data's content)."""
+    np_, nz, nx = 768, 16, 768
+    rng = np.random.default_rng(5)
+    nmeas = 41
+    proj = rng.standard_normal((nmeas, nz, nx))
+
+    def run(prec):
+        p = hip.make_params(nx, nz, np_, -60, 60, 3, padangle=1.0, weight=1, nhio=0, ner=2, precision=prec)
+        with hip.Reconstruction(p) as job:
+            job.load_projections(proj)
+            job.initial()
+            r = job.iterate(2)
+            return r, job.volume()
+
+    r_pairs, v_pairs = run(hip.FP32)
+    monkeypatch.setenv("NUDIM_TOEP_PAIRS", "0")
+    r_one, v_one = run(hip.FP32)
+    monkeypatch.delenv("NUDIM_TOEP_PAIRS")
+    r64, v64 = run(hip.FP64_PLAIN)
+    assert rel(v_pairs, v_one) < 1e-5
+    assert rel(v_pairs, v64) < 1e-4
+    assert np.max(np.abs(r_pairs - r64)) < 1e-5
+
+
 def test_toeplitz_plane_chunks(hip, orc, monkeypatch):
     """The fused Toeplitz passes over plane chunks (long rows of the 2N x 2N
     intermediate, forced here by a small row budget) equal the single-chunk
