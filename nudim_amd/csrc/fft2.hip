@@ -46,10 +46,12 @@ void launch_toep_h(int kind, int nlines, const cx<T> *in, cx<T> *out, long long 
                    long long out_so, long long out_sk, int P, hipStream_t st, const cx<T> *tw, const T *mid,
                    const ToepCg<T> &cg, int pstride) {
   constexpr int n = N1 * N2, TL = (N1 > H * N2 ? N1 : H * N2);  // threads per plane
-  // the lane-pair row inverse (82 VGPRs) runs 512-thread blocks: 8 planes per
-  // block, 64-byte plane runs (631 -> 516 us at 512^3); the other passes
-  // lose more occupancy than they gain with them
-  const int tmax = (H == 2 && kind == TK_ROWI) ? 512 : 256;
+  // the row passes run 512-thread blocks: twice the planes per block, so
+  // twice the contiguous bytes per line position (C3: row forward 383 -> 371
+  // us with 107 KB of LDS, row inverse 242 -> 226 us; 512^3 lane pairs: 953
+  // -> 849 us and 631 -> 509 us); the column passes lose more occupancy
+  // than they gain
+  const int tmax = (kind == TK_ROWI || kind == TK_ROWF) ? 512 : 256;
   int cap = std::max(1, std::min(tmax, g_fft2_threads * tmax / 256) / TL);
   int Gs = 0;
   while ((2 << Gs) <= cap && P % (2 << Gs) == 0) Gs++;
@@ -57,7 +59,7 @@ void launch_toep_h(int kind, int nlines, const cx<T> *in, cx<T> *out, long long 
     return (size_t)N2 * (N1 * (1 << gs) + (1 << gs)) * (toep_split(kind) ? 1 : 2) * sizeof(T) +
            (size_t)n * sizeof(cx<T>);
   };
-  while (Gs > 0 && lds_of(Gs) > 64 * 1024) Gs--;  // a block's LDS within 64 KiB
+  while (Gs > 0 && lds_of(Gs) > 160 * 1024) Gs--;  // a block's LDS within 160 KiB
   const int G = 1 << Gs;
   const size_t lds = lds_of(Gs);
   const int nchunk = P >> Gs;

@@ -444,7 +444,7 @@ __device__ __forceinline__ void pair_swap(cx<T> *a, int h) {
 }
 
 template <typename T, int N1, int N2, int KIND, int H>
-__global__ void __launch_bounds__(H == 2 ? 512 : 256)
+__global__ void __launch_bounds__((H == 2 || KIND == TK_ROWF || KIND == TK_ROWI) ? 512 : 256)
 k_toep(const cx<T> *in, cx<T> *out, long long in_so, long long in_sk, long long out_so, long long out_sk,
        const cx<T> *__restrict__ tw, int Gs, int nchunk, const T *__restrict__ mid, ToepCg<T> cg, int pstride) {
   constexpr int n = N1 * N2, H1 = N1 / 2, H2 = N2 / 2;
