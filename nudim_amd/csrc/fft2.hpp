@@ -95,6 +95,19 @@ __device__ __forceinline__ cx<T> wconst(int e) {
   return mk(tab.c[i], tab.s[i]);
 }
 
+// v * w_R^e with the quarter-turn multiples as exact sign swaps (the same
+// values as the product with the table's exact 0 and 1); e is a constant
+// once the caller's loops unroll, so the choice is made while compiling
+template <typename T, int S, int R>
+__device__ __forceinline__ cx<T> mul_wconst(cx<T> v, int e) {
+  const int i = ((e % R) + R) % R;
+  if (i == 0) return v;
+  if (4 * i == R) return S > 0 ? mk(-v.y, v.x) : mk(v.y, -v.x);      // S i
+  if (2 * i == R) return mk(-v.x, -v.y);                             // -1
+  if (4 * i == 3 * R) return S > 0 ? mk(v.y, -v.x) : mk(-v.y, v.x);  // -S i
+  return v * wconst<T, S, R>(e);
+}
+
 constexpr int split_factor(int R) {
   return (R % 8 == 0 && R != 8) ? 8 : (R % 4 == 0 && R != 4) ? 4 : (R % 2 == 0 && R != 2) ? 2
        : (R % 3 == 0 && R != 3) ? 3 : (R % 5 == 0 && R != 5) ? 5 : R;
@@ -119,7 +132,7 @@ __device__ __forceinline__ void dft_reg(cx<T> *v) {
       for (int j1 = 0; j1 < A; j1++) a[j1] = v[j2 + B * j1];
       dft_reg<T, A, S>(a);
 #pragma unroll
-      for (int k1 = 0; k1 < A; k1++) t[j2 + B * k1] = (j2 * k1 == 0) ? a[k1] : a[k1] * wconst<T, S, R>(j2 * k1);
+      for (int k1 = 0; k1 < A; k1++) t[j2 + B * k1] = mul_wconst<T, S, R>(a[k1], j2 * k1);
     }
 #pragma unroll
     for (int k1 = 0; k1 < A; k1++) {
@@ -144,7 +157,7 @@ __device__ __forceinline__ void dft_half_in(cx<T> *v) {
 #pragma unroll
   for (int j = 0; j < H; j++) {
     e[j] = v[j];
-    o[j] = (j == 0) ? v[0] : v[j] * wconst<T, S, R>(j);
+    o[j] = mul_wconst<T, S, R>(v[j], j);
   }
   dft_reg<T, H, S>(e);
   dft_reg<T, H, S>(o);
@@ -169,7 +182,7 @@ __device__ __forceinline__ void dft_half_out(cx<T> *v) {
   dft_reg<T, H, S>(e);
   dft_reg<T, H, S>(o);
 #pragma unroll
-  for (int m = 0; m < H; m++) v[m] = e[m] + ((m == 0) ? o[0] : o[m] * wconst<T, S, R>(m));
+  for (int m = 0; m < H; m++) v[m] = e[m] + mul_wconst<T, S, R>(o[m], m);
 }
 
 // Exchange of the four-step transform through LDS.  SPLIT: through a real
@@ -443,10 +456,13 @@ __device__ __forceinline__ void pair_swap(cx<T> *a, int h) {
   for (int i = 0; i < (BOTH ? 2 * Q : Q); i++) a[i] = c[i];
 }
 
-template <typename T, int N1, int N2, int KIND, int H>
+// GSC >= 0: log2 G fixed at compile time (the launch's Gs argument is then
+// GSC), so that every LDS address of the exchanges is a constant offset
+template <typename T, int N1, int N2, int KIND, int H, int GSC>
 __global__ void __launch_bounds__((H == 2 || KIND == TK_ROWF || KIND == TK_ROWI) ? 512 : 256)
 k_toep(const cx<T> *in, cx<T> *out, long long in_so, long long in_sk, long long out_so, long long out_sk,
-       const cx<T> *__restrict__ tw, int Gs, int nchunk, const T *__restrict__ mid, ToepCg<T> cg, int pstride) {
+       const cx<T> *__restrict__ tw, int Gs_arg, int nchunk, const T *__restrict__ mid, ToepCg<T> cg, int pstride) {
+  const int Gs = GSC >= 0 ? GSC : Gs_arg;
   constexpr int n = N1 * N2, H1 = N1 / 2, H2 = N2 / 2;
   constexpr int A = N1 / H;   // step-A values per lane
   constexpr int Q4 = N1 / 4;  // values a pair swaps (H = 2)
@@ -494,7 +510,7 @@ k_toep(const cx<T> *in, cx<T> *out, long long in_so, long long in_sk, long long 
         dft_reg<T, A, +1>(a);
         if (h) {
 #pragma unroll
-          for (int k = 1; k < A; k++) a[k] = a[k] * wconst<T, +1, N1>(k);
+          for (int k = 1; k < A; k++) a[k] = mul_wconst<T, +1, N1>(a[k], k);
         }
         pair_swap<T, Q4, true>(a, h);
       }
@@ -551,7 +567,7 @@ k_toep(const cx<T> *in, cx<T> *out, long long in_so, long long in_sk, long long 
       } else {
         if (h) {
 #pragma unroll
-          for (int j = 1; j < A; j++) a[j] = a[j] * wconst<T, -1, N1>(j);
+          for (int j = 1; j < A; j++) a[j] = mul_wconst<T, -1, N1>(a[j], j);
         }
         dft_reg<T, A, -1>(a);
       }
@@ -646,7 +662,7 @@ k_toep(const cx<T> *in, cx<T> *out, long long in_so, long long in_sk, long long 
         } else {
           if (h) {
 #pragma unroll
-            for (int j = 1; j < A; j++) a[j] = a[j] * wconst<T, -1, N1>(j);
+            for (int j = 1; j < A; j++) a[j] = mul_wconst<T, -1, N1>(a[j], j);
           }
           dft_reg<T, A, -1>(a);
         }
@@ -691,7 +707,7 @@ k_toep(const cx<T> *in, cx<T> *out, long long in_so, long long in_sk, long long 
         dft_reg<T, A, +1>(a);
         if (h) {
 #pragma unroll
-          for (int m = 1; m < A; m++) a[m] = a[m] * wconst<T, +1, N1>(m);
+          for (int m = 1; m < A; m++) a[m] = mul_wconst<T, +1, N1>(a[m], m);
         }
         pair_swap<T, Q4, false>(a, h);
         cx<T> *dst = out + (long long)o * out_so + plane;
