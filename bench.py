@@ -51,7 +51,7 @@ def kc_names(np_, nz, n, W):
     gather = ("k_gather_chunk<float,%d> (NFFT adjoint)" % W if nz % 64 == 0
               else "k_gather_w<float,%d> (NFFT adjoint)" % W)
     return {1: toep(1, "Toeplitz column pass"), 2: "k_interp_rsums<float,%d> (R sums at the known nodes)" % W, 3: gather, 4: grid,
-            5: "k_tv_fused_q<float,4>" if nz % 4 == 0 else "k_tv_fused<float>", 6: zf,
+            5: "k_tv_march<float>" if nz % 4 == 0 else "k_tv_fused<float>", 6: zf,
             7: toep(0, "Toeplitz row forward + CG update"), 8: toep(2, "Toeplitz row inverse + <p,q>"),
             9: toep(3, "two-input column pass, CG start"), 10: "k_cg_zupdate<float>"}
 

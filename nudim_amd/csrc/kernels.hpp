@@ -888,6 +888,145 @@ k_tv_fused_q(const T *__restrict__ u, const T *__restrict__ g, T *__restrict__ u
   }
 }
 
+// k_tv_fused_q (runs of 4 z per lane) marching down y: a lane owns one run
+// (x, z0..z0+3) over a chunk of R rows of the band and keeps the updated
+// columns it will reuse in registers -- (y-1, x) and (y, x) from the steps
+// before, (y, x-1) and (y-1, x+1) loaded one step earlier -- so a step loads
+// only (y+1, x), (y+1, x-1) and (y, x+1): 3 columns instead of 7, and the
+// y+-1 rows are never re-read from L2.  Each voxel's arithmetic is that of
+// k_tv_fused.  Grid: (runs / kThreads, chunks); one partial sum per block.
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_tv_march(const T *__restrict__ u, const T *__restrict__ g, T *__restrict__ un, T *__restrict__ gn, int ny, int nx,
+           int nz, int ylo, int yhi, int R, const double *__restrict__ sums, double lam, double *__restrict__ part_g,
+           double *__restrict__ part_a) {
+  constexpr int Q = 4;
+  __shared__ double sg[kThreads], sa[kThreads];
+  const T eps = (T)1e-8;
+  const double s1d = sqrt(sums[0]);
+  const T s1 = (T)s1d, inv_s1 = (T)(1.0 / s1d);
+  const T c = (T)(lam * sqrt(sums[1]));
+  auto upd = [&](T v, T gg) -> T {
+    if constexpr (sizeof(T) == 4) return v - c * (gg * inv_s1);
+    else return v - c * (gg / s1);
+  };
+  using RQ = Run<T, Q>;
+  auto nxtq = [&](unsigned b, T(&r)[Q]) {
+    const RQ v = ld_at(reinterpret_cast<const RQ *>(u), b), gg = ld_at(reinterpret_cast<const RQ *>(g), b);
+#pragma unroll
+    for (int k = 0; k < Q; k++) r[k] = upd(v.v[k], gg.v[k]);
+  };
+  auto nxt1 = [&](unsigned b) -> T { return upd(ld_at(u, b), ld_at(g, b)); };
+  auto voxel = [&](T cv, T zmv, T zpv, T ymv, T ympv, T xmv, T xmpv, T ypv, T ypmv, T ypxmv, T xpv, T xpmv,
+                   T ymxpv) -> T {
+    T t1 = cv - zmv;
+    T t2 = cv - ymv;
+    T t3 = cv - xmv;
+    T uzp = zpv;
+    T t4 = uzp - cv;
+    T t5 = uzp - ympv;
+    T t6 = uzp - xmpv;
+    T uyp = ypv;
+    T t7 = uyp - ypmv;
+    T t8 = uyp - cv;
+    T t9 = uyp - ypxmv;
+    T uxp = xpv;
+    T t10 = uxp - xpmv;
+    T t11 = uxp - ymxpv;
+    T t12 = uxp - cv;
+    return tv_rs(t1 + t2 + t3, eps + t1 * t1 + t2 * t2 + t3 * t3) - tv_rs(t4, eps + t4 * t4 + t5 * t5 + t6 * t6) -
+           tv_rs(t8, eps + t7 * t7 + t8 * t8 + t9 * t9) - tv_rs(t12, eps + t10 * t10 + t11 * t11 + t12 * t12);
+  };
+  const int lane = threadIdx.x & 63;
+  double accg = 0, acca = 0;
+  const unsigned lb = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+  const unsigned bx = lb % gridDim.x;
+  const int y0 = ylo + (int)(lb / gridDim.x) * R, y1 = min(y0 + R, yhi + 1);
+  constexpr unsigned E = sizeof(T);
+  const unsigned hz = (unsigned)nz / Q, rl = (unsigned)nx * hz, sx = (unsigned)nz * E, sy = (unsigned)nx * sx;
+  const unsigned run = bx * kThreads + threadIdx.x;
+  if (run < rl) {
+    const unsigned x = run / hz, zr = run - x * hz;
+    const unsigned xo = x * sx + zr * Q * E;
+    const unsigned dxm = x > 0 ? sx : 0u, dxp = x + 1 < (unsigned)nx ? sx : 0u;
+    const bool hm = zr > 0, hp = zr + 1 < hz;
+    T cc[Q], ym[Q], xm[Q], ymxp[Q], yp[Q], ypxm[Q], xp[Q];
+    {
+      const unsigned o = (unsigned)y0 * sy + xo, om = (unsigned)(y0 > 0 ? y0 - 1 : 0) * sy + xo;
+      nxtq(o, cc);
+      nxtq(om, ym);
+      nxtq(o - dxm, xm);
+      nxtq(om + dxp, ymxp);
+    }
+    for (int y = y0; y < y1; y++) {
+      const unsigned o = (unsigned)y * sy + xo;
+      const unsigned om = (unsigned)(y > 0 ? y - 1 : 0) * sy + xo, op = (unsigned)(y < ny - 1 ? y + 1 : y) * sy + xo;
+      nxtq(op, yp);
+      nxtq(op - dxm, ypxm);
+      nxtq(o + dxp, xp);
+      T c_m = __shfl_up(cc[Q - 1], 1), yp_m = __shfl_up(yp[Q - 1], 1), xp_m = __shfl_up(xp[Q - 1], 1);
+      T c_p = __shfl_down(cc[0], 1), ym_p = __shfl_down(ym[0], 1), xm_p = __shfl_down(xm[0], 1);
+      if (hm && lane == 0) {
+        c_m = nxt1(o - E);
+        yp_m = nxt1(op - E);
+        xp_m = nxt1(o + dxp - E);
+      }
+      if (hp && lane == 63) {
+        c_p = nxt1(o + Q * E);
+        ym_p = nxt1(om + Q * E);
+        xm_p = nxt1(o - dxm + Q * E);
+      }
+      if (!hm) {
+        c_m = cc[0];
+        yp_m = yp[0];
+        xp_m = xp[0];
+      }
+      if (!hp) {
+        c_p = cc[Q - 1];
+        ym_p = ym[Q - 1];
+        xm_p = xm[Q - 1];
+      }
+      RQ gq, cq;
+#pragma unroll
+      for (int k = 0; k < Q; k++) {
+        gq.v[k] = voxel(cc[k], k > 0 ? cc[k - 1] : c_m, k < Q - 1 ? cc[k + 1] : c_p, ym[k],
+                        k < Q - 1 ? ym[k + 1] : ym_p, xm[k], k < Q - 1 ? xm[k + 1] : xm_p, yp[k],
+                        k > 0 ? yp[k - 1] : yp_m, ypxm[k], xp[k], k > 0 ? xp[k - 1] : xp_m, ymxp[k]);
+        cq.v[k] = cc[k];
+      }
+      st_at(reinterpret_cast<RQ *>(un), o, cq);
+      st_at(reinterpret_cast<RQ *>(gn), o, gq);
+#pragma unroll
+      for (int k = 0; k < Q; k++) {
+        accg += (double)gq.v[k] * (double)gq.v[k];
+        if (cc[k] < (T)0) acca += (double)cc[k] * (double)cc[k];
+      }
+      // the next row's columns
+#pragma unroll
+      for (int k = 0; k < Q; k++) {
+        ym[k] = cc[k];
+        cc[k] = yp[k];
+        xm[k] = ypxm[k];
+        ymxp[k] = xp[k];
+      }
+    }
+  }
+  sg[threadIdx.x] = accg;
+  sa[threadIdx.x] = acca;
+  __syncthreads();
+  for (int h = kThreads / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < h) {
+      sg[threadIdx.x] += sg[threadIdx.x + h];
+      sa[threadIdx.x] += sa[threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part_g[lb] = sg[0];
+    part_a[lb] = sa[0];
+  }
+}
+
 template <typename T>
 __global__ void k_copy_real(T *__restrict__ d, const T *__restrict__ s, long long total) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;

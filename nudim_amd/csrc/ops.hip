@@ -878,7 +878,13 @@ T *tv_sweeps(T *u0, T *u1, T *g0, T *g1, int ny, int nx, int nz, int ylo, int yh
   // 512^3 in FP32: 583 against 534 us a sweep, at 3 waves per SIMD.)
   const bool quads = nz % 4 == 0;
   const int gx4 = quads ? (int)std::min<long long>(cdiv(rowlen / 4, kThreads), 128) : 0;
-  NUDIM_CHECK(np_ <= nb && nrows * gx4 <= nb, "TV partial-sum buffer too small");
+  // the same sweeps marching down y over chunks of R rows (k_tv_march)
+  static const int march_rows = getenv("NUDIM_TV_ROWS") ? atoi(getenv("NUDIM_TV_ROWS")) : 8;
+  const bool march = quads && march_rows > 0;
+  const int R = std::max(1, march_rows);
+  const dim3 mgrid(march ? (unsigned)cdiv(rowlen / 4, kThreads) : 1u, (unsigned)cdiv(nrows, R));
+  const int npm = (int)(mgrid.x * mgrid.y);
+  NUDIM_CHECK(np_ <= nb && nrows * gx4 <= nb && npm <= nb, "TV partial-sum buffer too small");
   NUDIM_CHECK((double)V * sizeof(T) < 4294967296.0, "TV kernels address volumes of fewer than 2^32 bytes");
   const dim3 grid(gx, nrows);
   // the sweeps write band rows only and read one row on each side of the
@@ -898,14 +904,18 @@ T *tv_sweeps(T *u0, T *u1, T *g0, T *g1, int ny, int nx, int nz, int ylo, int yh
   for (int s = 0; s + 1 < niter; s++) {
     {
       ProfScope ps(KC_TV, st, 2.0 * vb);
-      if (quads)
+      if (march)
+        hipLaunchKernelGGL(k_tv_march<T>, mgrid, dim3(kThreads), 0, st, uc, gc, un, gn, ny, nx, nz, ylo, yhi, R, sums,
+                           lamd, part_g, part_a);
+      else if (quads)
         hipLaunchKernelGGL((k_tv_fused_q<T, 4>), dim3(gx4, nrows), dim3(kThreads), 0, st, uc, gc, un, gn, ny, nx, nz,
                            ylo, yhi, sums, lamd, part_g, part_a);
       else
         hipLaunchKernelGGL(k_tv_fused<T>, grid, dim3(kThreads), 0, st, uc, gc, un, gn, ny, nx, nz, ylo, yhi, sums,
                            lamd, part_g, part_a);
     }
-    hipLaunchKernelGGL(k_sum_scalars, dim3(1), dim3(256), 0, st, part_g, part_a, quads ? nrows * gx4 : np_, sums);
+    hipLaunchKernelGGL(k_sum_scalars, dim3(1), dim3(256), 0, st, part_g, part_a,
+                       march ? npm : (quads ? nrows * gx4 : np_), sums);
     std::swap(uc, un);
     std::swap(gc, gn);
   }
