@@ -459,7 +459,7 @@ __device__ __forceinline__ void pair_swap(cx<T> *a, int h) {
 // GSC >= 0: log2 G fixed at compile time (the launch's Gs argument is then
 // GSC), so that every LDS address of the exchanges is a constant offset
 template <typename T, int N1, int N2, int KIND, int H, int GSC>
-__global__ void __launch_bounds__((H == 2 || KIND == TK_ROWF || KIND == TK_ROWI) ? 512 : 256)
+__global__ void __launch_bounds__((KIND == TK_ROWF || KIND == TK_ROWI) ? 512 : 256)
 k_toep(const cx<T> *in, cx<T> *out, long long in_so, long long in_sk, long long out_so, long long out_sk,
        const cx<T> *__restrict__ tw, int Gs_arg, int nchunk, const T *__restrict__ mid, ToepCg<T> cg, int pstride) {
   const int Gs = GSC >= 0 ? GSC : Gs_arg;
