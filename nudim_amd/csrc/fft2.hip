@@ -7,6 +7,12 @@ namespace nudim {
 // threads per block of the fixed-length kernels (a tuning knob: larger
 // blocks give longer contiguous plane runs, smaller ones more blocks per CU)
 int g_fft2_threads = getenv("NUDIM_FFT2_THREADS") ? atoi(getenv("NUDIM_FFT2_THREADS")) : 256;
+// lane pairs in the NFFT grid FFT's 48-point stage (NUDIM_TOEP_PAIRS=0: off,
+// read at each launch like the Toeplitz passes' switch at each job)
+static bool toep_pairs_on() {
+  const char *e = getenv("NUDIM_TOEP_PAIRS");
+  return !(e && std::string(e) == "0");
+}
 
 namespace {
 
@@ -15,27 +21,40 @@ bool launch_shape(const cx<T> *in, cx<T> *out, int nlines, const LineIO &li, con
                   hipStream_t st, const cx<T> *tw, const cx<T> *mid, long long mid_so) {
   constexpr int n = N1 * N2;
   if (mid) return false;  // the pointwise-product form runs on the Toeplitz passes
-  int cap = std::max(1, std::min(256, g_fft2_threads) / N1);
-  int Gs = 0;
-  while ((2 << Gs) <= cap && P % (2 << Gs) == 0) Gs++;
-  const int G = 1 << Gs;
-  size_t lds = ((size_t)N2 * (N1 * G + G) + n) * sizeof(cx<T>);
-  if (lds > 160 * 1024) return false;
-  const int nchunk = P >> Gs;
-  dim3 grid((unsigned)((long long)nlines * nchunk)), block(N1 * G);
   // n/2 elements centred on position 0: the NFFT's N = n/2 modes
   const bool hin = li.cnt * 2 == n && li.shift == -n / 4;
   const bool hout = !hin && lo.cnt * 2 == n && lo.shift == -n / 4;
-#define NUDIM_FFT2(SG, HI, HO) \
-  hipLaunchKernelGGL((k_fft2<T, N1, N2, SG, HI, HO>), grid, block, lds, st, in, out, li, lo, tw, Gs, nchunk)
+  // the half-input 48-point stage runs on lane pairs in 512-thread blocks
+  // (as the Toeplitz row passes of n = 1536)
+  const bool pairs = N1 == 48 && hin && toep_pairs_on();
+  const int TL = pairs ? std::max(N1, 2 * N2) : N1, tmax = pairs ? 512 : 256;
+  int cap = std::max(1, std::min(tmax, g_fft2_threads * tmax / 256) / TL);
+  int Gs = 0;
+  while ((2 << Gs) <= cap && P % (2 << Gs) == 0) Gs++;
+  auto lds_of = [&](int gs) { return ((size_t)N2 * (N1 * (1 << gs) + (1 << gs)) + n) * sizeof(cx<T>); };
+  while (Gs > 0 && lds_of(Gs) > 160 * 1024) Gs--;
+  const size_t lds = lds_of(Gs);
+  if (lds > 160 * 1024) return false;
+  const int G = 1 << Gs;
+  const int nchunk = P >> Gs;
+  dim3 grid((unsigned)((long long)nlines * nchunk)), block(TL * G);
+#define NUDIM_FFT2(SG, HI, HO, HH) \
+  hipLaunchKernelGGL((k_fft2<T, N1, N2, SG, HI, HO, HH>), grid, block, lds, st, in, out, li, lo, tw, Gs, nchunk)
+  if constexpr (N1 == 48) {
+    if (pairs) {
+      if (sign < 0) NUDIM_FFT2(-1, true, false, 2);
+      else NUDIM_FFT2(+1, true, false, 2);
+      return true;
+    }
+  }
   if (sign < 0) {
-    if (hin) NUDIM_FFT2(-1, true, false);
-    else if (hout) NUDIM_FFT2(-1, false, true);
-    else NUDIM_FFT2(-1, false, false);
+    if (hin) NUDIM_FFT2(-1, true, false, 1);
+    else if (hout) NUDIM_FFT2(-1, false, true, 1);
+    else NUDIM_FFT2(-1, false, false, 1);
   } else {
-    if (hin) NUDIM_FFT2(+1, true, false);
-    else if (hout) NUDIM_FFT2(+1, false, true);
-    else NUDIM_FFT2(+1, false, false);
+    if (hin) NUDIM_FFT2(+1, true, false, 1);
+    else if (hout) NUDIM_FFT2(+1, false, true, 1);
+    else NUDIM_FFT2(+1, false, false, 1);
   }
 #undef NUDIM_FFT2
   return true;

@@ -288,12 +288,18 @@ __device__ __forceinline__ cx<T> mul_quarter(cx<T> v, int q) {
 // HOUT: only the n/2 outputs centred on position 0 are stored, i.e. k2 in
 // [3N2/4, N2) and [0, N2/4) of step B: Y[m' - N2/4] is output m' < N2/2 of
 // the DFT of v[j] exp(-S 2 pi i j/4).
-template <typename T, int N1, int N2, int S, bool HIN, bool HOUT>
-__global__ void __launch_bounds__(256)
+//
+// H = 2 (HIN only): two adjacent lanes share line position r = j2 and split
+// the half-input N1-point DFT by output parity, as in the Toeplitz passes:
+// lane h computes outputs k1 = 2s + h; blocks of max(N1, 2 N2) * G threads.
+template <typename T, int N1, int N2, int S, bool HIN, bool HOUT, int H>
+__global__ void __launch_bounds__(H == 2 ? 512 : 256)
 k_fft2(const cx<T> *in, cx<T> *out, LineIO li, LineIO lo, const cx<T> *__restrict__ tw, int Gs, int nchunk) {
   constexpr int n = N1 * N2, Q1 = N1 / 4, Q2 = N2 / 4;
+  constexpr int A = N1 / H;  // step-A values per lane
   static_assert(!HIN || N1 % 4 == 0, "centred input needs N1 % 4 == 0");
   static_assert(!HOUT || N2 % 4 == 0, "centred output needs N2 % 4 == 0");
+  static_assert(H == 1 || (H == 2 && HIN), "lane pairs for the half-input form only");
   extern __shared__ unsigned char smem_raw[];
   cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
   const int G = 1 << Gs, G1 = G - 1;
@@ -305,13 +311,17 @@ k_fft2(const cx<T> *in, cx<T> *out, LineIO li, LineIO lo, const cx<T> *__restric
   const int o = lb / nchunk, chunk = lb - o * nchunk;
   const int tid = threadIdx.x;
   for (int i = tid; i < n; i += blockDim.x) stw[i] = tw[i];
-  const int g = tid & G1, r = tid >> Gs;  // r = j2 in step A, k1 in step B
-  const bool actA = r < N2, actB = r < N1;
+  // step A: lane h of line position r = j2, plane g; step B: k1 = kb, plane gb
+  const int h = H == 2 ? (tid & 1) : 0;
+  const int g = (tid >> (H - 1)) & G1, r = tid >> (Gs + H - 1);
+  const int gb = tid & G1, kb = tid >> Gs;
+  const bool actA = r < N2, actB = kb < N1;
+  auto k1of = [&](int s) { return H == 1 ? s : 2 * s + h; };
   const cx<T> *src = in + (long long)o * li.so + (long long)chunk * G + g;
   const T so_scale = li.scale_o ? (T)li.scale_o[o] : (T)1;
 
   // ---- step A: load positions j2 + N2 j1, N1-point DFT ----
-  cx<T> a[N1];
+  cx<T> a[A];
   if (actA) {
     if constexpr (HIN) {
       // every loaded position is in range: all N1/2 loads in flight together
@@ -331,9 +341,17 @@ k_fft2(const cx<T> *in, cx<T> *out, LineIO li, LineIO lo, const cx<T> *__restric
 #pragma unroll
         for (int jp = 0; jp < N1 / 2; jp++) a[jp] = so_scale * a[jp];
       }
-      dft_half_in<T, N1, S>(a);
+      if constexpr (H == 1) {
+        dft_half_in<T, N1, S>(a);
+      } else {
+        if (h) {
 #pragma unroll
-      for (int k = 0; k < N1; k++) a[k] = mul_quarter<T, S>(a[k], k);
+          for (int j = 1; j < A; j++) a[j] = mul_wconst<T, S, N1>(a[j], j);
+        }
+        dft_reg<T, A, S>(a);
+      }
+#pragma unroll
+      for (int s = 0; s < A; s++) a[s] = mul_quarter<T, S>(a[s], k1of(s));
     } else {
       // Loads are unconditional (index clamped into range, out-of-range lanes
       // zeroed afterwards) so that all N1 of them are in flight together.
@@ -361,20 +379,21 @@ k_fft2(const cx<T> *in, cx<T> *out, LineIO li, LineIO lo, const cx<T> *__restric
   __syncthreads();  // twiddle table complete
   if (actA) {
 #pragma unroll
-    for (int k1 = 0; k1 < N1; k1++) {
+    for (int s = 0; s < A; s++) {
+      const int k1 = k1of(s);
       cx<T> w = stw[r * k1];
       if (S > 0) w = conjx(w);
-      lds[r * rowA + k1 * G + g] = a[k1] * w;
+      lds[r * rowA + k1 * G + g] = a[s] * w;
     }
   }
   __syncthreads();
 
-  // ---- step B: N2-point DFT over j2 for fixed k1 ----
+  // ---- step B: N2-point DFT over j2 for fixed k1 = kb ----
   if (!actB) return;
   cx<T> b[N2];
 #pragma unroll
-  for (int j2 = 0; j2 < N2; j2++) b[j2] = lds[j2 * rowA + r * G + g];
-  cx<T> *dst = out + (long long)o * lo.so + (long long)chunk * G + g;
+  for (int j2 = 0; j2 < N2; j2++) b[j2] = lds[j2 * rowA + kb * G + gb];
+  cx<T> *dst = out + (long long)o * lo.so + (long long)chunk * G + gb;
   const T oo_scale = lo.scale_o ? (T)lo.scale_o[o] : (T)1;
   constexpr int NO = HOUT ? N2 / 2 : N2;
   int kk[NO];
@@ -384,11 +403,11 @@ k_fft2(const cx<T> *in, cx<T> *out, LineIO li, LineIO lo, const cx<T> *__restric
     dft_half_out<T, N2, S>(b);
 #pragma unroll
     for (int mp = 0; mp < NO; mp++)
-      kk[mp] = (mp < Q2) ? r + N1 * (mp + 3 * Q2) + n / 4 - n : r + N1 * (mp - Q2) + n / 4;
+      kk[mp] = (mp < Q2) ? kb + N1 * (mp + 3 * Q2) + n / 4 - n : kb + N1 * (mp - Q2) + n / 4;
   } else {
     dft_reg<T, N2, S>(b);
 #pragma unroll
-    for (int k2 = 0; k2 < NO; k2++) kk[k2] = wrap_pos(r + N1 * k2 - lo.shift, n);
+    for (int k2 = 0; k2 < NO; k2++) kk[k2] = wrap_pos(kb + N1 * k2 - lo.shift, n);
   }
   if (lo.scale_k) {
     T sc[NO];
