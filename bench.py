@@ -182,6 +182,61 @@ def traffic_from_profiles(kclass):
     return None
 
 
+def cli_leg(args):
+    """The drop-in `reconstruction` binary on the 256^3 workload's files:
+    71 projection files of 384 x 256 values (%le) written into a scratch
+    ./files, then the reference's command line (nhio 30 + ner 50, TV height
+    128 centre 192) in FP32, with its own phase times (NUDIM_CLI_TIMES) at
+    16 text-I/O threads and at 1.  The outputs are recovered-0.txt (%le, full
+    384 x 384 x 256 grid) and recovered.txt (%f, the central 384^2 x 256
+    crop): 37.7M lines each."""
+    import shutil
+    import subprocess
+    import tempfile
+    _, proj = phantom.config_workload("c3")
+    exe = os.path.join(ROOT, "nudim_amd", "bin", "reconstruction")
+    flags = ["--prefix=prj", "--pad=384", "--nx=384", "--nz=256", "--max=70", "--min=-70", "-t", "2",
+             "--padangle=1", "--weight", "--nhio=30", "--ner=50", "--denoise", "--method=TV", "--height=128",
+             "--center=192"]
+    d = tempfile.mkdtemp(prefix="nudim_cli_")
+    try:
+        os.makedirs(os.path.join(d, "files"))
+        t0 = time.perf_counter()
+        for j in range(proj.shape[0]):
+            v = proj[j].ravel()
+            with open(os.path.join(d, "files", "prj-%d.txt" % j), "w") as fh:
+                fh.write(("%le\n" * v.size) % tuple(v))
+        gen_s = time.perf_counter() - t0
+        runs = {}
+        for threads in (16, 1):
+            env = dict(os.environ, NUDIM_PRECISION="fp32", NUDIM_CLI_TIMES="1", NUDIM_IO_THREADS=str(threads))
+            t0 = time.perf_counter()
+            r = subprocess.run([exe] + flags, cwd=d, env=env, capture_output=True, text=True, timeout=900)
+            wall = time.perf_counter() - t0
+            if r.returncode != 0:
+                raise RuntimeError("reconstruction failed: %s" % r.stderr[-500:])
+            line = [ln for ln in r.stderr.splitlines() if ln.startswith("nudim-cli-times")][-1].split()
+            ph = {line[i]: float(line[i + 1]) for i in range(1, len(line) - 1, 2) if line[i] != "io-threads"}
+            ph["wall"] = wall
+            runs["io_threads_%d" % threads] = ph
+            log("  cli, %d I/O threads: %s" % (threads, " ".join("%s %.2f s" % kv for kv in ph.items())))
+            for f in ("recovered-0.txt", "recovered.txt"):
+                os.remove(os.path.join(d, f))
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    # the simulator (projection_sim -m 2D1D) at the same sizes: a 256^3
+    # density (random: the cost does not depend on the values) to 71 tilts at
+    # np = 384, FP64 as the reference
+    dens = np.random.default_rng(0).random((256, 256, 256))
+    nd.project_2d1d(dens, 384, -70, 70, 2)  # first call: device set-up
+    t0 = time.perf_counter()
+    nd.project_2d1d(dens, 384, -70, 70, 2)
+    sim_s = time.perf_counter() - t0
+    log("  projection_sim 2D1D, 256^3 -> 71 x 256 x 384 (FP64): %.3f s" % sim_s)
+    return {"files": "71 x 98304 values (%le)", "generated_s": gen_s, "precision": "f32", "runs": runs,
+            "projection_sim_2d1d_256cube_71tilts_s": sim_s}
+
+
 def probe(prm, proj, copy_mib):
     """One FP32 pass and one device-to-device hipMemcpy of copy_mib MiB (the
     runtime's blit kernel, plain loads and stores: the scale for the
@@ -353,6 +408,7 @@ def main():
     ap.add_argument("--serve-fp64", action="store_true", help="also serve the trace on the plain FP64 path")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving leg")
     ap.add_argument("--batch", action="store_true", help="run only the uneven-batch leg (48 volumes)")
+    ap.add_argument("--cli", action="store_true", help="run only the drop-in CLI on the 256^3 workload's files")
     ap.add_argument("--batch-jobs", type=int, default=48)
     ap.add_argument("--batch-slots", type=int, default=8)
     ap.add_argument("--probe", type=int, default=0, metavar="MIB",
@@ -370,6 +426,9 @@ def main():
         return
     if args.batch:
         print(json.dumps({"batch": batch(args)}), flush=True)
+        return
+    if args.cli:
+        print(json.dumps({"cli": cli_leg(args)}), flush=True)
         return
     if args.serve_only:
         out = {"serving": serve(args, nd.FP32)}

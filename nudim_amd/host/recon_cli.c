@@ -73,6 +73,12 @@ static void show_help(const char *s, int parallel) {
                         "    --center=<ncenter>.       Sample center for constraint (in pixel).\n");
 }
 
+static double now_s(void) {
+  struct timespec t;
+  clock_gettime(CLOCK_MONOTONIC, &t);
+  return (double)t.tv_sec + 1e-9 * (double)t.tv_nsec;
+}
+
 static int fail_lib(const char *what) {
   fprintf(stderr, "%s: %s\n", what, nudim_last_error());
   return 1;
@@ -250,8 +256,12 @@ int nudim_recon_main(int argc, char *argv[], int parallel) {
   const char *dev = getenv("NUDIM_DEVICE");
   prm.device = dev ? atoi(dev) : 0;
 
+  /* phase times on stderr with NUDIM_CLI_TIMES set (stdout as the reference's) */
+  double tp[7];
+  tp[0] = now_s();
   nudim_ctx *ctx = NULL;
   if (nudim_create(&prm, &ctx) != 0) return fail_lib("nudim_create");
+  tp[1] = now_s();
   int nmeas = 0, M = 0, nang = 0;
   nudim_sizes(ctx, &nmeas, &M, &nang);
 
@@ -274,10 +284,12 @@ int nudim_recon_main(int argc, char *argv[], int parallel) {
     }
   }
   free(prefix);
+  tp[2] = now_s();
 
   if (nudim_load_projections(ctx, proj) != 0) return fail_lib("nudim_load_projections");
   free(proj);
   if (nudim_initial(ctx) != 0) return fail_lib("nudim_initial");
+  tp[3] = now_s();
 
   size_t V = (size_t)np * np * nz;
   double *img = (double *)malloc(sizeof(double) * V);
@@ -288,6 +300,7 @@ int nudim_recon_main(int argc, char *argv[], int parallel) {
     fprintf(stderr, "cannot write %s\n", f0name);
     return 1;
   }
+  tp[4] = now_s();
 
   /* density before / after the denoiser at pass 100 (serial, :637-658), or
    * its gradient magnitude at pass 0 (parallel, :770-791) */
@@ -322,6 +335,7 @@ int nudim_recon_main(int argc, char *argv[], int parallel) {
     }
   }
 
+  tp[5] = now_s();
   if (nudim_get_volume(ctx, img) != 0) return fail_lib("nudim_get_volume");
   /* central nx x nx crop of every plane, in the reference's loop order */
   size_t nc = 0;
@@ -336,6 +350,12 @@ int nudim_recon_main(int argc, char *argv[], int parallel) {
     fprintf(stderr, "cannot write %s\n", fname);
     return 1;
   }
+  tp[6] = now_s();
+  if (getenv("NUDIM_CLI_TIMES"))
+    fprintf(stderr,
+            "nudim-cli-times create %.3f read %.3f load+initial %.3f write0 %.3f iterate %.3f write %.3f "
+            "io-threads %d\n",
+            tp[1] - tp[0], tp[2] - tp[1], tp[3] - tp[2], tp[4] - tp[3], tp[5] - tp[4], tp[6] - tp[5], io_threads);
 
   free(img);
   free(before);
