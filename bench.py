@@ -237,6 +237,37 @@ def cli_leg(args):
             "projection_sim_2d1d_256cube_71tilts_s": sim_s}
 
 
+def device_copy_gbs(mib=1024, reps=5):
+    """Bandwidth of the runtime's device-to-device copy (hipMemcpy: its blit
+    kernel, plain loads and stores) on this GPU now: bytes read + written
+    over the best of reps copies of mib MiB -- the practical ceiling for a
+    streaming kernel, beside the datasheet peak."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipMemset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
+    nbytes = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), nbytes) or hip.hipMalloc(ctypes.byref(b), nbytes):
+        raise RuntimeError("hipMalloc failed")
+    try:
+        if hip.hipMemset(a, 1, nbytes) or hip.hipDeviceSynchronize():
+            raise RuntimeError("hipMemset failed")
+        best = None
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            if hip.hipMemcpy(b, a, nbytes, 3) or hip.hipDeviceSynchronize():  # device to device
+                raise RuntimeError("hipMemcpy failed")
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+    return 2.0 * nbytes / best / 1e9
+
+
 def probe(prm, proj, copy_mib):
     """One FP32 pass and one device-to-device hipMemcpy of copy_mib MiB (the
     runtime's blit kernel, plain loads and stores: the scale for the
@@ -523,6 +554,12 @@ def main():
             % (npass, e, acc["er_no_tv"]["max_abs_rfactor_drift"]))
 
     serving = serve(args, nd.FP32) if args.serve_jobs > 0 else None
+    try:
+        copy_gbs = device_copy_gbs()
+        log("  device copy (hipMemcpy, 1 GiB): %.0f GB/s read + write" % copy_gbs)
+    except Exception as e:  # reported context, never required
+        copy_gbs = None
+        log("device copy measurement skipped: %s" % e)
 
     cpu = None
     if not args.no_cpu:
@@ -553,7 +590,8 @@ def main():
                      "unit": "GB/s", "frac": kd["achieved_GBs"] / HBM_PEAK_GBS,
                      "traffic": traffic_from_profiles(dom) if args.config == "c3" else None,
                      "algorithmic_bytes": kd["bytes_per_launch"],
-                     "avg_us": kd["avg_us"]},
+                     "avg_us": kd["avg_us"], "device_copy_GBs": copy_gbs,
+                     "frac_of_device_copy": (kd["achieved_GBs"] / copy_gbs) if copy_gbs else None},
         "kernels": {KC_NAMES.get(k): {kk: round(vv, 4) for kk, vv in v.items() if kk != "name"}
                     for k, v in kern.items()},
         "serving": serving,
