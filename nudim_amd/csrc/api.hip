@@ -89,6 +89,17 @@ __global__ void k_assemble(const cx<double> *__restrict__ tomo, const int *__res
   meas[i] = v;
 }
 
+// R sums from the quadratic form (reconstruction.c:574-582 over the known
+// nodes K): sum_K |A f - m|^2 = f^H T_K f - 2 Re <f, A_K^H m> + sum_K |m|^2,
+// and the same with + for |A f + m|^2
+__global__ void k_rform_final(const double *__restrict__ q, const double *__restrict__ x,
+                              const double *__restrict__ m2, double *__restrict__ d1, double *__restrict__ d2, int P) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  d1[p] = q[p] - 2.0 * x[p] + m2[p];
+  d2[p] = q[p] + 2.0 * x[p] + m2[p];
+}
+
 // R = sqrt(sum_p d1 / sum_p d2) (reconstruction.c:603-604); one block of 256
 // threads, fixed-order sums
 __global__ void k_rfactor_final(const double *__restrict__ d1, const double *__restrict__ d2, int P,
@@ -273,6 +284,12 @@ struct Job : JobBase {
   DevBuf twz;
   bool proj_start = false;  // projected_start(): b_K in bk
   DevBuf bk;
+  // R from the quadratic form (with proj_start unless NUDIM_R_FORM=nfft):
+  // ck = A_K^H m (unweighted), m2[p] = sum_K |m|^2, the column partials of
+  // f^H T_K f from the CG start's two-input column pass, their sums and
+  // Re <f, ck> per plane
+  bool rform = false;
+  DevBuf ck, m2, qpart, qsum, xr;
   DevBuf dknown;  // with proj_start: the known nodes (R needs A fh only there)
   int nknown = 0;
   static constexpr int kRBlocks = 1024;  // node strides of the fused R interpolation
@@ -317,6 +334,19 @@ struct Job : JobBase {
         dknown.alloc(kn.size() * sizeof(int));
         NUDIM_HIP(hipMemcpy(dknown.p, kn.data(), kn.size() * sizeof(int), hipMemcpyHostToDevice));
       }
+      const char *rf = getenv("NUDIM_R_FORM");
+      rform = !(rf && std::string(rf) == "nfft");
+      if (rform) {
+        // T_K: the kernel of unit weights on the known nodes
+        std::vector<double> wk(M);
+        for (int m = 0; m < M; m++) wk[m] = g.kind[m] == 1 ? 1.0 : 0.0;
+        toep.init_rform(g.x.data(), wk.data(), M, st);
+        ck.alloc(NN * sizeof(cx<T>));
+        m2.alloc(P * sizeof(double));
+        qpart.alloc((size_t)2 * N * P * sizeof(double));
+        qsum.alloc(P * sizeof(double));
+        xr.alloc(P * sizeof(double));
+      }
     }
     dkind.alloc(M * sizeof(int));
     NUDIM_HIP(hipMemcpy(dkind.p, g.kind.data(), M * sizeof(int), hipMemcpyHostToDevice));
@@ -354,7 +384,8 @@ struct Job : JobBase {
     NUDIM_HIP(hipMemsetAsync(img, 0, V * sizeof(T), st));
   }
 
-  // b_K = A^H W K m (the known nodes' measured values, weighted)
+  // b_K = A^H W K m (the known nodes' measured values, weighted); with
+  // rform also ck = A^H K m and m2 = sum_K |m|^2 per plane
   void known_adjoint() {
     NUDIM_CHECK(proj_start && bk.p, "b_K requested without the projected CG start");
     cx<T> *t = cg.t.template as<cx<T>>();
@@ -362,11 +393,21 @@ struct Job : JobBase {
                        dw.as<double>(), t, MM, P);
     NUDIM_HIP(hipGetLastError());
     op.adjoint(t, bk.as<cx<T>>(), st);
+    if (rform) {
+      hipLaunchKernelGGL(k_known_weighted<T>, dim3(cdiv(MM, 256)), dim3(256), 0, st, meas.as<cx<T>>(),
+                         dkind.as<int>(), (const double *)nullptr, t, MM, P);
+      NUDIM_HIP(hipGetLastError());
+      op.adjoint(t, ck.as<cx<T>>(), st);
+      // |0 - m|^2 over the known nodes
+      cx<T> *zero = cg.v.template as<cx<T>>();
+      hipLaunchKernelGGL(k_zero_cx<T>, dim3(cdiv(MM, 256)), dim3(256), 0, st, zero, MM);
+      cg.red.template rfactor<T>(zero, meas.as<cx<T>>(), dkind.as<int>(), M, m2.as<double>(), qsum.as<double>(), st);
+    }
   }
 
   size_t bytes() const override {
     return op.bytes() + toep.bytes() + cg.bytes() + dw.bytes + dkind.bytes + dorder.bytes + meas.bytes + y.bytes +
-           bk.bytes + dknown.bytes + rpart.bytes +
+           bk.bytes + dknown.bytes + rpart.bytes + ck.bytes + m2.bytes + qpart.bytes + qsum.bytes + xr.bytes +
            Fh.bytes + F0.bytes + img_a.bytes + img_b.bytes + img_c.bytes + grad.bytes + grad2.bytes + d1.bytes +
            d2.bytes + tvpart_g.bytes +
            tvpart_a.bytes + tvsums.bytes + twz.bytes;
@@ -480,7 +521,11 @@ struct Job : JobBase {
     std::swap(img, img_last);  // img_last = img (reconstruction.c:551)
     zfft_forward(img_last, Fh.as<cx<T>>(), F0.as<cx<T>>());
     cx<T> *v = cg.v.template as<cx<T>>();
-    if (proj_start) {
+    if (proj_start && rform) {
+      // R from the quadratic form: Re <fh, A_K^H m> here, fh^H T_K fh from
+      // the CG start's column pass below
+      cg.red.template redot<T>(Fh.as<cx<T>>(), ck.as<cx<T>>(), (long long)N * N, xr.as<double>(), st);
+    } else if (proj_start) {
       // R needs A fh only at the known nodes: the interpolation sums the R
       // terms itself (no A fh stored)
       double *p1 = rpart.as<double>(), *p2 = p1 + (size_t)kRBlocks * P;
@@ -493,12 +538,22 @@ struct Job : JobBase {
       hipLaunchKernelGGL(k_projector<T>, dim3(cdiv(MM, 256)), dim3(256), 0, st, v, meas.as<cx<T>>(),
                          dkind.as<int>(), y.as<cx<T>>(), MM, P);
     }
-    hipLaunchKernelGGL(k_rfactor_final, dim3(1), dim3(256), 0, st, d1.as<double>(), d2.as<double>(), P, rslot);
-    NUDIM_HIP(hipGetLastError());
+    if (!rform) {
+      hipLaunchKernelGGL(k_rfactor_final, dim3(1), dim3(256), 0, st, d1.as<double>(), d2.as<double>(), P, rslot);
+      NUDIM_HIP(hipGetLastError());
+    }
     if (proj_start) {
       // z = b_K + T_U Fh - T f0, then the CG steps
       toep.apply_start(Fh.as<cx<T>>(), F0.as<cx<T>>(), bk.as<cx<T>>(), cg.z.template as<cx<T>>(),
-                       cg.rowpart.template as<double>(), st);
+                       cg.rowpart.template as<double>(), st, rform ? qpart.as<double>() : nullptr);
+      if (rform) {
+        hipLaunchKernelGGL(k_sum_partials, dim3(cdiv(P, 8)), dim3(256), 0, st, qpart.as<double>(), 2 * N, P,
+                           qsum.as<double>());
+        hipLaunchKernelGGL(k_rform_final, dim3(cdiv(P, 256)), dim3(256), 0, st, qsum.as<double>(), xr.as<double>(),
+                           m2.as<double>(), d1.as<double>(), d2.as<double>(), P);
+        hipLaunchKernelGGL(k_rfactor_final, dim3(1), dim3(256), 0, st, d1.as<double>(), d2.as<double>(), P, rslot);
+        NUDIM_HIP(hipGetLastError());
+      }
       // the last step's f += alpha p happens in the z transform's loads
       // (f itself is not needed afterwards: the next pass rebuilds f0)
       cg.solve_from_residual(F0.as<cx<T>>(), 10, st, true, true);
@@ -813,8 +868,9 @@ extern "C" int nudim_estimate_bytes(const nudim_params *p, size_t *bytes) {
   b += (size_t)N * n * P * cb + (size_t)n * n * P * cb;           // NFFT X, grid
   b += (size_t)M * W * W * (4 + rb) + (size_t)n * n * 4 + M * W * 16;  // gather maps, window tables
   if (form == NUDIM_CG_TOEPLITZ) b += (size_t)4 * N * N * (cb + rb) + (size_t)N * 2 * N * P * cb + NN * cb;
-  if (projected_start(form, (int)N))
-    b += (size_t)4 * N * N * rb + (size_t)N * 2 * N * P * cb + NN * cb + (size_t)2 * 1024 * P * 8 + M * 4;
+  if (projected_start(form, (int)N))  // T_U, X2, b_K, R partials, known list; c_K and the quadratic form's partials
+    b += (size_t)4 * N * N * rb + (size_t)N * 2 * N * P * cb + NN * cb + (size_t)2 * 1024 * P * 8 + M * 4 +
+         NN * cb + (size_t)2 * N * P * 8;
   b += 2 * NN * cb + 3 * MM * cb;                                   // CG z, p; r, v, t
   b += 2 * MM * cb + 2 * NN * cb;                                   // meas, y, Fh, F0
   b += 2 * NN * rb + ((p->denoise && p->method == NUDIM_TV) ? 3 * NN * rb : 0);  // volumes, TV gradients

@@ -128,8 +128,14 @@ struct ToeplitzOp {
   // the kernel of the weights w2[M] (host) as a second operator T2
   void init_second(const double *x, const double *w2, int M, hipStream_t st);
   // z = add + T2 a - T b on the fused passes (fused() and init_second done);
-  // zrows (may be null): [N][P] row sums of |z|^2
-  void apply_start(const cx<T> *a, const cx<T> *b, const cx<T> *add, cx<T> *z, double *zrows, hipStream_t st);
+  // zrows (may be null): [N][P] row sums of |z|^2; qpart (may be null, needs
+  // init_rform): [L][P] column sums of a^H T3 a (the third kernel's quadratic
+  // form of a, by Parseval over the column pass's spectrum of a)
+  void apply_start(const cx<T> *a, const cx<T> *b, const cx<T> *add, cx<T> *z, double *zrows, hipStream_t st,
+                   double *qpart = nullptr);
+  // the kernel of the weights w3[M] (host) as a third operator T3
+  std::shared_ptr<const ToepSpectrum<T>> spec3;
+  void init_rform(const double *x, const double *w3, int M, hipStream_t st);
   void apply(const cx<T> *p, cx<T> *q, hipStream_t st);
   // fused CGNR step on the fixed-length passes (when fused()): p from z and
   // the old p (see ToepCg), q = T p, part[N][P] = row sums of Re <p, q>

@@ -201,7 +201,10 @@ enum { TK_ROWF = 0, TK_COL = 1, TK_ROWI = 2, TK_COL2 = 3 };
 //                With z null the pass transforms its input as it is.
 //   row inverse: q = the cropped inverse (+ add[] when add is non-null);
 //                with p non-null part[row][plane] = sum Re conj(p) q
-//   two-input column pass: IFFT(mid FFT(in) - mid2 FFT(in2))
+//   two-input column pass: IFFT(mid FFT(in) - mid2 FFT(in2)); with mid3
+//                and qpart also qpart[column][plane] = sum over the column
+//                of mid3 |FFT(in)|^2 (the quadratic form of a third
+//                circulant: in^H C3 in by Parseval)
 template <typename T>
 struct ToepCg {
   const cx<T> *z;
@@ -212,6 +215,8 @@ struct ToepCg {
   const cx<T> *in2;
   const T *mid2;
   const cx<T> *add;
+  const T *mid3;
+  double *qpart;
 };
 
 // Batched line FFT over a plane-innermost array.  Block b (renumbered so

@@ -661,8 +661,18 @@ k_toep(const cx<T> *in, cx<T> *out, long long in_so, long long in_sk, long long 
     // forward N2-point DFT, real product, inverse N2-point DFT, twiddle, exchange
     const T *mrow = mid + (long long)o * n;
     T mv[N2];
+    double qf = 0;  // TK_COL2 with mid3: this thread's part of sum mid3 |FFT(in)|^2
     if (actB) {
       dft_reg<T, N2, -1>(b);
+      if constexpr (KIND == TK_COL2) {
+        if (cg.mid3) {
+          const T *m3 = cg.mid3 + (long long)o * n;
+#pragma unroll
+          for (int k2 = 0; k2 < N2; k2++) mv[k2] = m3[kb + N1 * k2];
+#pragma unroll
+          for (int k2 = 0; k2 < N2; k2++) qf += (double)mv[k2] * ((double)b[k2].x * b[k2].x + (double)b[k2].y * b[k2].y);
+        }
+      }
 #pragma unroll
       for (int k2 = 0; k2 < N2; k2++) mv[k2] = mrow[kb + N1 * k2];
 #pragma unroll
@@ -732,6 +742,20 @@ k_toep(const cx<T> *in, cx<T> *out, long long in_so, long long in_sk, long long 
         cx<T> *dst = out + (long long)o * out_so + plane;
 #pragma unroll
         for (int i = 0; i < Q4; i++) dst[(long long)(r + N2 * (Q4 * h + i)) * out_sk] = a[i];
+      }
+    }
+    if constexpr (KIND == TK_COL2) {
+      if (cg.qpart) {
+        // fixed-order sum over the N1 step-B threads of each plane
+        __syncthreads();  // exchange buffer free
+        double *sh = reinterpret_cast<double *>(smem_raw);
+        if (actB) sh[kb * G + gb] = qf;
+        __syncthreads();
+        if (kb == 0) {
+          double t = 0;
+          for (int rr = 0; rr < N1; rr++) t += sh[rr * G + gb];
+          cg.qpart[(long long)o * pstride + planeB] = t;
+        }
       }
     }
   }

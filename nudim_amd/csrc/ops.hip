@@ -559,8 +559,13 @@ void ToeplitzOp<T>::init_second(const double *x, const double *w2, int M, hipStr
 }
 
 template <typename T>
+void ToeplitzOp<T>::init_rform(const double *x, const double *w3, int M, hipStream_t st) {
+  spec3 = toeplitz_spectrum<T>(N, x, w3, M, st);
+}
+
+template <typename T>
 void ToeplitzOp<T>::apply_start(const cx<T> *a, const cx<T> *b, const cx<T> *add, cx<T> *z, double *zrows,
-                                 hipStream_t st) {
+                                 hipStream_t st, double *qpart) {
   NUDIM_CHECK(fused() && spec2, "apply_start needs the fused passes and a second kernel");
   const long long NP = (long long)N * P, LC = (long long)L * pc;
   const cx<T> *tw = fft.tw.template as<cx<T>>();
@@ -582,6 +587,11 @@ void ToeplitzOp<T>::apply_start(const cx<T> *a, const cx<T> *b, const cx<T> *add
       ProfScope ps(KC_TOEPLITZ_COL2, st, 3.0 * N * L * pc * sizeof(cx<T>) + 2.0 * L * L * sizeof(T));
       c.in2 = xb;
       c.mid2 = spec->khat_re.template as<T>();
+      if (qpart) {
+        NUDIM_CHECK(spec3, "the quadratic form needs the third kernel (init_rform)");
+        c.mid3 = spec3->khat_re.template as<T>();
+        c.qpart = qpart + p0;
+      }
       toep_launch<T>(TK_COL2, L, L, xa, xb, pc, LC, pc, LC, pc, st, tw, spec2->khat_re.template as<T>(), c, P, pairs);
     }
     {
