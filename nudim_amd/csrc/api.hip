@@ -241,6 +241,22 @@ static NfftTuning tuned_nfft(const nudim_params *p, const Geometry &g, hipStream
   return t;
 }
 
+// a measurement already made for the job's node set (false: none)
+static bool cached_nfft(const nudim_params *p, NfftTuning *t) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune.find(tune_key(p));
+  if (it == g_tune.end()) return false;
+  *t = it->second;
+  return true;
+}
+
+// R from the known-node quadratic form (NUDIM_R_FORM=nfft: through the
+// forward NFFT); see Job::rform
+static bool rform_on() {
+  const char *e = getenv("NUDIM_R_FORM");
+  return !(e && std::string(e) == "nfft");
+}
+
 static void resolve_knobs(const nudim_params *p, int *n, int *W, int *form) {
   if (p->precision == NUDIM_FP64_PLAIN) {
     *n = p->nfft_n > 0 ? p->nfft_n : 2 * next_pow2(p->np);
@@ -303,7 +319,12 @@ struct Job : JobBase {
     M = g.M;
     resolve_knobs(&p, &n, &W, &form);
     if (autotune_enabled(&p)) {
-      const NfftTuning t = tuned_nfft(&p, g, s);
+      // With the projected CG start and R from the quadratic form the
+      // passes run no NFFT (only the load and the initial INFFT do): a
+      // measurement made for this node set is used, none is started.
+      NfftTuning t{n, W, -1.0, -1.0};
+      if (!(projected_start(form, p.np) && rform_on())) t = tuned_nfft(&p, g, s);
+      else cached_nfft(&p, &t);
       n = t.n;
       W = t.W;
     }
@@ -334,8 +355,7 @@ struct Job : JobBase {
         dknown.alloc(kn.size() * sizeof(int));
         NUDIM_HIP(hipMemcpy(dknown.p, kn.data(), kn.size() * sizeof(int), hipMemcpyHostToDevice));
       }
-      const char *rf = getenv("NUDIM_R_FORM");
-      rform = !(rf && std::string(rf) == "nfft");
+      rform = rform_on();
       if (rform) {
         // T_K: the kernel of unit weights on the known nodes
         std::vector<double> wk(M);
