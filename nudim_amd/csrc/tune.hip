@@ -2,8 +2,9 @@
 //
 // The reference fixes its NFFT at n = 2 nextpow2(np) with 14 Kaiser-Bessel
 // taps in FP64 (src/reconstruction.c:393-405).  The FP32 form needs far less:
-// its NFFT feeds the R sums, the known-node adjoint b_K and the initial
-// INFFT, where a relative error near FP32 rounding is enough.  For a job's
+// its NFFT feeds the known-node adjoints at load, the initial INFFT and,
+// when R is taken through the NFFT, the R sums, where a relative error near
+// FP32 rounding is enough.  For a job's
 // node set this measures, on the device, candidate (n, W) pairs:
 //   - n among the fixed-length FFT sizes from N + 16 up to the default
 //     nice_fft_size(2N) (so a job never holds more than the default sizing
