@@ -257,6 +257,20 @@ static bool rform_on() {
   return !(e && std::string(e) == "nfft");
 }
 
+// Whether the projected-start solve keeps its ten search directions apart
+// (bytes: their size) and the z transform applies f + sum_k alpha_k p_k,
+// instead of f being read and written in nine row passes.  Up to 4 GiB of
+// directions the z transform reads its eleven streams near the copy rate;
+// at 24 GB (np = 768, nz = 512) they drop to about a third of it and the
+// pass is slower than with the per-step update.  NUDIM_KEEP_P=0 / 1 turns
+// it off / on whatever the size.
+static bool keep_on(size_t bytes) {
+  const char *e = getenv("NUDIM_KEEP_P");
+  if (e && std::string(e) == "0") return false;
+  if (e && std::string(e) == "1") return true;
+  return bytes <= ((size_t)4 << 30);
+}
+
 static void resolve_knobs(const nudim_params *p, int *n, int *W, int *form) {
   if (p->precision == NUDIM_FP64_PLAIN) {
     *n = p->nfft_n > 0 ? p->nfft_n : 2 * next_pow2(p->np);
@@ -340,6 +354,7 @@ struct Job : JobBase {
     cg.init(N, M, P, form, &op, form == NUDIM_CG_TOEPLITZ ? &toep : nullptr, dw.as<double>());
     proj_start = projected_start(form, N);
     if (proj_start) {
+      if (keep_on((size_t)10 * NN * sizeof(cx<T>))) cg.keep_directions(10);
       // kernel of T_U: the weights of the nodes the projector does not
       // replace by measured values (kind != 1)
       std::vector<double> wu(M);
@@ -462,18 +477,22 @@ struct Job : JobBase {
 
   // out = z IFFT of g (+ alpha p per plane when pv is non-null: the CG's last
   // f += alpha p, folded into the loads)
-  void zfft_inverse(const cx<T> *g, T *out, const cx<T> *pv = nullptr, const double *alpha = nullptr) {
+  // the z IFFT of g + sum_{s < nk} alpha_s p_s (p_s at pv + s N^2 P, alpha_s
+  // per plane at alpha + s P)
+  void zfft_inverse(const cx<T> *g, T *out, const cx<T> *pv = nullptr, const double *alpha = nullptr, int nk = 1) {
     int Gs = line_chunk_log2(P, 1 << 30, (int)sizeof(cx<T>), FftEpt<T>::value);
     long long nlines = (long long)N * N;
     size_t lds = (size_t)P * ((1 << Gs) + 1) * sizeof(cx<T>);
-    ProfScope ps(KC_ZFFT, st, (double)V * sizeof(cx<T>) * (pv ? 2.0 : 1.0) + (double)V * sizeof(T));
-    if (zfft2_launch<T>(P, true, nlines, nullptr, g, pv, alpha, nullptr, nullptr, out, twz.as<cx<T>>(), st)) {
+    if (!pv) nk = 0;
+    ProfScope ps(KC_ZFFT, st, (double)V * sizeof(cx<T>) * (1.0 + nk) + (double)V * sizeof(T));
+    if (zfft2_launch<T>(P, true, nlines, nullptr, g, pv, alpha, nullptr, nullptr, out, twz.as<cx<T>>(), st, nk,
+                        NN)) {
       NUDIM_HIP(hipGetLastError());
       return;
     }
-    if (pv)
-      hipLaunchKernelGGL(k_axpy_planes<T>, dim3(cdiv(NN, 256)), dim3(256), 0, st, const_cast<cx<T> *>(g), pv, alpha,
-                         1.0, NN, P);
+    for (int s = 0; s < nk; s++)
+      hipLaunchKernelGGL(k_axpy_planes<T>, dim3(cdiv(NN, 256)), dim3(256), 0, st, const_cast<cx<T> *>(g),
+                         pv + (long long)s * NN, alpha + (long long)s * P, 1.0, NN, P);
     hipLaunchKernelGGL(k_zfft_inv<T>, dim3(cdiv(nlines, 1 << Gs)), dim3(kFftThreads), lds, st, g, out, fdz,
                        twz.as<cx<T>>(), Gs, nlines);
     NUDIM_HIP(hipGetLastError());
@@ -574,10 +593,14 @@ struct Job : JobBase {
         hipLaunchKernelGGL(k_rfactor_final, dim3(1), dim3(256), 0, st, d1.as<double>(), d2.as<double>(), P, rslot);
         NUDIM_HIP(hipGetLastError());
       }
-      // the last step's f += alpha p happens in the z transform's loads
-      // (f itself is not needed afterwards: the next pass rebuilds f0)
+      // f += sum_k alpha_k p_k happens in the z transform's loads (f itself
+      // is not needed afterwards: the next pass rebuilds f0); without kept
+      // directions only the last step's update is left there
       cg.solve_from_residual(F0.as<cx<T>>(), 10, st, true, true);
-      zfft_inverse(F0.as<cx<T>>(), img, cg.pv.template as<cx<T>>(), cg.alpha.template as<double>());
+      if (cg.kept >= 10)
+        zfft_inverse(F0.as<cx<T>>(), img, cg.pk.template as<cx<T>>(), cg.alk.template as<double>(), 10);
+      else
+        zfft_inverse(F0.as<cx<T>>(), img, cg.pv.template as<cx<T>>(), cg.alpha.template as<double>());
     } else {
       cg.solve(y.as<cx<T>>(), F0.as<cx<T>>(), 10, false, st);
       zfft_inverse(F0.as<cx<T>>(), img);
@@ -891,6 +914,7 @@ extern "C" int nudim_estimate_bytes(const nudim_params *p, size_t *bytes) {
   if (projected_start(form, (int)N))  // T_U, X2, b_K, R partials, known list; c_K and the quadratic form's partials
     b += (size_t)4 * N * N * rb + (size_t)N * 2 * N * P * cb + NN * cb + (size_t)2 * 1024 * P * 8 + M * 4 +
          NN * cb + (size_t)2 * N * P * 8;
+  if (projected_start(form, (int)N) && keep_on(10 * NN * cb)) b += 10 * NN * cb + (size_t)10 * P * 8;  // kept directions
   b += 2 * NN * cb + 3 * MM * cb;                                   // CG z, p; r, v, t
   b += 2 * MM * cb + 2 * NN * cb;                                   // meas, y, Fh, F0
   b += 2 * NN * rb + ((p->denoise && p->method == NUDIM_TV) ? 3 * NN * rb : 0);  // volumes, TV gradients

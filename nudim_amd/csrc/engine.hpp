@@ -29,10 +29,12 @@ void toep_launch(int kind, int n, int nlines, const cx<T> *in, cx<T> *out, long 
                  const ToepCg<T> &cg, int pstride, bool pairs);
 
 // z-axis transforms of [line][z] volumes on the fixed-length kernels
-// (k_zfft2, fft2.hpp); false when n has no specialisation
+// (k_zfft2, fft2.hpp); false when n has no specialisation.  The inverse
+// adds sum_{s < nk} alpha[s n + z] pv[s pstride + i] to its input.
 template <typename T>
 bool zfft2_launch(int n, bool inv, long long nlines, const T *img_in, const cx<T> *g_in, const cx<T> *pv,
-                  const double *alpha, cx<T> *fh, cx<T> *f0, T *img_out, const cx<T> *tw, hipStream_t st);
+                  const double *alpha, cx<T> *fh, cx<T> *f0, T *img_out, const cx<T> *tw, hipStream_t st,
+                  int nk = 0, long long pstride = 0);
 
 // FP32 NFFT sizing for a node set (tune.hip): the fastest measured (n, W)
 // whose trafo and adjoint stay within 2e-6 relative L2 of the FP64 NFFT at
@@ -177,14 +179,22 @@ struct Cgnr {
   DevBuf z, pv, q, r, v, t;   // z, p, q: [N^2][P]; r, v, t: [M][P]
   DevBuf dot_z, coef, sc;     // [P] each
   DevBuf zz2, alpha, rowpart; // fused Toeplitz steps: [P], [P], [N][P]
+  // kept directions: p_k of every step [kept][N^2][P] and alpha_k [kept][P]
+  DevBuf pk, alk;
+  int kept = 0;
   Reducer red;
   size_t bytes() const;
   void init(int N, int M, int P, int form, NfftOp<T> *op, ToeplitzOp<T> *toep, const double *w);
+  // keep the p_k and alpha_k of up to K steps apart, so that a deferred solve
+  // leaves f untouched and the caller applies f + sum_k alpha_k p_k once
+  void keep_directions(int K);
   // solver_before_loop_complex + nsteps loop_one_step (fhat in/out; f0_zero skips A*0)
   void solve(const cx<T> *y, cx<T> *fhat, int nsteps, bool f0_zero, hipStream_t st);
   // the same on the fused Toeplitz passes with the starting residual given
   // in z (z = A^H W (y - A fhat)).  With defer_last the last step's
-  // fhat += alpha p is left to the caller (p in pv, alpha per plane in alpha).
+  // fhat += alpha p is left to the caller (p in pv, alpha per plane in alpha);
+  // with kept directions (nsteps <= kept) every step's update is: fhat is not
+  // touched, p_k is in pk + k N^2 P and alpha_k in alk + k P.
   // With z_rows, rowpart holds the [N][P] row sums of |z|^2 (apply_start).
   void solve_from_residual(cx<T> *fhat, int nsteps, hipStream_t st, bool defer_last = false, bool z_rows = false);
 };

@@ -217,6 +217,7 @@ struct ToepCg {
   const cx<T> *add;
   const T *mid3;
   double *qpart;
+  const cx<T> *pin;  // p_{k-1} when the solve keeps each step's p apart (p_k written to p)
 };
 
 // Batched line FFT over a plane-innermost array.  Block b (renumbered so

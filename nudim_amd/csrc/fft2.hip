@@ -167,16 +167,17 @@ template void toep_launch<double>(int, int, int, const cx<double> *, cx<double> 
 // fixed-length kernel exists (the caller then uses the generic kernels)
 template <typename T>
 bool zfft2_launch(int n, bool inv, long long nlines, const T *img_in, const cx<T> *g_in, const cx<T> *pv,
-                  const double *alpha, cx<T> *fh, cx<T> *f0, T *img_out, const cx<T> *tw, hipStream_t st) {
+                  const double *alpha, cx<T> *fh, cx<T> *f0, T *img_out, const cx<T> *tw, hipStream_t st, int nk,
+                  long long pstride) {
 #define NUDIM_ZFFT(NN, A, B)                                                                                     \
   case NN: {                                                                                                     \
     const dim3 grid((unsigned)((nlines + 256 / A - 1) / (256 / A)));                                            \
     if (inv)                                                                                                     \
       hipLaunchKernelGGL((k_zfft2<T, A, B, true>), grid, dim3(256), 0, st, img_in, g_in, pv, alpha, fh, f0,      \
-                         img_out, tw, nlines);                                                                   \
+                         img_out, tw, nlines, nk, pstride);                                                      \
     else                                                                                                         \
       hipLaunchKernelGGL((k_zfft2<T, A, B, false>), grid, dim3(256), 0, st, img_in, g_in, pv, alpha, fh, f0,     \
-                         img_out, tw, nlines);                                                                   \
+                         img_out, tw, nlines, nk, pstride);                                                      \
     return true;                                                                                                 \
   }
   if (!g_use_fft2) return false;
@@ -191,10 +192,11 @@ bool zfft2_launch(int n, bool inv, long long nlines, const T *img_in, const cx<T
 }
 
 template bool zfft2_launch<float>(int, bool, long long, const float *, const cx<float> *, const cx<float> *,
-                                  const double *, cx<float> *, cx<float> *, float *, const cx<float> *, hipStream_t);
+                                  const double *, cx<float> *, cx<float> *, float *, const cx<float> *, hipStream_t,
+                                  int, long long);
 template bool zfft2_launch<double>(int, bool, long long, const double *, const cx<double> *, const cx<double> *,
                                    const double *, cx<double> *, cx<double> *, double *, const cx<double> *,
-                                   hipStream_t);
+                                   hipStream_t, int, long long);
 
 // Returns false when no fixed-length kernel exists for n (the caller then
 // uses the generic Stockham kernel).

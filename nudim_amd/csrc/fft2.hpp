@@ -547,8 +547,9 @@ k_toep(const cx<T> *in, cx<T> *out, long long in_so, long long in_sk, long long 
         for (int i = 0; i < NV; i++) zv[i] = cg.z[pos(i)];
         if (cg.zz_old) {
           cx<T> pv[NV];
+          const cx<T> *pold = cg.pin ? cg.pin : cg.p;
 #pragma unroll
-          for (int i = 0; i < NV; i++) pv[i] = cg.p[pos(i)];
+          for (int i = 0; i < NV; i++) pv[i] = pold[pos(i)];
           if (cg.alpha) {
             const T al = (T)cg.alpha[plane];
             cx<T> fv[NV];
@@ -770,14 +771,15 @@ k_toep(const cx<T> *in, cx<T> *out, long long in_so, long long in_sk, long long 
 // r + N2 ((j1 + N1/2) mod N1), S^-1(k1 + N1 k2) = k1 + N1 ((k2 + N2/2) mod N2).
 //   forward (INV false): d[z] = img[S(z)];  D = DFT_-(d);  fh[p] = D[S(p)];
 //                        f0[z] = d[z] (when f0 is non-null)
-//   inverse (INV true):  d[k] = g[S(k)] (+ alpha[S(k)] pv[S(k)] when pv is
-//                        non-null);  e = DFT_+(d);  img[z] = Re e[S(z)] / n
+//   inverse (INV true):  d[k] = g[S(k)] + sum_{s < nk} alpha[s n + S(k)]
+//                        pv[s pstride + S(k)] (nk = 0 when pv is null);
+//                        e = DFT_+(d);  img[z] = Re e[S(z)] / n
 // tw holds exp(-2 pi i e / n), e < n.
 template <typename T, int N1, int N2, bool INV>
 __global__ void __launch_bounds__(256)
 k_zfft2(const T *__restrict__ img_in, const cx<T> *__restrict__ g_in, const cx<T> *__restrict__ pv,
         const double *__restrict__ alpha, cx<T> *__restrict__ fh, cx<T> *__restrict__ f0, T *__restrict__ img_out,
-        const cx<T> *__restrict__ tw, long long nlines) {
+        const cx<T> *__restrict__ tw, long long nlines, int nk, long long pstride) {
   constexpr int n = N1 * N2, G = 256 / N1, RS = G * N2 + 1;
   constexpr int S = INV ? +1 : -1;
   static_assert(256 % N1 == 0 && N1 >= N2 && N2 % 2 == 0 && N1 % 2 == 0, "lane layout");
@@ -802,11 +804,15 @@ k_zfft2(const T *__restrict__ img_in, const cx<T> *__restrict__ g_in, const cx<T
       }
     }
     if constexpr (INV) {
-      if (pv) {
+      // f + sum_s alpha_s p_s, the steps in order
+#pragma unroll 1
+      for (int s = 0; s < nk; s++) {
+        const cx<T> *ps = pv + s * pstride + base;
+        const double *as = alpha + s * n;
 #pragma unroll
         for (int j1 = 0; j1 < N1; j1++) {
           const int zs = r + N2 * ((j1 + N1 / 2) % N1);
-          a[j1] = a[j1] + (T)alpha[zs] * pv[base + zs];
+          a[j1] = a[j1] + (T)as[zs] * ps[zs];
         }
       }
     } else {

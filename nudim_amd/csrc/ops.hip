@@ -650,6 +650,7 @@ void ToeplitzOp<T>::apply_cg(const ToepCg<T> &cg, cx<T> *q, hipStream_t st) {
     ToepCg<T> c = cg;
     c.z += p0;
     c.p += p0;
+    if (c.pin) c.pin += p0;
     if (c.f) c.f += p0;
     c.zz += p0;
     if (c.zz_old) c.zz_old += p0;
@@ -724,7 +725,15 @@ void Reducer::rfactor(const cx<T> *f, const cx<T> *meas, const int *kind, long l
 template <typename T>
 size_t Cgnr<T>::bytes() const {
   return z.bytes + pv.bytes + q.bytes + r.bytes + v.bytes + t.bytes + dot_z.bytes + coef.bytes + sc.bytes +
-         zz2.bytes + alpha.bytes + rowpart.bytes + red.part.bytes + red.part2.bytes;
+         zz2.bytes + alpha.bytes + rowpart.bytes + pk.bytes + alk.bytes + red.part.bytes + red.part2.bytes;
+}
+
+template <typename T>
+void Cgnr<T>::keep_directions(int K) {
+  NUDIM_CHECK(form == NUDIM_CG_TOEPLITZ && toep->fused() && K > 0, "kept directions need the fused Toeplitz steps");
+  pk.alloc((size_t)K * N * N * P * sizeof(cx<T>));
+  alk.alloc((size_t)K * P * sizeof(double));
+  kept = K;
 }
 
 template <typename T>
@@ -765,15 +774,25 @@ void Cgnr<T>::solve_from_residual(cx<T> *fhat, int nsteps, hipStream_t st, bool 
   hipLaunchKernelGGL(k_cg_scalars, dim3(cdiv(P, 64)), dim3(64), 0, st, dz, cf, s, P, 0);
   NUDIM_HIP(hipGetLastError());
   // step k: row pass forms p_k = z_k + beta_k p_{k-1} (p_0 = z_0) and
-  // applies the previous step's f += alpha p_{k-1}; q = T p_k with the
-  // row sums of <p_k, q>; alpha_k = |z_k|^2 / <p_k, q>; z -= alpha_k q;
-  // |z_{k+1}|^2.  The last step's f update follows the loop.
+  // applies the previous step's f += alpha p_{k-1} (kept directions: writes
+  // p_k to its own buffer instead); q = T p_k with the row sums of
+  // <p_k, q>; alpha_k = |z_k|^2 / <p_k, q>; z -= alpha_k q; |z_{k+1}|^2.
+  // The last step's f update follows the loop.
   cx<T> *qp = q.as<cx<T>>();
   double *zz = dz, *zz_old = zz2.as<double>(), *pq = s, *al = alpha.as<double>();
   const int PCs = plane_group_log2(P);
   const dim3 rg(kRedBlocks, P >> PCs);
+  const bool keep = defer_last && nsteps <= kept;
   for (int it = 0; it < nsteps; it++) {
     ToepCg<T> cg{zp, pp, fhat, zz, it ? zz_old : nullptr, it ? al : nullptr, rowpart.as<double>()};
+    if (keep) {
+      // p_k to its own buffer, p_{k-1} read from the previous one, no f update
+      cg.p = pk.as<cx<T>>() + it * NN;
+      cg.pin = it ? cg.p - NN : nullptr;
+      cg.f = nullptr;
+      cg.alpha = nullptr;
+      al = alk.as<double>() + (long long)it * P;
+    }
     toep->apply_cg(cg, qp, st);
     hipLaunchKernelGGL(k_sum_partials, dim3(cdiv(P, 8)), dim3(256), 0, st, rowpart.as<double>(), N, P, pq);
     {
