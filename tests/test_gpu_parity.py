@@ -202,10 +202,10 @@ def test_toeplitz_lane_pairs_job(hip, monkeypatch):
     assert np.max(np.abs(r_pairs - r64)) < 1e-5
 
 
-@pytest.mark.parametrize("nz", [32, 48])
+@pytest.mark.parametrize("nz", [64, 48])
 def test_kept_directions_job(hip, monkeypatch, nz):
     """The projected-start solve with its ten search directions kept apart
-    (f + sum_k alpha_k p_k applied once, in the z transform's loads; nz = 32
+    (f + sum_k alpha_k p_k applied once, in the z transform's loads; nz = 64
     on the register z transform, nz = 48 on the generic one with separate
     updates) equals the solve that updates f in every row pass to FP32
     rounding, and the FP64 plain path to the FP32 bar, over HIO and ER
