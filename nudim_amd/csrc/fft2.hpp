@@ -782,7 +782,7 @@ k_zfft2(const T *__restrict__ img_in, const cx<T> *__restrict__ g_in, const cx<T
         const cx<T> *__restrict__ tw, long long nlines, int nk, long long pstride) {
   constexpr int n = N1 * N2, G = 256 / N1, RS = G * N2 + 1;
   constexpr int S = INV ? +1 : -1;
-  static_assert(256 % N1 == 0 && N1 >= N2 && N2 % 2 == 0 && N1 % 2 == 0, "lane layout");
+  static_assert(256 % N1 == 0 && N1 >= N2 && N2 % 2 == 0 && N1 % 8 == 0, "lane layout");
   __shared__ cx<T> lx[N1 * RS];
   __shared__ cx<T> stw[n];
   const int tid = threadIdx.x, r = tid % N1, g = tid / N1;
@@ -804,15 +804,25 @@ k_zfft2(const T *__restrict__ img_in, const cx<T> *__restrict__ g_in, const cx<T
       }
     }
     if constexpr (INV) {
-      // f + sum_s alpha_s p_s, the steps in order
+      // f + sum_s alpha_s p_s, the steps in order, 8 positions at a time
+      // with their loads issued together (loaded and used one by one, the
+      // loads were waited for one at a time)
 #pragma unroll 1
       for (int s = 0; s < nk; s++) {
         const cx<T> *ps = pv + s * pstride + base;
         const double *as = alpha + s * n;
 #pragma unroll
-        for (int j1 = 0; j1 < N1; j1++) {
-          const int zs = r + N2 * ((j1 + N1 / 2) % N1);
-          a[j1] = a[j1] + (T)as[zs] * ps[zs];
+        for (int c = 0; c < N1; c += 8) {
+          cx<T> pl[8];
+          double al[8];
+#pragma unroll
+          for (int u = 0; u < 8; u++) {
+            const int zs = r + N2 * ((c + u + N1 / 2) % N1);
+            pl[u] = ps[zs];
+            al[u] = as[zs];
+          }
+#pragma unroll
+          for (int u = 0; u < 8; u++) a[c + u] = a[c + u] + (T)al[u] * pl[u];
         }
       }
     } else {
