@@ -48,3 +48,21 @@ def test_estimate_bytes_scales(hip):
     small = hip.estimate_bytes(hip.make_params(128, 128, 128, -60, 60, 3))
     big = hip.estimate_bytes(hip.make_params(384, 256, 384, -70, 70, 2))
     assert big > 10 * small > 0
+
+
+def test_estimate_counts_kept_directions(hip, monkeypatch):
+    """The admission estimate holds the projected-start solve's ten kept
+    directions (complex FP32 N^2 nz each, and ten FP64 alphas per plane)
+    exactly when a job keeps them: up to 4 GiB of directions (the 256^3
+    workload, 3 GB), not at np = 768, nz = 512 (24 GB) unless forced."""
+    def est(np_, nz, keep):
+        if keep is None:
+            monkeypatch.delenv("NUDIM_KEEP_P", raising=False)
+        else:
+            monkeypatch.setenv("NUDIM_KEEP_P", keep)
+        return hip.estimate_bytes(hip.make_params(np_, nz, np_, -70, 70, 2, weight=1, precision=hip.FP32))
+    for np_, nz, kept in ((384, 256, True), (768, 512, False)):
+        extra = 10 * np_ * np_ * nz * 8 + 10 * nz * 8
+        assert est(np_, nz, None) - est(np_, nz, "0") == (extra if kept else 0)
+        assert est(np_, nz, "1") - est(np_, nz, "0") == extra
+
