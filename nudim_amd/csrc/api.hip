@@ -260,10 +260,11 @@ static bool rform_on() {
 // Whether the projected-start solve keeps its ten search directions apart
 // (bytes: their size) and the z transform applies f + sum_k alpha_k p_k,
 // instead of f being read and written in nine row passes.  Up to 4 GiB of
-// directions the z transform reads its eleven streams near the copy rate;
-// at 24 GB (np = 768, nz = 512) they drop to about a third of it and the
-// pass is slower than with the per-step update.  NUDIM_KEEP_P=0 / 1 turns
-// it off / on whatever the size.
+// directions (3 GB at np = 384, nz = 256) this is 2-3 % faster per pass; at
+// 24 GB (np = 768, nz = 512) the z transform sums its eleven streams at
+// about 3.3 TB/s and the pass is only level with the per-step update, so
+// larger jobs keep the per-step update and the memory.  NUDIM_KEEP_P=0 / 1
+// turns it off / on whatever the size.
 static bool keep_on(size_t bytes) {
   const char *e = getenv("NUDIM_KEEP_P");
   if (e && std::string(e) == "0") return false;
