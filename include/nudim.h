@@ -106,8 +106,12 @@ int nudim_estimate_bytes(const nudim_params *p, size_t *bytes);
  * measured on the device for its node set, once per process and node set: the
  * fastest pair among the fixed-length FFT sizes up to the default n whose
  * trafo and adjoint stay within 2e-6 relative L2 of the FP64 NFFT at the
- * reference's sizes.  nudim_create does the same; calling this first moves
- * the measurement out of the first job.  (No reference counterpart: the
+ * reference's sizes.  That is, for a job whose passes run the NFFT (cg_form
+ * NUDIM_CG_NFFT, or NUDIM_CG_START=nfft / NUDIM_R_FORM=nfft); every other
+ * FP32 job runs its NFFT only at load and in the initial INFFT and always
+ * uses the defaults n = smallest 2^a 3^b 5^c >= 2 np, W = 8, measured or
+ * not.  nudim_create does the same; calling this first moves the
+ * measurement out of the first job.  (No reference counterpart: the
  * reference fixes n = 2 nextpow2(np), 14 taps, src/reconstruction.c:393-405.) */
 int nudim_autotune(const nudim_params *p, int *n, int *W);
 /* kernel timing with HIP events on the job's stream during nudim_iterate:

@@ -91,13 +91,16 @@ __global__ void k_assemble(const cx<double> *__restrict__ tomo, const int *__res
 
 // R sums from the quadratic form (reconstruction.c:574-582 over the known
 // nodes K): sum_K |A f - m|^2 = f^H T_K f - 2 Re <f, A_K^H m> + sum_K |m|^2,
-// and the same with + for |A f + m|^2
+// and the same with + for |A f + m|^2.  Near a consistent f the first sum is
+// a cancellation of terms of size |m|^2, each carrying the FP32 transform's
+// relative error (~1e-6), so it can come out slightly negative: both sums
+// are clamped at zero (R's absolute accuracy floor is then about 1e-3).
 __global__ void k_rform_final(const double *__restrict__ q, const double *__restrict__ x,
                               const double *__restrict__ m2, double *__restrict__ d1, double *__restrict__ d2, int P) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= P) return;
-  d1[p] = q[p] - 2.0 * x[p] + m2[p];
-  d2[p] = q[p] + 2.0 * x[p] + m2[p];
+  d1[p] = fmax(q[p] - 2.0 * x[p] + m2[p], 0.0);
+  d2[p] = fmax(q[p] + 2.0 * x[p] + m2[p], 0.0);
 }
 
 // R = sqrt(sum_p d1 / sum_p d2) (reconstruction.c:603-604); one block of 256
@@ -204,6 +207,10 @@ static void validate(const nudim_params *p) {
   NUDIM_CHECK(p->precision == NUDIM_FP64_PLAIN || p->precision == NUDIM_FP32, "unknown precision");
   NUDIM_CHECK(!(p->denoise && p->method == NUDIM_DIFFUSION && p->nz != p->np),
               "DIFFUSION denoising needs nz == np (the reference clamps x with the z extent)");
+  // the TV sweep kernels address the volume with 32-bit byte offsets
+  NUDIM_CHECK(!(p->denoise && p->method == NUDIM_TV &&
+                (double)p->np * p->np * p->nz * (p->precision == NUDIM_FP32 ? 4 : 8) >= 4294967296.0),
+              "TV denoising needs a volume of fewer than 2^32 bytes (np^2 nz x 4 in FP32, x 8 in FP64)");
 }
 
 static void make_geometry(const nudim_params *p, Geometry *g) {
@@ -241,15 +248,6 @@ static NfftTuning tuned_nfft(const nudim_params *p, const Geometry &g, hipStream
   return t;
 }
 
-// a measurement already made for the job's node set (false: none)
-static bool cached_nfft(const nudim_params *p, NfftTuning *t) {
-  std::lock_guard<std::mutex> lk(g_tune_mu);
-  auto it = g_tune.find(tune_key(p));
-  if (it == g_tune.end()) return false;
-  *t = it->second;
-  return true;
-}
-
 // R from the known-node quadratic form (NUDIM_R_FORM=nfft: through the
 // forward NFFT); see Job::rform
 static bool rform_on() {
@@ -271,6 +269,15 @@ static bool keep_on(size_t bytes) {
   if (e && std::string(e) == "1") return true;
   return bytes <= ((size_t)4 << 30);
 }
+
+static bool projected_start(int form, int N);
+
+// Whether a job's passes run the NFFT.  With the projected CG start and R
+// from the quadratic form they do not (only the load and the initial INFFT
+// do): such a job always takes the default sizes, n = nice_fft_size(2 np)
+// and 8 taps (what the measurement picks at sigma = 2 on the configured node
+// sets), so its result does not depend on what the process measured before.
+static bool passes_run_nfft(int form, int np) { return !(projected_start(form, np) && rform_on()); }
 
 static void resolve_knobs(const nudim_params *p, int *n, int *W, int *form) {
   if (p->precision == NUDIM_FP64_PLAIN) {
@@ -333,13 +340,8 @@ struct Job : JobBase {
     P = p.nz;
     M = g.M;
     resolve_knobs(&p, &n, &W, &form);
-    if (autotune_enabled(&p)) {
-      // With the projected CG start and R from the quadratic form the
-      // passes run no NFFT (only the load and the initial INFFT do): a
-      // measurement made for this node set is used, none is started.
-      NfftTuning t{n, W, -1.0, -1.0};
-      if (!(projected_start(form, p.np) && rform_on())) t = tuned_nfft(&p, g, s);
-      else cached_nfft(&p, &t);
+    if (autotune_enabled(&p) && passes_run_nfft(form, p.np)) {
+      const NfftTuning t = tuned_nfft(&p, g, s);
       n = t.n;
       W = t.W;
     }
@@ -945,7 +947,7 @@ extern "C" int nudim_autotune(const nudim_params *p, int *n, int *W) {
   NUDIM_CHECK(n && W, "null output");
   int form;
   resolve_knobs(p, n, W, &form);
-  if (autotune_enabled(p)) {
+  if (autotune_enabled(p) && passes_run_nfft(form, p->np)) {
     NUDIM_HIP(hipSetDevice(p->device));
     Geometry g;
     make_geometry(p, &g);

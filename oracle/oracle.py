@@ -266,6 +266,26 @@ def assemble(p, proj):
     return slices, kind
 
 
+def random_fill(slices, kind, seed=0):
+    """--random padding (src/reconstruction.c:434-469): every sample that is
+    not known (unknown rays and the held-out 0 degree ray) gets
+    gsl_rng_uniform(rng) * 10 as its real part, planes outer and nodes inner,
+    from gsl_rng_default = gsl_rng_mt19937 with the default seed 0, which
+    GSL's mt19937 seeding maps to 4357.  The stream is numpy's MT19937 with
+    the init_genrand seeding (`_legacy_seeding`), an implementation of the
+    same published generator (Matsumoto and Nishimura 1998) independent of
+    the product's; gsl_rng_uniform for mt19937 is the 32-bit output / 2^32.
+    Returns a new array."""
+    nz, M = slices.shape
+    unk = np.flatnonzero(np.asarray(kind) != 1)
+    g = np.random.MT19937()
+    g._legacy_seeding(4357 if seed == 0 else seed)
+    draws = g.random_raw(nz * unk.size).astype(np.float64) / 4294967296.0 * 10
+    out = np.array(slices, np.complex128, copy=True)
+    out[:, unk] = draws.reshape(nz, unk.size)
+    return out
+
+
 def initial(p, slices):
     img0 = np.zeros((p.nz, p.np, p.np))
     rc = lib().oracle_initial(C.byref(p), np.ascontiguousarray(slices), img0)

@@ -66,3 +66,21 @@ def test_estimate_counts_kept_directions(hip, monkeypatch):
         assert est(np_, nz, None) - est(np_, nz, "0") == (extra if kept else 0)
         assert est(np_, nz, "1") - est(np_, nz, "0") == extra
 
+
+
+def test_tv_volume_limit_refused_up_front(hip):
+    """TV jobs of 2^32 bytes or more are refused by create and by the
+    admission estimate (before any device work), not at their first pass."""
+    p = hip.make_params(1024, 1024, 1024, -60, 60, 3, denoise=True, method=hip.TV)
+    with pytest.raises(hip.NudimError, match="fewer than 2\\^32 bytes"):
+        hip.estimate_bytes(p)
+    with pytest.raises(hip.NudimError, match="fewer than 2\\^32 bytes"):
+        hip.Reconstruction(p)
+    # the same volume in FP32 (4 GiB exactly) is refused as well; one plane pair fewer fits
+    p = hip.make_params(1024, 1024, 1024, -60, 60, 3, denoise=True, method=hip.TV, precision=hip.FP32)
+    with pytest.raises(hip.NudimError, match="fewer than 2\\^32 bytes"):
+        hip.estimate_bytes(p)
+    assert hip.estimate_bytes(hip.make_params(1024, 1022, 1024, -60, 60, 3, denoise=True, method=hip.TV,
+                                              precision=hip.FP32)) > 0
+    # without denoising the size is not limited by the TV kernels
+    assert hip.estimate_bytes(hip.make_params(1024, 1024, 1024, -60, 60, 3)) > 0

@@ -1,7 +1,9 @@
-"""FP32 NFFT sizing by measurement (nudim_autotune): the chosen (n, W) is a
-fixed-length FFT size no larger than the default, its NFFT is within the
-stated 2e-6 of the oracle's, and a job created without explicit sizes runs
-exactly as one given the tuned sizes."""
+"""FP32 NFFT sizing by measurement (nudim_autotune): for a job whose passes
+run the NFFT the chosen (n, W) is a fixed-length FFT size no larger than the
+default, its NFFT is within the stated 2e-6 of the oracle's, and a job
+created without explicit sizes runs exactly as one given the tuned sizes.
+A job whose passes run no NFFT takes the default sizes whatever was
+measured before in the process."""
 import numpy as np
 import pytest
 
@@ -19,7 +21,7 @@ def params(hip, np_, nz, **kw):
 
 @pytest.mark.parametrize("np_", [64, 192])
 def test_autotune_choice_and_accuracy(hip, orc, np_):
-    p = params(hip, np_, 64)
+    p = params(hip, np_, 64, cg_form=hip.CG_NFFT)
     n, W = hip.autotune(p)
     assert n in FIXED_N and np_ + 16 <= n <= 2 * np_ and W in (4, 6, 8, 10, 12)
     assert hip.autotune(p) == (n, W)  # cached
@@ -42,10 +44,10 @@ def test_autotuned_job_equals_explicit_sizes(hip, orc, monkeypatch):
     rng = np.random.default_rng(5)
     dens = blob_phantom((nz, np_, np_), 10, rng, sig=(1.5, 3.0), slab=np_ // 2)
     proj = orc.project_2d1d(dens, np_, -60, 60, 5)
-    n, W = hip.autotune(params(hip, np_, nz))
+    n, W = hip.autotune(params(hip, np_, nz, cg_form=hip.CG_NFFT))
     out = []
     for kw in ({}, {"nfft_n": n, "nfft_w": W}):
-        with hip.Reconstruction(params(hip, np_, nz, **kw)) as job:
+        with hip.Reconstruction(params(hip, np_, nz, cg_form=hip.CG_NFFT, **kw)) as job:
             job.load_projections(proj)
             job.initial()
             out.append((job.volume(), job.iterate(3), job.volume()))
@@ -53,4 +55,29 @@ def test_autotuned_job_equals_explicit_sizes(hip, orc, monkeypatch):
         assert np.array_equal(a, b)
     # switched off: the fixed default sizing
     monkeypatch.setenv("NUDIM_AUTOTUNE", "0")
-    assert hip.autotune(params(hip, np_, nz)) == (128, 8)
+    assert hip.autotune(params(hip, np_, nz, cg_form=hip.CG_NFFT)) == (128, 8)
+
+
+def test_no_nfft_job_ignores_earlier_measurement(hip, orc):
+    """A Toeplitz-form FP32 job (projected CG start, R from the quadratic
+    form: no NFFT in its passes) gives the same result before and after the
+    process measured sizes for its node set, and equal to the default sizes."""
+    np_, nz = 48, 32
+    rng = np.random.default_rng(11)
+    dens = blob_phantom((nz, np_, np_), 8, rng, sig=(1.5, 3.0), slab=np_ // 2)
+    proj = orc.project_2d1d(dens, np_, -60, 60, 5)
+
+    def run(**kw):
+        with hip.Reconstruction(params(hip, np_, nz, **kw)) as job:
+            job.load_projections(proj)
+            job.initial()
+            r = job.iterate(2)
+            return job.volume(), r
+
+    before = run()
+    assert hip.autotune(params(hip, np_, nz)) == (96, 8)  # defaults, not measured
+    hip.autotune(params(hip, np_, nz, cg_form=hip.CG_NFFT))  # measures this node set
+    after = run()
+    explicit = run(nfft_n=96, nfft_w=8)
+    for a, b, c in zip(before, after, explicit):
+        assert np.array_equal(a, b) and np.array_equal(a, c)

@@ -108,6 +108,54 @@ def test_cli_end_to_end(binary, orc, tmp_path, prec):
     assert np.max(np.abs(printed - rt)) < (1e-6 if prec == "fp64" else 1e-4)
 
 
+@pytest.mark.gpu
+def test_cli_dumps_at_pass_100(binary, hip, orc, tmp_path):
+    """--denoise past pass 100: the serial program writes grad_before.txt and
+    grad_after.txt ("%f", [z][y][x]) with the volume before and after the
+    denoiser of pass 100 (src/reconstruction.c:637-658).  The files are the
+    library's own dump of that pass, byte for byte; the dumped volumes are
+    pinned to the oracle: the ER constraint of "before" is the oracle's
+    pass 100 without the denoiser from the same volume, and "after" is the
+    oracle's TV-height of "before"."""
+    np_ = nz = 16
+    rng = np.random.default_rng(23)
+    dens = blob_phantom((nz, np_, np_), 6, rng, sig=(1.5, 3.0), slab=8)
+    proj = orc.project_2d1d(dens, np_, -60, 60, 5)
+    files = tmp_path / "files"
+    files.mkdir()
+    for j in range(proj.shape[0]):
+        np.savetxt(files / ("prj-%d.txt" % j), proj[j].ravel(), fmt="%f")
+    args = ["--prefix=prj", "--pad=16", "--nx=16", "--nz=16", "--max=60", "--min=-60", "-t", "5",
+            "--padangle=1", "--nhio=2", "--ner=99", "--weight", "--denoise", "--method=TV", "--height=8",
+            "--center=8"]
+    r = run(args, tmp_path, env={"NUDIM_PRECISION": "fp64"})
+    assert r.returncode == 0, r.stderr
+    before_txt = (tmp_path / "grad_before.txt").read_bytes()
+    after_txt = (tmp_path / "grad_after.txt").read_bytes()
+    # the same job through the library, with full-precision dumps at pass 100
+    proj_in = np.stack([np.loadtxt(files / ("prj-%d.txt" % j)).reshape(nz, np_) for j in range(proj.shape[0])])
+    hp = hip.make_params(np_, nz, np_, -60, 60, 5, padangle=1.0, weight=1, nhio=2, ner=99, denoise=True,
+                         method=hip.TV, height=8, center=8)
+    before = np.zeros((nz, np_, np_))
+    after = np.zeros((nz, np_, np_))
+    with hip.Reconstruction(hp) as job:
+        job.load_projections(proj_in)
+        job.initial()
+        job.set_dump(100, before, after)
+        job.iterate(100)
+        v100 = job.volume()
+        job.iterate(1)
+    fmt = lambda a: "".join("%f\n" % v for v in a.ravel()).encode()
+    assert before_txt == fmt(before)
+    assert after_txt == fmt(after)
+    assert not np.array_equal(before, after)
+    op = orc.params(np_, nz, np_, -60, 60, 5, padangle=1.0, weight=1, nhio=2, ner=99, height=8, center=8)
+    slices, kind = orc.assemble(op, proj_in)
+    er, _ = orc.iterate(op, v100, slices, kind, 100)  # pass 100 without the denoiser
+    assert rel(orc.constraint(before, v100, np_, 0, 1, 8, 8), er) < 1e-10
+    assert rel(after, orc.tv_height(before, 8, 8)) < 1e-11
+
+
 # ---- projection_sim ---------------------------------------------------------
 
 SIM = os.path.join(ROOT, "nudim_amd", "bin", "projection_sim")

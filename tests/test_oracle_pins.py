@@ -286,6 +286,62 @@ def test_diffusion_invariants(orc):
         orc.diffusion2d(np.zeros((3, 4, 4)))
 
 
+@pytest.mark.parametrize("h,c", [(0.5, 0.5), (1.5, 0.1)])
+def test_diffusion_conductance_closed_form(orc, h, c):
+    """Finite kappa: a single spike of height h in a zero plane.  In one
+    lexicographic sweep the voxel above the spike (row y-1) and the one to
+    its left (column x-1) are visited before it, each sees the spike as its
+    only non-zero neighbour difference g = h, and moves by lam c(g) g with the
+    Perona-Malik rational conductance c(g) = 1 / (1 + g^2 / kappa^2): 1/2 at
+    g = kappa, 1/10 at g = 3 kappa.  The exponential conductance
+    exp(-g^2 / kappa^2) (e^-1, e^-9) or a sign or neighbour mistake fails."""
+    lam, kappa = 0.25, 0.5  # exact in float, as the reference's float arguments
+    u = np.zeros((6, 6, 6))
+    u[2, 3, 3] = h
+    got = orc.diffusion2d(u, lam=lam, kappa=kappa, niter=1)
+    a = lam * c * h
+    assert got[2, 2, 3] == a
+    assert got[2, 3, 2] == a
+    # voxels before them in the sweep saw only zeros
+    assert np.all(got[2, :2] == 0) and np.all(got[2, 2, :3] == 0)
+    # other planes are independent
+    assert np.all(got[[0, 1, 3, 4, 5]] == 0)
+
+
+def test_mt19937_stream_known_answers():
+    """The --random stream: numpy's MT19937 with init_genrand seeding gives
+    the generator's published known answers (seed 5489: first output
+    3499211612, 10000th output 4123659995, the value C++ [rand.predef]
+    requires of std::mt19937)."""
+    g = np.random.MT19937()
+    g._legacy_seeding(5489)
+    r = g.random_raw(10000)
+    assert r[0] == 3499211612 and r[-1] == 4123659995
+
+
+def test_random_fill(orc):
+    """--random fills exactly the unknown and held-out samples, plane by
+    plane in node order, with values in [0, 10) from one stream; the known
+    samples keep the measured values."""
+    np_ = nz = 16
+    p = orc.params(np_, nz, np_, -60, 60, 5)
+    rng = np.random.default_rng(1)
+    proj = rng.standard_normal((25, nz, np_))
+    slices, kind = orc.assemble(p, proj)
+    filled = orc.random_fill(slices, kind)
+    known = kind == 1
+    assert np.array_equal(filled[:, known], slices[:, known])
+    v = filled[:, ~known]
+    assert np.all(v.imag == 0) and np.all((v.real >= 0) & (v.real < 10))
+    g = np.random.MT19937()
+    g._legacy_seeding(4357)
+    first = g.random_raw(3) / 4294967296.0 * 10
+    assert np.array_equal(v[0, :3].real, first)
+    # the second plane continues the stream
+    g._legacy_seeding(4357)
+    assert np.array_equal(v[1, :2].real, g.random_raw(v.shape[1] + 2)[-2:] / 4294967296.0 * 10)
+
+
 def test_constraint_closed_form(orc):
     rng = np.random.default_rng(10)
     img = rng.standard_normal((3, 8, 8))
