@@ -150,6 +150,20 @@ int nudim_tv(int precision, double *u, int nz, int ny, int nx, float lam, int ni
 /* total_variation_discrete_height, src/denoise.c:153-249 */
 int nudim_tv_height(int precision, double *u, int nz, int ny, int nx, float lam, int niter,
                     int nheight, int ncenter);
+/* No counterpart in the reference: nudim_tv_height on caller DEVICE memory
+ * that allocates nothing, for callers that cannot call hipMalloc (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid allocation).
+ * d_u is a device array double [nz][ny][nx], in/out; the work is enqueued on
+ * `stream` (a hipStream_t; NULL: the calling thread's default stream) on the
+ * current device, and the call returns without waiting for it.  The sweeps'
+ * buffers live in a file-scope device array of 68 MiB per GPU, held until the
+ * process ends: 4 nz ny nx sizeof(T) + 2 x (ny 128 x 8) + 256 bytes, rounded
+ * to 256-byte sections, must fit (FP64: about 2.2M voxels, 128^3 fits;
+ * FP32: about 4.4M).  Larger volumes fail with NUDIM_EINVAL.  Calls on one GPU
+ * share that array: calls that overlap on one GPU (other streams, other
+ * threads) race. */
+int nudim_tv_height_noalloc(int precision, double *d_u, int nz, int ny, int nx, float lam, int niter,
+                            int nheight, int ncenter, void *stream);
 /* aniso_diffusion_2d, src/denoise.c:320-378 (requires nz == nx, see :349-350) */
 int nudim_diffusion2d(int precision, double *u, int nz, int ny, int nx, float lambda,
                       float kappa, int niter);

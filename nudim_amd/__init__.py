@@ -49,7 +49,8 @@ EXPORTS = [
     "nudim_load_projections", "nudim_load_samples", "nudim_initial", "nudim_iterate",
     "nudim_get_volume", "nudim_set_volume", "nudim_set_dump", "nudim_device_bytes",
     "nudim_estimate_bytes", "nudim_autotune", "nudim_synchronize", "nudim_profile", "nudim_profile_read", "nudim_nfft_trafo", "nudim_nfft_adjoint",
-    "nudim_cgnr", "nudim_zfft", "nudim_tv", "nudim_tv_height", "nudim_diffusion2d",
+    "nudim_cgnr", "nudim_zfft", "nudim_tv", "nudim_tv_height", "nudim_tv_height_noalloc",
+    "nudim_diffusion2d",
     "nudim_constraint", "nudim_projection_ft", "nudim_project_2d1d",
     "nudim_server_create", "nudim_server_submit", "nudim_server_next", "nudim_server_stats",
     "nudim_server_destroy",
@@ -113,6 +114,7 @@ def _declare(L):
         "nudim_zfft": (i, [i, i, i, dp, i]),
         "nudim_tv": (i, [i, dp, i, i, i, f, i]),
         "nudim_tv_height": (i, [i, dp, i, i, i, f, i, i, i]),
+        "nudim_tv_height_noalloc": (i, [i, v, i, i, i, f, i, i, i, v]),
         "nudim_diffusion2d": (i, [i, dp, i, i, i, f, f, i]),
         "nudim_constraint": (i, [i, dp, dp, i, i, i, i, i, i, i]),
         "nudim_projection_ft": (i, [i, i, i, i, i, dp, dp]),
@@ -325,6 +327,16 @@ def tv_height(u, nheight, ncenter, lam=0.2, niter=40, precision=FP64_PLAIN):
     v = np.array(u, np.float64, copy=True, order="C")
     _check(lib().nudim_tv_height(precision, _ptr(v), *v.shape, lam, niter, nheight, ncenter))
     return v
+
+
+def tv_height_noalloc(dev_ptr, shape, nheight, ncenter, lam=0.2, niter=40, precision=FP64_PLAIN, stream=None):
+    """nudim_tv_height_noalloc: TV-height in place on a device array of
+    float64 [nz][ny][nx] at address dev_ptr, enqueued on `stream` (a
+    hipStream_t address; None: the thread's default stream).  The caller
+    synchronises the stream before reading the result."""
+    nz, ny, nx = shape
+    _check(lib().nudim_tv_height_noalloc(precision, C.c_void_p(dev_ptr), nz, ny, nx, lam, niter, nheight, ncenter,
+                                         None if stream is None else C.c_void_p(stream)))
 
 
 def diffusion2d(u, lam=0.25, kappa=0.005, niter=80, precision=FP64_PLAIN):

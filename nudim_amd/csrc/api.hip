@@ -1159,6 +1159,94 @@ extern "C" int nudim_tv_height(int precision, double *u, int nz, int ny, int nx,
   API_END
 }
 
+// ---------------------------------------------------------------------------
+// TV-height on caller device memory with a file-scope scratch array (no
+// device allocation): for callers that cannot call hipMalloc
+// ---------------------------------------------------------------------------
+
+// Scratch of nudim_tv_height_noalloc: the device-layout volume u [y][x][z],
+// the second u buffer, the two gradient buffers, the two partial-norm arrays
+// and the norm pair.  Each GPU holds its own copy until the process ends.
+static constexpr size_t kScratchBytes = (size_t)68 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// its device address on each GPU (hipGetSymbolAddress, once per device)
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[64];
+
+static unsigned char *scratch_on_current_device() {
+  int dev = -1;
+  NUDIM_HIP(hipGetDevice(&dev));
+  NUDIM_CHECK(dev >= 0 && dev < 64, "device ordinal out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    NUDIM_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    NUDIM_CHECK(ptr != nullptr, "hipGetSymbolAddress(scratch) returned a null address");
+    g_scratch_ptr[dev] = ptr;
+  }
+  return static_cast<unsigned char *>(g_scratch_ptr[dev]);
+}
+
+static size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+__global__ void k_zero_words(uint4 *__restrict__ d, long long nwords) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nwords) d[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+// scratch bytes the TV sweeps of one [nz][ny][nx] volume take: norm pair,
+// two partial-sum arrays, four volume buffers
+template <typename T>
+static size_t tv_scratch_bytes(int nz, int ny, int nx) {
+  const size_t V = (size_t)nz * ny * nx;
+  return 256 + 2 * round256((size_t)tv_blocks(ny, nx, nz) * 8) + 4 * round256(V * sizeof(T));
+}
+
+template <typename T>
+static void tv_height_noalloc(double *d_u, int nz, int ny, int nx, float lam, int niter, int ylo, int yhi,
+                              hipStream_t st) {
+  const size_t need = tv_scratch_bytes<T>(nz, ny, nx);
+  if (need > kScratchBytes)
+    throw Error(NUDIM_EINVAL, "nudim_tv_height_noalloc: a " + std::to_string(nz) + "x" + std::to_string(ny) + "x" +
+                                  std::to_string(nx) + " volume needs " + std::to_string(need) +
+                                  " bytes of scratch; the maximum is " + std::to_string(kScratchBytes));
+  unsigned char *base = scratch_on_current_device();
+  const long long V = (long long)nz * ny * nx;
+  const int nb = tv_blocks(ny, nx, nz);
+  const size_t vb = round256((size_t)V * sizeof(T)), pb = round256((size_t)nb * 8);
+  double *sums = reinterpret_cast<double *>(base);
+  double *part_g = reinterpret_cast<double *>(base + 256);
+  double *part_a = reinterpret_cast<double *>(base + 256 + pb);
+  unsigned char *vols = base + 256 + 2 * pb;
+  T *u0 = reinterpret_cast<T *>(vols), *u1 = reinterpret_cast<T *>(vols + vb);
+  T *g0 = reinterpret_cast<T *>(vols + 2 * vb), *g1 = reinterpret_cast<T *>(vols + 3 * vb);
+  // the gradient buffers keep the previous call's values: clear both (they
+  // must be zero outside the band rows)
+  hipLaunchKernelGGL(k_zero_words, dim3(cdiv(2 * vb / 16, 256)), dim3(256), 0, st, reinterpret_cast<uint4 *>(g0),
+                     (long long)(2 * vb / 16));
+  hipLaunchKernelGGL(k_vol_from_ref<T>, dim3(cdiv(V, 256)), dim3(256), 0, st, (const double *)d_u, u0, ny, nx, nz);
+  NUDIM_HIP(hipGetLastError());
+  T *r = tv_sweeps<T>(u0, u1, g0, g1, ny, nx, nz, ylo, yhi, lam, niter, part_g, part_a, sums, nb, st);
+  hipLaunchKernelGGL(k_vol_to_ref<T>, dim3(cdiv(V, 256)), dim3(256), 0, st, (const T *)r, d_u, ny, nx, nz);
+  NUDIM_HIP(hipGetLastError());
+}
+
+extern "C" int nudim_tv_height_noalloc(int precision, double *d_u, int nz, int ny, int nx, float lam, int niter,
+                                       int nheight, int ncenter, void *stream) {
+  API_BEGIN
+  NUDIM_CHECK(d_u && nz > 0 && ny > 0 && nx > 0 && niter >= 0, "invalid argument");
+  NUDIM_CHECK(precision == NUDIM_FP32 || precision == NUDIM_FP64_PLAIN, "invalid precision");
+  int ylo = (2 * ncenter - nheight) / 2, yhi = (2 * ncenter + nheight) / 2;
+  ylo = std::max(ylo, 0);
+  yhi = std::min(yhi, ny - 1);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (precision == NUDIM_FP32) tv_height_noalloc<float>(d_u, nz, ny, nx, lam, niter, ylo, yhi, st);
+  else tv_height_noalloc<double>(d_u, nz, ny, nx, lam, niter, ylo, yhi, st);
+  return NUDIM_OK;
+  API_END
+}
+
 template <typename T>
 static void diffusion_parity(double *u, int nz, int ny, int nx, float lambda, float kappa, int niter) {
   VolTmp<T> v(u, nz, ny, nx);

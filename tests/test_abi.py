@@ -68,6 +68,16 @@ def test_estimate_counts_kept_directions(hip, monkeypatch):
 
 
 
+def test_noalloc_rejects_oversize_before_device_work(hip):
+    """The no-allocation TV entry point checks the volume against its fixed
+    scratch on the host, before touching the device (so no GPU is needed
+    here and the address is never dereferenced)."""
+    with pytest.raises(hip.NudimError, match="bytes of scratch; the maximum is 71303168"):
+        hip.tv_height_noalloc(0x1000, (256, 256, 256), 64, 128)
+    with pytest.raises(hip.NudimError, match="bytes of scratch"):
+        hip.tv_height_noalloc(0x1000, (140, 128, 128), 64, 64, precision=hip.FP64_PLAIN)
+
+
 def test_tv_volume_limit_refused_up_front(hip):
     """TV jobs of 2^32 bytes or more are refused by create and by the
     admission estimate (before any device work), not at their first pass."""
