@@ -9,12 +9,16 @@ One step is one pass of the HIO/ER loop body over the whole volume
 (z FFT, per-plane projector + 10-step CGNR for all 256 planes, z IFFT,
 40 TV sweeps, density constraint), inputs resident in HBM.
 
-value  = passes/s of the FP32 batched form (NUDIM_FP32).
-Beside it: passes/s of the plain FP64 path (NUDIM_FP64_PLAIN: literal
-NFFT3-order CGNR with the reference's NFFT sizes), their ratio, the relative
-L2 difference between the two after the same passes, the dominant kernel's
-roofline figure (HIP events over the timed region) and a CPU baseline (the
-FP64 C restatement on the host cores, bounded sample).
+value  = passes/s of the FP32 batched form (NUDIM_FP32), direct launches.
+Beside it: the same passes replayed as hipGraphs, passes/s of the plain FP64
+path (NUDIM_FP64_PLAIN: literal NFFT3-order CGNR with the reference's NFFT
+sizes), their ratio, the relative L2 difference between the two after the
+same passes, the dominant kernel's roofline figure (HIP events over the
+timed region), a CPU baseline (the FP64 C restatement on the host cores and
+on one core, bounded samples), the uneven batch (48 volumes through the job
+server) and the serving traces (100 Poisson arrivals of the mixed-size mix
+at 4 jobs/s, a below-capacity trace, and a short trace of the 128^3 and
+256^3 classes on both the FP32 form and the FP64 plain path).
 """
 import argparse
 import json
@@ -34,9 +38,10 @@ HBM_PEAK_GBS = 8000.0
 # profiler classes of the library (nudim_profile) and the kernels they time;
 # the template arguments follow the workload: the register FFT split of each
 # length (fft2.hip), the NFFT window W, the z length
-FFT2_SPLIT = {128: (16, 8), 256: (16, 16), 384: (24, 16), 512: (32, 16), 768: (32, 24), 1024: (32, 32),
-              1536: (48, 32)}
-ZFFT2_SPLIT = {64: (8, 8), 128: (16, 8), 256: (16, 16), 512: (32, 16)}
+FFT2_SPLIT = {128: (16, 8), 160: (20, 8), 192: (16, 12), 240: (20, 12), 256: (16, 16), 288: (24, 12),
+              320: (20, 16), 384: (24, 16), 480: (24, 20), 512: (32, 16), 576: (24, 24), 640: (32, 20),
+              768: (32, 24), 1024: (32, 32), 1536: (48, 32)}
+ZFFT2_SPLIT = {64: (8, 8), 96: (16, 6), 128: (16, 8), 192: (16, 12), 256: (16, 16), 512: (32, 16)}
 
 
 def kc_names(np_, nz, n, W):
@@ -118,12 +123,11 @@ def cpu_model():
     return "unknown CPU"
 
 
-def cpu_baseline(prm, proj):
-    """FP64 C restatement (the oracle) on the host: Fourier step of a bounded
-    sample of planes, TV sweeps and z FFTs, scaled to one pass of the whole
-    volume (about 10-20 s of host work)."""
+def cpu_baseline(prm, proj, threads):
+    """FP64 C restatement (the oracle) on `threads` host threads: Fourier
+    step of a bounded sample of planes, TV sweeps and z FFTs, scaled to one
+    pass of the whole volume (about 10-20 s of host work)."""
     from oracle import oracle
-    threads = min(os.cpu_count() or 1, 16)
     oracle.set_threads(threads)
     op = oracle.params(prm["nx"], prm["nz"], prm["np_"], prm["gam_min"], prm["gam_max"], prm["interval"],
                        padangle=prm["padangle"], weight=prm["weight"], denoise=1, method=1,
@@ -143,13 +147,13 @@ def cpu_baseline(prm, proj):
     oracle.fft_strided(data, -1)
     t_zfft = time.perf_counter() - t1
     t1 = time.perf_counter()
-    sweeps = 8
+    sweeps = 8 if threads > 1 else 2
     oracle.tv_height(img, prm.get("height", np_), prm.get("center", np_ // 2), niter=sweeps)
     t_tv = (time.perf_counter() - t1) * 40 / sweeps
     t_iter = t_fourier + 2 * t_zfft + t_tv
     sample = t_planes + t_zfft + t_tv * sweeps / 40
-    log("  cpu: setup %.1fs, %d planes %.2fs -> Fourier step %.1fs, z-FFT %.2fs, TV(40) %.1fs -> %.1f s/pass"
-        % (t_setup, nplanes, t_planes, t_fourier, t_zfft, t_tv, t_iter))
+    log("  cpu, %d threads: setup %.1fs, %d planes %.2fs -> Fourier step %.1fs, z-FFT %.2fs, TV(40) %.1fs -> "
+        "%.1f s/pass" % (threads, t_setup, nplanes, t_planes, t_fourier, t_zfft, t_tv, t_iter))
     return {"value": 1.0 / t_iter, "unit": "volume-iter/s", "cores": threads, "kind": "port",
             "sample": "FP64 C restatement of reconstruction.c (-O3, OpenMP over planes, %d threads on %s): "
                       "projector + 10-step NFFT CGNR of %d of %d planes (scaled x%.0f), %d TV sweeps (scaled x%.0f), "
@@ -291,16 +295,22 @@ def probe(prm, proj, copy_mib):
     job.close()
 
 
-def serve(args, precision):
-    """Serving leg: a Poisson trace of jobs drawn from the serving mix (128^3
-    50 %, 256^3 35 %, 512^3 15 %; nhio=20 + ner=40 with TV), submitted at
-    their arrival times to the continuous-batching server (admission by HBM
-    budget, args.serve_slots jobs in flight).  Latency = finish - arrival."""
-    rng = np.random.default_rng(args.serve_seed)
-    names = [m[0] for m in phantom.SERVE_MIX]
-    frac = np.array([m[1] for m in phantom.SERVE_MIX])
-    cls = rng.choice(len(names), args.serve_jobs, p=frac / frac.sum())
-    gaps = rng.exponential(1.0 / args.serve_rate, args.serve_jobs)
+def serve(args, precision, jobs=None, rate=None, seed=None, classes=None):
+    """Serving leg: a Poisson trace of `jobs` jobs at `rate` arrivals per
+    second drawn from the serving mix (128^3 50 %, 256^3 35 %, 512^3 15 %;
+    nhio=20 + ner=40 with TV; `classes` restricts the mix to those classes,
+    renormalised), submitted at their arrival times to the
+    continuous-batching server (admission by HBM budget, args.serve_slots
+    jobs in flight).  Latency = finish - arrival."""
+    jobs = args.serve_jobs if jobs is None else jobs
+    rate = args.serve_rate if rate is None else rate
+    seed = args.serve_seed if seed is None else seed
+    rng = np.random.default_rng(seed)
+    mix = [m for m in phantom.SERVE_MIX if classes is None or m[0] in classes]
+    names = [m[0] for m in mix]
+    frac = np.array([m[1] for m in mix])
+    cls = rng.choice(len(names), jobs, p=frac / frac.sum())
+    gaps = rng.exponential(1.0 / rate, jobs)
     arrivals = np.cumsum(gaps) - gaps[0]
     # a few distinct phantoms per class, cycled over the trace
     inputs = {}
@@ -309,7 +319,7 @@ def serve(args, precision):
         inputs[c] = [phantom.config_workload(names[c], seed=7000 + 10 * c + k)
                      for k in range(1 if names[c] == "s512" else 2)]
     log("  serving: %d jobs (%s), inputs generated in %.1f s" % (
-        args.serve_jobs, ", ".join("%s x%d" % (names[c], int((cls == c).sum())) for c in sorted(set(cls))),
+        jobs, ", ".join("%s x%d" % (names[c], int((cls == c).sum())) for c in sorted(set(cls))),
         time.perf_counter() - t0))
     # start-up calibration of each job class's NFFT sizing (nudim_autotune),
     # as a server does before it takes requests
@@ -329,9 +339,9 @@ def serve(args, precision):
         meta = {}
         start = time.perf_counter()
         k = 0
-        while k < args.serve_jobs or len(reps) < args.serve_jobs:
+        while k < jobs or len(reps) < jobs:
             now = time.perf_counter() - start
-            while k < args.serve_jobs and arrivals[k] <= now:
+            while k < jobs and arrivals[k] <= now:
                 prm, proj = inputs[cls[k]][k % len(inputs[cls[k]])]
                 p = nd.make_params(prm["nx"], prm["nz"], prm["np_"], prm["gam_min"], prm["gam_max"],
                                    prm["interval"], padangle=prm["padangle"], weight=prm["weight"],
@@ -340,7 +350,7 @@ def serve(args, precision):
                 jid = srv.submit(p, proj)
                 meta[jid] = (names[cls[k]], prm["nhio"] + prm["ner"])
                 k += 1
-            wait_ms = 50 if k >= args.serve_jobs else max(1, int(1e3 * (arrivals[k] - now)))
+            wait_ms = 50 if k >= jobs else max(1, int(1e3 * (arrivals[k] - now)))
             got = srv.next(timeout_ms=min(wait_ms, 50))
             if got is not None:
                 rep = got[0]
@@ -355,17 +365,20 @@ def serve(args, precision):
     by = {}
     for j, r in reps.items():
         by.setdefault(meta[j][0], []).append(r["t_end"] - r["t_submit"])
-    out = {"jobs": len(reps), "arrival_rate_jobs_per_s": args.serve_rate, "slots": args.serve_slots,
-           "offered_jobs_per_s": args.serve_jobs / max(arrivals[-1], 1e-9) if args.serve_jobs > 1 else None,
+    by_n = {}
+    for j in reps:
+        by_n[meta[j][0]] = by_n.get(meta[j][0], 0) + 1
+    out = {"jobs": len(reps), "jobs_by_class": by_n, "arrival_rate_jobs_per_s": rate, "slots": args.serve_slots,
+           "offered_jobs_per_s": jobs / max(arrivals[-1], 1e-9) if jobs > 1 else None,
            "achieved_jobs_per_s": len(reps) / span, "volume_iters_per_s": passes / span,
            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
            "mean_latency_s": float(lat.mean()), "makespan_s": span,
            "p50_latency_s_by_class": {k: float(np.percentile(v, 50)) for k, v in sorted(by.items())},
+           "p99_latency_s_by_class": {k: float(np.percentile(v, 99)) for k, v in sorted(by.items())},
            "calibration_s": calib_s, "small_job_cus": args.serve_small_cus,
            "precision": "f32" if precision == nd.FP32 else "f64",
            "policy": "fifo" if args.serve_fifo else "shortest work first, 30 s aging",
-           "mix": "%s; seed %d" % (", ".join("%s %.0f%%" % (n, 100 * f) for n, f in phantom.SERVE_MIX),
-                                   args.serve_seed)}
+           "mix": "%s; seed %d" % (", ".join("%s %.0f%%" % (n, 100 * f / frac.sum()) for n, f in mix), seed)}
     log("  serving %s: %d jobs in %.1f s: p50 %.2f s, p99 %.2f s, %.1f volume-iter/s, achieved %.2f jobs/s "
         "(offered %.2f)" % (out["precision"], len(reps), span, out["p50_latency_s"], out["p99_latency_s"],
                             out["volume_iters_per_s"], out["achieved_jobs_per_s"], out["offered_jobs_per_s"] or 0))
@@ -373,9 +386,14 @@ def serve(args, precision):
 
 
 def batch(args):
-    """Uneven batch: 48 independent volumes (phantom.batch_instances) submitted
-    together to the job server, which co-schedules them (args.batch_slots in
-    flight, least work first); against the same jobs one at a time."""
+    """Uneven batch: 48 independent volumes (phantom.batch_instances: 15+
+    sizes from 64^3 to 384 x 384 x 256, 36 distinct node geometries)
+    submitted together to the job server, which co-schedules them
+    (args.batch_slots in flight, least work first).  Timed three ways: the
+    first run in the process (each geometry's tables -- window and
+    deconvolution tables, gather maps, the three Toeplitz kernel spectra --
+    built on first use), the same again with the geometries resident, and
+    the resident jobs one at a time."""
     t0 = time.perf_counter()
     inst = phantom.batch_instances(args.batch_jobs)
     log("  batch: %d instances generated in %.1f s" % (len(inst), time.perf_counter() - t0))
@@ -387,15 +405,8 @@ def batch(args):
                               precision=nd.FP32)
 
     passes = sum(prm["nhio"] + prm["ner"] for prm, _ in inst)
-    # NFFT sizing of every geometry measured before either timed run
-    t0 = time.perf_counter()
-    for prm, _ in inst:
-        nd.autotune(params(prm))
-    calib_s = time.perf_counter() - t0
-    log("  batch: NFFT sizing of %d geometries measured in %.1f s" % (len({(p["np_"], p["nz"]) for p, _ in inst}),
-                                                                    calib_s))
     res = {}
-    for slots in (1, args.batch_slots):
+    for name, slots in (("cold", args.batch_slots), ("resident", args.batch_slots), ("one_at_a_time", 1)):
         with nd.Server(max_active=slots) as srv:
             t = time.perf_counter()
             for prm, proj in inst:
@@ -405,17 +416,22 @@ def batch(args):
                 if rep["status"] != 0:
                     raise RuntimeError("batch job failed: %s" % rep["error"])
             dt = time.perf_counter() - t
-        res[slots] = dt
-        log("  batch, %d in flight: %d volumes (%d volume-iters) in %.2f s -> %.1f volume-iter/s"
-            % (slots, len(inst), passes, dt, passes / dt))
+        res[name] = dt
+        log("  batch, %s, %d in flight: %d volumes (%d volume-iters) in %.2f s -> %.1f volume-iter/s"
+            % (name, slots, len(inst), passes, dt, passes / dt))
     sizes = {}
     for prm, _ in inst:
         k = "%d^3 np %d" % (prm["nz"], prm["np_"])
         sizes[k] = sizes.get(k, 0) + 1
+    geoms = len({(p["np_"], p["gam_max"], p["interval"]) for p, _ in inst})
     return {"volumes": len(inst), "volume_iters": passes, "slots": args.batch_slots,
-            "seconds": res[args.batch_slots], "volume_iters_per_s": passes / res[args.batch_slots],
-            "seconds_one_at_a_time": res[1], "speedup_over_one_at_a_time": res[1] / res[args.batch_slots],
-            "calibration_s": calib_s, "small_job_cus": args.serve_small_cus, "sizes": sizes}
+            "seconds": res["resident"], "volume_iters_per_s": passes / res["resident"],
+            "seconds_cold": res["cold"], "volume_iters_per_s_cold": passes / res["cold"],
+            "seconds_one_at_a_time": res["one_at_a_time"],
+            "speedup_over_one_at_a_time": res["one_at_a_time"] / res["resident"],
+            "node_geometries": geoms, "sizes": sizes,
+            "schedule": "FP32, ner=30 with TV over the slab; every volume its own job, stream and buffers; "
+                        "least work first, %d in flight" % args.batch_slots}
 
 
 def main():
@@ -429,7 +445,12 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
-    ap.add_argument("--serve-jobs", type=int, default=12, help="jobs in the serving trace (0: no serving leg)")
+    ap.add_argument("--serve-jobs", type=int, default=100, help="jobs in the serving trace (0: no serving legs)")
+    ap.add_argument("--below-jobs", type=int, default=10, help="jobs in the below-capacity serving trace")
+    ap.add_argument("--below-rate", type=float, default=0.3, help="its arrivals per second")
+    ap.add_argument("--fp64-jobs", type=int, default=8,
+                    help="jobs of the 128^3 and 256^3 classes served on both paths (0: none)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch leg")
     ap.add_argument("--serve-rate", type=float, default=4.0, help="mean arrivals per second (Poisson)")
     ap.add_argument("--serve-slots", type=int, default=4, help="jobs in flight at once")
     ap.add_argument("--serve-seed", type=int, default=5)
@@ -474,6 +495,16 @@ def main():
     dt32, r32, _ = timed(j32, args.steps)
     rate32 = args.steps / dt32
     log("  %d passes in %.3f s -> %.2f passes/s" % (args.steps, dt32, rate32))
+    # the same passes replayed as captured hipGraphs (one graph per starting
+    # buffer state and HIO/ER switch: at most six states, so twelve warm-up
+    # passes capture them all)
+    jg = make_job(prm, proj, nd.FP32, use_graph=True)
+    gwarm = max(args.warmup, 12)
+    jg.iterate(gwarm)
+    dtg, rg, _ = timed(jg, args.steps)
+    jg.close()
+    graph = {"passes_per_s": args.steps / dtg, "steps": args.steps, "warmup": gwarm}
+    log("  hipGraph replay: %d passes in %.3f s -> %.2f passes/s" % (args.steps, dtg, args.steps / dtg))
     # per-class HIP-event timing over separate timed runs; the dominant
     # kernel is the class with the most device time per pass
     classes = [args.kclass] if args.kclass else list(KC_CANDIDATES)
@@ -553,7 +584,21 @@ def main():
         log("  ER without TV, %d passes: rel L2 FP32 vs FP64 %.2e, R drift %.2e"
             % (npass, e, acc["er_no_tv"]["max_abs_rfactor_drift"]))
 
+    batch_out = None if args.no_batch else batch(args)
     serving = serve(args, nd.FP32) if args.serve_jobs > 0 else None
+    below = None
+    if args.serve_jobs > 0 and args.below_jobs > 0:
+        below = serve(args, nd.FP32, jobs=args.below_jobs, rate=args.below_rate, seed=args.serve_seed + 1)
+    fp64_serving = None
+    if args.serve_jobs > 0 and args.fp64_jobs > 0:
+        # the same short trace on both paths; the 512^3 class is left out of
+        # it because one such job takes about 90 s on the FP64 plain path
+        small = ("s128", "s256")
+        fp64_serving = {"classes": list(small),
+                        "f32": serve(args, nd.FP32, jobs=args.fp64_jobs, rate=1.0, seed=args.serve_seed + 2,
+                                     classes=small),
+                        "f64": serve(args, nd.FP64_PLAIN, jobs=args.fp64_jobs, rate=1.0, seed=args.serve_seed + 2,
+                                     classes=small)}
     try:
         copy_gbs = device_copy_gbs()
         log("  device copy (hipMemcpy, 1 GiB): %.0f GB/s read + write" % copy_gbs)
@@ -564,7 +609,13 @@ def main():
     cpu = None
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(prm, proj)
+            # the host's cores as the process sees them, at most the 16 that a
+            # one-GPU share of the host holds; and one core
+            nproc = os.cpu_count() or 1
+            cpu = cpu_baseline(prm, proj, min(nproc, 16))
+            cpu["nproc"] = nproc
+            one = cpu_baseline(prm, proj, 1)
+            cpu["one_core"] = {k: one[k] for k in ("value", "unit", "cores", "sample")}
         except Exception as e:  # the baseline is reported, never required
             log("cpu baseline skipped: %s" % e)
 
@@ -583,6 +634,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": workload_name(args.config, prm),
                    "grid": [prm["nz"], prm["np_"], prm["np_"]], "nfft": NFFT_CHOICE or None},
+        "graph": graph,
         "fp64_plain_passes_per_s": rate64,
         "speedup_vs_fp64_plain": (rate32 / rate64) if rate64 else None,
         "accuracy": acc or None,
@@ -594,7 +646,10 @@ def main():
                      "frac_of_device_copy": (kd["achieved_GBs"] / copy_gbs) if copy_gbs else None},
         "kernels": {KC_NAMES.get(k): {kk: round(vv, 4) for kk, vv in v.items() if kk != "name"}
                     for k, v in kern.items()},
+        "batch": batch_out,
         "serving": serving,
+        "serving_below_capacity": below,
+        "serving_fp32_vs_fp64": fp64_serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -15,7 +15,7 @@ INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libnudim_hip.so")
 BIN = os.path.join(HERE, "bin")
 
-HIP_SOURCES = ["ops.hip", "api.hip", "fft2.hip", "serve.hip", "tune.hip"]
+HIP_SOURCES = ["ops.hip", "api.hip", "fft2.hip", "fft2_more.hip", "serve.hip", "tune.hip"]
 CPP_SOURCES = ["geom.cpp"]
 ARCH = os.environ.get("NUDIM_OFFLOAD_ARCH", "gfx950")
 # Packed FP32 instructions are switched off for all device code.

@@ -146,11 +146,14 @@ def test_cgnr_register_fft_sizes(hip, orc, N):
     assert rel(got32, ref) < 1e-4
 
 
-@pytest.mark.parametrize("N", [192, 256, 384, 512, 768])
+@pytest.mark.parametrize("N", [80, 96, 120, 144, 160, 192, 240, 256, 288, 320, 384, 512, 768])
 def test_cgnr_toeplitz_lengths(hip, orc, N):
     """The fused Toeplitz passes at the register-FFT lengths of the full
     workloads, L = 2N = 384 (24 x 16), 512 (32 x 16), 768 (32 x 24, the
     256^3 workload at np = 384), 1024 (32 x 32) and 1536 (48 x 32, np = 768),
+    and of the uneven batch's grids: 160 (20 x 8), 192 (16 x 12), 240
+    (20 x 12), 288 (24 x 12), 320 (20 x 16), 480 (24 x 20), 576 (24 x 24) and
+    640 (32 x 20), with the FP32 NFFT grid FFT at n = 2N on the same lengths,
     against the FP64 NFFT CGNR at the reference's sizes (n = 2 nextpow2(N),
     W = 14), whose parity with the oracle is pinned at N = 32-128 above and
     on one plane here for N <= 256: FP64 Toeplitz to 1e-9, FP32 to 1e-4."""
@@ -443,6 +446,37 @@ def test_end_to_end_small(hip, orc, denoise, method, height):
         # the oracle's own response to an input perturbation of FP32 size
         f32, _ = oracle_passes(orc, op, perturbed(img0, 1e-6, 4), slices, kind, 4)
         assert rel(v4_32, img_ref) < 4 * rel(f32, img_ref) + 1e-4
+
+
+@pytest.mark.parametrize("np_,nz", [(80, 96), (48, 192)])
+def test_end_to_end_z_lengths(hip, orc, np_, nz):
+    """Jobs on the register z transforms of length 96 (16 x 6) and 192
+    (16 x 12), the first with the fused Toeplitz passes at L = 160: initial
+    volume and two ER passes with TV over the height band against the oracle
+    (FP64 plain to 1e-10 before the denoiser's sensitivity sets in, i.e. the
+    initial volume and first pass; FP32 within 1e-4 initially)."""
+    rng = np.random.default_rng(np_ + nz)
+    dens = blob_phantom((nz, np_, np_), 10, rng, sig=(1.5, 3.0), slab=np_ // 2)
+    proj = orc.project_2d1d(dens, np_, -60, 60, 3)
+    kw = dict(padangle=1.0, weight=1, nhio=0, ner=2, denoise=1, method=1, height=np_ // 2, center=np_ // 2)
+    op = orc.params(np_, nz, np_, -60, 60, 3, **kw)
+    slices, kind = orc.assemble(op, proj)
+    img0 = orc.initial(op, slices)
+    img1, r1 = orc.iterate(op, img0, slices, kind, 0)
+    for prec, tol0, tol1 in ((hip.FP64_PLAIN, 1e-11, 1e-8), (hip.FP32, 2e-5, None)):
+        with hip.Reconstruction(hip.make_params(np_, nz, np_, -60, 60, 3, precision=prec, **kw)) as job:
+            job.load_projections(proj)
+            job.initial()
+            v0 = job.volume()
+            r = job.iterate(1)
+            v1 = job.volume()
+        assert rel(v0, img0) < tol0
+        if tol1:
+            assert rel(v1, img1) < tol1 and abs(r[0] - r1) < 1e-12
+        else:
+            assert abs(r[0] - r1) < 1e-5
+            f32, _ = oracle_passes(orc, op, perturbed(img0, 1e-6, 4), slices, kind, 1)
+            assert rel(v1, img1) < 4 * rel(f32, img1) + 1e-4
 
 
 @pytest.mark.parametrize("denoise,method,np_", [(0, 1, 32), (1, 0, 32), (0, 1, 64), (1, 0, 64)])
