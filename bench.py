@@ -339,8 +339,12 @@ def serve(args, precision, jobs=None, rate=None, seed=None, classes=None):
         meta = {}
         start = time.perf_counter()
         k = 0
+        last_log = 0.0
         while k < jobs or len(reps) < jobs:
             now = time.perf_counter() - start
+            if now - last_log >= 20.0:
+                last_log = now
+                log("    %.0f s: %d of %d jobs submitted, %d done" % (now, k, jobs, len(reps)))
             while k < jobs and arrivals[k] <= now:
                 prm, proj = inputs[cls[k]][k % len(inputs[cls[k]])]
                 p = nd.make_params(prm["nx"], prm["nz"], prm["np_"], prm["gam_min"], prm["gam_max"],
@@ -382,6 +386,43 @@ def serve(args, precision, jobs=None, rate=None, seed=None, classes=None):
     log("  serving %s: %d jobs in %.1f s: p50 %.2f s, p99 %.2f s, %.1f volume-iter/s, achieved %.2f jobs/s "
         "(offered %.2f)" % (out["precision"], len(reps), span, out["p50_latency_s"], out["p99_latency_s"],
                             out["volume_iters_per_s"], out["achieved_jobs_per_s"], out["offered_jobs_per_s"] or 0))
+    return out
+
+
+def small_configs():
+    """The 128^3 configurations: c1 (noise-free ER passes without the
+    denoiser) and c2 (SNR 1, HIO + ER with TV or DIFFUSION over the height
+    band), passes/s of the FP32 form and the FP64 plain path, and the
+    denoiser's device time per pass (HIP events: TV sweeps, or the diffusion
+    call with each z plane held in LDS)."""
+    out = {}
+    for cfg, method, kc in (("c1", None, 0), ("c2", nd.TV, nd.KC_TV), ("c2", nd.DIFFUSION, nd.KC_DIFFUSION)):
+        prm, proj = phantom.config_workload(cfg)
+        name = cfg if method is None else "%s_%s" % (cfg, "tv" if method == nd.TV else "diffusion")
+        res = {}
+        for prec in (nd.FP32, nd.FP64_PLAIN):
+            p = nd.make_params(prm["nx"], prm["nz"], prm["np_"], prm["gam_min"], prm["gam_max"], prm["interval"],
+                               padangle=prm["padangle"], weight=prm["weight"], nhio=prm["nhio"], ner=prm["ner"],
+                               denoise=method is not None, method=method if method is not None else nd.TV,
+                               height=prm.get("height"), center=prm.get("center"), precision=prec)
+            with nd.Reconstruction(p) as job:
+                job.load_projections(proj)
+                job.initial()
+                job.iterate(2)
+                steps = 20 if prec == nd.FP32 else 5
+                dt, _, _ = timed(job, steps)
+                r = {"passes_per_s": steps / dt}
+                if kc:
+                    _, _, (kms, kl, kb) = timed(job, steps, kc)
+                    if kl:
+                        r["denoiser_ms_per_pass"] = kms / steps
+                        r["denoiser_algorithmic_GBs"] = kb / (kms * 1e-3) / 1e9
+                res["f32" if prec == nd.FP32 else "f64"] = r
+        res["speedup"] = res["f32"]["passes_per_s"] / res["f64"]["passes_per_s"]
+        out[name] = res
+        log("  %s: FP32 %.1f passes/s, FP64 plain %.1f passes/s (%.1fx)%s" % (
+            name, res["f32"]["passes_per_s"], res["f64"]["passes_per_s"], res["speedup"],
+            "; denoiser %.3f ms/pass FP32" % res["f32"]["denoiser_ms_per_pass"] if kc else ""))
     return out
 
 
@@ -584,6 +625,7 @@ def main():
         log("  ER without TV, %d passes: rel L2 FP32 vs FP64 %.2e, R drift %.2e"
             % (npass, e, acc["er_no_tv"]["max_abs_rfactor_drift"]))
 
+    configs_128 = small_configs()
     batch_out = None if args.no_batch else batch(args)
     serving = serve(args, nd.FP32) if args.serve_jobs > 0 else None
     below = None
@@ -646,6 +688,7 @@ def main():
                      "frac_of_device_copy": (kd["achieved_GBs"] / copy_gbs) if copy_gbs else None},
         "kernels": {KC_NAMES.get(k): {kk: round(vv, 4) for kk, vv in v.items() if kk != "name"}
                     for k, v in kern.items()},
+        "configs_128": configs_128,
         "batch": batch_out,
         "serving": serving,
         "serving_below_capacity": below,

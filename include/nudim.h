@@ -119,7 +119,8 @@ int nudim_autotune(const nudim_params *p, int *n, int *W);
  * 4 other line FFT passes, 5 TV sweeps, 6 z-axis transforms, 7 Toeplitz row
  * forward pass (with the CG vector update), 8 Toeplitz row inverse pass
  * (with the <p, q> row sums), 9 two-input column pass of the projected CG
- * start, 10 CG vector update z -= alpha q.  Selecting a
+ * start, 10 CG vector update z -= alpha q, 11 anisotropic diffusion (one
+ * timed launch group per call: all 80 sweeps).  Selecting a
  * class resets the counters.  (No reference counterpart: the reference only
  * prints total wall time, src/reconstruction-parallel.c:881-883.) */
 int nudim_profile(nudim_ctx *ctx, int kernel_class);

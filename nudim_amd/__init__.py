@@ -22,7 +22,7 @@ FP64_PLAIN, FP32 = 0, 1
 DIFFUSION, TV = 0, 1
 CG_AUTO, CG_NFFT, CG_TOEPLITZ = 0, 1, 2
 KC_TOEPLITZ_COL, KC_INTERP, KC_GATHER, KC_LINE_FFT, KC_TV, KC_ZFFT = 1, 2, 3, 4, 5, 6
-KC_TOEPLITZ_ROWF, KC_TOEPLITZ_ROWI, KC_TOEPLITZ_COL2, KC_CG_UPDATE = 7, 8, 9, 10
+KC_TOEPLITZ_ROWF, KC_TOEPLITZ_ROWI, KC_TOEPLITZ_COL2, KC_CG_UPDATE, KC_DIFFUSION = 7, 8, 9, 10, 11
 
 _LIB = None
 

@@ -536,20 +536,10 @@ struct Job : JobBase {
   }
 
   void diffusion(T *u) {
-    const int niter = 80;
-    double lam = (double)(prm.diffusion_lambda > 0 ? prm.diffusion_lambda : 0.25f), kappa = (double)0.005f;
-    int ny = N, nx = N;
-    int taumax = 2 * (niter - 1) + (ny - 1) + (nx - 1);
-    for (int tau = 0; tau <= taumax; tau++) {
-      int smin = tau - (ny - 1) - (nx - 1);
-      smin = smin > 0 ? (smin + 1) / 2 : 0;
-      int smax = std::min(niter - 1, tau / 2);
-      if (smax < smin) continue;
-      long long cnt = (long long)(smax - smin + 1) * ny * P;
-      hipLaunchKernelGGL(k_diffusion_tau<T>, dim3(cdiv(cnt, 256)), dim3(256), 0, st, u, ny, nx, P, niter, tau, lam,
-                         kappa * kappa);
-    }
-    NUDIM_HIP(hipGetLastError());
+    // aniso_diffusion_2d(img, 0.25f, 0.005f, 80, ...), reconstruction.c:627
+    // (reconstruction-parallel.c:764 passes 0.4f)
+    const double lam = (double)(prm.diffusion_lambda > 0 ? prm.diffusion_lambda : 0.25f), kappa = (double)0.005f;
+    diffusion_sweeps<T>(u, N, N, P, 80, lam, kappa * kappa, st);
   }
 
   void copy_to_host_ref(const T *u, double *out) {
@@ -876,7 +866,7 @@ extern "C" int nudim_set_dump(nudim_ctx *c, int nn, double *before, double *afte
 }
 
 extern "C" int nudim_profile(nudim_ctx *c, int kernel_class) {
-  if (!c || kernel_class < 0 || kernel_class > KC_CG_UPDATE) return fail(NUDIM_EINVAL, "invalid profile request");
+  if (!c || kernel_class < 0 || kernel_class > KC_DIFFUSION) return fail(NUDIM_EINVAL, "invalid profile request");
   c->prof.reset(kernel_class);
   return NUDIM_OK;
 }
@@ -1250,18 +1240,8 @@ extern "C" int nudim_tv_height_noalloc(int precision, double *d_u, int nz, int n
 template <typename T>
 static void diffusion_parity(double *u, int nz, int ny, int nx, float lambda, float kappa, int niter) {
   VolTmp<T> v(u, nz, ny, nx);
-  double lam = (double)lambda, k = (double)kappa;
-  int taumax = 2 * (niter - 1) + (ny - 1) + (nx - 1);
-  for (int tau = 0; niter > 0 && tau <= taumax; tau++) {
-    int smin = tau - (ny - 1) - (nx - 1);
-    smin = smin > 0 ? (smin + 1) / 2 : 0;
-    int smax = std::min(niter - 1, tau / 2);
-    if (smax < smin) continue;
-    long long cnt = (long long)(smax - smin + 1) * ny * nz;
-    hipLaunchKernelGGL(k_diffusion_tau<T>, dim3(cdiv(cnt, 256)), dim3(256), 0, v.ts.s, v.u.template as<T>(), ny, nx, nz,
-                       niter, tau, lam, k * k);
-  }
-  NUDIM_HIP(hipGetLastError());
+  const double lam = (double)lambda, k = (double)kappa;
+  diffusion_sweeps<T>(v.u.template as<T>(), ny, nx, nz, niter, lam, k * k, v.ts.s);
   v.out(u);
 }
 

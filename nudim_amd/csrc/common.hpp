@@ -111,6 +111,7 @@ enum KernelClass {
   KC_TOEPLITZ_ROWI = 8, // Toeplitz row inverse pass with the <p, q> row sums
   KC_TOEPLITZ_COL2 = 9, // two-input column pass of the projected CG start
   KC_CG_UPDATE = 10,    // CG vector update z -= alpha q with |z|^2
+  KC_DIFFUSION = 11,    // anisotropic diffusion (all sweeps of one call)
 };
 
 // Records an event pair around every launch of one kernel class and

@@ -155,6 +155,13 @@ template <typename T>
 T *tv_sweeps(T *u0, T *u1, T *g0, T *g1, int ny, int nx, int nz, int ylo, int yhi, float lam, int niter,
              double *part_g, double *part_a, double *sums, int nb, hipStream_t st);
 
+// aniso_diffusion_2d (denoise.c:320-378) on a [ny][nx][nz] volume: niter
+// in-place Gauss-Seidel sweeps of every z plane, each plane in LDS when it
+// fits (k_diffusion_plane), else one launch per hyperplane (k_diffusion_tau);
+// both give the same result bit for bit
+template <typename T>
+void diffusion_sweeps(T *u, int ny, int nx, int nz, int niter, double lam, double ksq, hipStream_t st);
+
 // per-plane reduction helpers
 struct Reducer {
   DevBuf part, part2;  // [kRedBlocks][P] doubles

@@ -957,6 +957,35 @@ T *tv_sweeps(T *u0, T *u1, T *g0, T *g1, int ny, int nx, int nz, int ylo, int yh
   return u0;
 }
 
+template <typename T>
+void diffusion_sweeps(T *u, int ny, int nx, int nz, int niter, double lam, double ksq, hipStream_t st) {
+  if (niter <= 0) return;
+  // algorithmic: read and write the volume once per sweep
+  ProfScope ps(KC_DIFFUSION, st, 2.0 * niter * (double)ny * nx * nz * sizeof(T));
+  const size_t lds = (size_t)ny * nx * sizeof(T);
+  const char *e = getenv("NUDIM_DIFFUSION_LDS");  // "0": the hyperplane launches only
+  const bool lds_on = !(e && std::string(e) == "0");
+  if (lds_on && lds <= (size_t)160 * 1024) {
+    hipLaunchKernelGGL(k_diffusion_plane<T>, dim3(nz), dim3(1024), lds, st, u, ny, nx, nz, niter, lam, ksq);
+    NUDIM_HIP(hipGetLastError());
+    return;
+  }
+  const int taumax = 2 * (niter - 1) + (ny - 1) + (nx - 1);
+  for (int tau = 0; tau <= taumax; tau++) {
+    int smin = tau - (ny - 1) - (nx - 1);
+    smin = smin > 0 ? (smin + 1) / 2 : 0;
+    const int smax = std::min(niter - 1, tau / 2);
+    if (smax < smin) continue;
+    const long long cnt = (long long)(smax - smin + 1) * ny * nz;
+    hipLaunchKernelGGL(k_diffusion_tau<T>, dim3(cdiv(cnt, 256)), dim3(256), 0, st, u, ny, nx, nz, niter, tau, lam,
+                       ksq);
+  }
+  NUDIM_HIP(hipGetLastError());
+}
+
+template void diffusion_sweeps<float>(float *, int, int, int, int, double, double, hipStream_t);
+template void diffusion_sweeps<double>(double *, int, int, int, int, double, double, hipStream_t);
+
 template float *tv_sweeps<float>(float *, float *, float *, float *, int, int, int, int, int, float, int, double *,
                                    double *, double *, int, hipStream_t);
 template double *tv_sweeps<double>(double *, double *, double *, double *, int, int, int, int, int, float, int,

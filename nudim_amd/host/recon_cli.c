@@ -253,6 +253,12 @@ int nudim_recon_main(int argc, char *argv[], int parallel) {
   }
   const char *prec = getenv("NUDIM_PRECISION");
   prm.precision = (prec && (strcmp(prec, "fp32") == 0 || strcmp(prec, "FP32") == 0)) ? NUDIM_FP32 : NUDIM_FP64_PLAIN;
+  /* without density compensation the polar normal equations are badly
+     scaled (node density ~ 1/r) and the FP32 CGNR ends ~5e-2 from the FP64
+     result after its ten steps; the FP64 path is accurate either way */
+  if (prm.precision == NUDIM_FP32 && !config_weight)
+    fprintf(stderr, "warning: NUDIM_PRECISION=fp32 without --weight is accurate to about 5e-2 only; "
+                    "add --weight or use the FP64 path\n");
   const char *dev = getenv("NUDIM_DEVICE");
   prm.device = dev ? atoi(dev) : 0;
 

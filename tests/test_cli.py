@@ -65,6 +65,18 @@ def test_help_and_unknown_option(binary, tmp_path):
     assert r.returncode == 1
 
 
+def test_fp32_without_weight_warns(binary, tmp_path):
+    """The FP32 form without --weight is accurate to ~5e-2 only: the CLI
+    says so on stderr before it starts (whatever happens after: here there
+    are no projection files)."""
+    args = ["--prefix=prj", "--nx=16", "--nz=16", "-p", "16", "--nhio=0", "--ner=1", "-t", "30",
+            "--padangle=10", "--max=60", "--min=-60"]
+    r = run(args, tmp_path, env={"NUDIM_PRECISION": "fp32"})
+    assert "without --weight is accurate to about 5e-2" in r.stderr
+    r = run(args + ["--weight"], tmp_path, env={"NUDIM_PRECISION": "fp32"})
+    assert "without --weight" not in r.stderr
+
+
 def test_missing_projection_file(binary, tmp_path):
     (tmp_path / "files").mkdir()
     r = run(["--prefix=prj", "--nx=16", "--nz=16", "-p", "16", "--nhio=0", "--ner=1", "-t", "30",

@@ -162,3 +162,31 @@ def test_diffusion_conductance_closed_form(hip, orc, precision):
     v = blob_phantom((12, 12, 12), 4, rng, sig=(1.5, 3.0)) * 0.01
     tol = 1e-12 if precision == 0 else 1e-5
     assert rel(hip.diffusion2d(v, niter=5, precision=precision), orc.diffusion2d(v, niter=5)) < tol
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_diffusion_plane_in_lds_equals_hyperplane_launches(hip, monkeypatch, precision):
+    """The diffusion with each z plane held in LDS (one launch for all sweeps)
+    equals the one-launch-per-hyperplane form bit for bit, on the parity
+    entry point and inside a DIFFUSION job (np = nz = 64, HIO and ER)."""
+    rng = np.random.default_rng(17)
+    v = blob_phantom((24, 40, 24), 6, rng, sig=(1.5, 3.0)) * 0.01 + 1e-3 * rng.standard_normal((24, 40, 24))
+    a = hip.diffusion2d(v, niter=12, precision=precision)
+    monkeypatch.setenv("NUDIM_DIFFUSION_LDS", "0")
+    b = hip.diffusion2d(v, niter=12, precision=precision)
+    monkeypatch.delenv("NUDIM_DIFFUSION_LDS")
+    assert np.array_equal(a, b) and not np.array_equal(a, v)
+    np_ = nz = 64
+    dens = blob_phantom((nz, np_, np_), 10, rng, sig=(1.5, 3.0), slab=np_ // 2)
+    proj = hip.project_2d1d(dens, np_, -60, 60, 5)
+    out = []
+    for env in (None, "0"):
+        if env:
+            monkeypatch.setenv("NUDIM_DIFFUSION_LDS", env)
+        p = hip.make_params(np_, nz, np_, -60, 60, 5, padangle=1.0, weight=1, nhio=1, ner=1, denoise=True,
+                            method=hip.DIFFUSION, precision=precision)
+        with hip.Reconstruction(p) as job:
+            job.load_projections(proj)
+            job.initial()
+            out.append((job.iterate(2), job.volume()))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
