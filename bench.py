@@ -273,26 +273,38 @@ def device_copy_gbs(mib=1024, reps=5):
 
 
 def probe(prm, proj, copy_mib):
-    """One FP32 pass and one device-to-device hipMemcpy of copy_mib MiB (the
-    runtime's blit kernel, plain loads and stores: the scale for the
-    FETCH_SIZE / WRITE_SIZE counters): the program the counter runs profile."""
+    """One FP32 pass, one FP64 plain pass, one device-to-device hipMemcpy of
+    copy_mib MiB (the runtime's blit kernel) and one pass each of the
+    library's read-only and copy streaming kernels over copy_mib MiB (plain
+    16-byte loads and stores): the program the counter runs profile, the
+    three known-size kernels being the scale of FETCH_SIZE / WRITE_SIZE."""
     import ctypes
-    import torch
-    job = make_job(prm, proj, nd.FP32)
-    job.iterate(1)
-    job.synchronize()
+    for prec in (nd.FP32, nd.FP64_PLAIN):
+        job = make_job(prm, proj, prec)
+        job.iterate(1)
+        job.synchronize()
+        job.close()
     nbytes = copy_mib << 20
-    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    torch.cuda.synchronize()
-    hip = ctypes.CDLL("libamdhip64.so")
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
     hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
-    rc = hip.hipMemcpy(ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(a.data_ptr()), nbytes, 3)  # device to device
-    torch.cuda.synchronize()
-    if rc != 0:
-        raise RuntimeError("hipMemcpy failed: %d" % rc)
-    log("probe: one FP32 pass; hipMemcpy device to device of %d MiB" % copy_mib)
-    job.close()
+    hip.hipMemset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), nbytes) or hip.hipMalloc(ctypes.byref(b), nbytes):
+        raise RuntimeError("hipMalloc failed")
+    try:
+        if hip.hipMemset(a, 1, nbytes) or hip.hipDeviceSynchronize():
+            raise RuntimeError("hipMemset failed")
+        if hip.hipMemcpy(b, a, nbytes, 3) or hip.hipDeviceSynchronize():  # device to device
+            raise RuntimeError("hipMemcpy failed")
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+    rd = nd.stream_probe(nbytes, 0, reps=1)
+    cp = nd.stream_probe(nbytes, 1, reps=1)
+    log("probe: one FP32 pass, one FP64 plain pass; hipMemcpy device to device of %d MiB; k_stream_read %.0f GB/s, "
+        "k_stream_copy %.0f GB/s over %d MiB" % (copy_mib, rd, cp, copy_mib))
 
 
 def serve(args, precision, jobs=None, rate=None, seed=None, classes=None):
@@ -492,6 +504,7 @@ def main():
     ap.add_argument("--fp64-jobs", type=int, default=8,
                     help="jobs of the 128^3 and 256^3 classes served on both paths (0: none)")
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch leg")
+    ap.add_argument("--no-small", action="store_true", help="skip the 128^3 configurations")
     ap.add_argument("--serve-rate", type=float, default=4.0, help="mean arrivals per second (Poisson)")
     ap.add_argument("--serve-slots", type=int, default=4, help="jobs in flight at once")
     ap.add_argument("--serve-seed", type=int, default=5)
@@ -625,7 +638,7 @@ def main():
         log("  ER without TV, %d passes: rel L2 FP32 vs FP64 %.2e, R drift %.2e"
             % (npass, e, acc["er_no_tv"]["max_abs_rfactor_drift"]))
 
-    configs_128 = small_configs()
+    configs_128 = None if args.no_small else small_configs()
     batch_out = None if args.no_batch else batch(args)
     serving = serve(args, nd.FP32) if args.serve_jobs > 0 else None
     below = None
@@ -647,6 +660,9 @@ def main():
     except Exception as e:  # reported context, never required
         copy_gbs = None
         log("device copy measurement skipped: %s" % e)
+    stream = {"read_GBs": nd.stream_probe(1 << 30, 0), "copy_GBs": nd.stream_probe(1 << 30, 1)}
+    log("  streaming kernels over 1 GiB: read only %.0f GB/s, copy %.0f GB/s read + write"
+        % (stream["read_GBs"], stream["copy_GBs"]))
 
     cpu = None
     if not args.no_cpu:
@@ -684,7 +700,7 @@ def main():
                      "unit": "GB/s", "frac": kd["achieved_GBs"] / HBM_PEAK_GBS,
                      "traffic": traffic_from_profiles(dom) if args.config == "c3" else None,
                      "algorithmic_bytes": kd["bytes_per_launch"],
-                     "avg_us": kd["avg_us"], "device_copy_GBs": copy_gbs,
+                     "avg_us": kd["avg_us"], "device_copy_GBs": copy_gbs, "stream_kernels_GBs": stream,
                      "frac_of_device_copy": (kd["achieved_GBs"] / copy_gbs) if copy_gbs else None},
         "kernels": {KC_NAMES.get(k): {kk: round(vv, 4) for kk, vv in v.items() if kk != "name"}
                     for k, v in kern.items()},

@@ -128,6 +128,11 @@ int nudim_profile(nudim_ctx *ctx, int kernel_class);
 int nudim_profile_read(nudim_ctx *ctx, double *ms, long *launches, double *bytes);
 /* wait for all device work of the context */
 int nudim_synchronize(nudim_ctx *ctx);
+/* bandwidth probe (no reference counterpart; measurement only): the best of
+ * `reps` passes of a plain 16-byte-vector kernel over `bytes` of device
+ * memory on `device`, kind 0 read only, kind 1 copy; gbs = bytes moved
+ * (read + written) per second / 1e9 */
+int nudim_stream_probe(int device, size_t bytes, int kind, int reps, double *gbs);
 
 /* ---- parity entry points (stateless; host arrays in the reference layouts) ---- */
 

@@ -48,7 +48,7 @@ EXPORTS = [
     "nudim_last_error", "nudim_default_params", "nudim_create", "nudim_destroy", "nudim_sizes",
     "nudim_load_projections", "nudim_load_samples", "nudim_initial", "nudim_iterate",
     "nudim_get_volume", "nudim_set_volume", "nudim_set_dump", "nudim_device_bytes",
-    "nudim_estimate_bytes", "nudim_autotune", "nudim_synchronize", "nudim_profile", "nudim_profile_read", "nudim_nfft_trafo", "nudim_nfft_adjoint",
+    "nudim_estimate_bytes", "nudim_autotune", "nudim_synchronize", "nudim_stream_probe", "nudim_profile", "nudim_profile_read", "nudim_nfft_trafo", "nudim_nfft_adjoint",
     "nudim_cgnr", "nudim_zfft", "nudim_tv", "nudim_tv_height", "nudim_tv_height_noalloc",
     "nudim_diffusion2d",
     "nudim_constraint", "nudim_projection_ft", "nudim_project_2d1d",
@@ -106,6 +106,7 @@ def _declare(L):
         "nudim_estimate_bytes": (i, [C.POINTER(Params), C.POINTER(C.c_size_t)]),
         "nudim_autotune": (i, [C.POINTER(Params), C.POINTER(i), C.POINTER(i)]),
         "nudim_synchronize": (i, [v]),
+        "nudim_stream_probe": (i, [i, C.c_size_t, i, i, C.POINTER(d)]),
         "nudim_profile": (i, [v, i]),
         "nudim_profile_read": (i, [v, C.POINTER(d), C.POINTER(C.c_long), C.POINTER(d)]),
         "nudim_nfft_trafo": (i, [i, i, i, dp, i, i, i, dp, dp]),
@@ -264,6 +265,14 @@ class Reconstruction:
 
     def synchronize(self):
         _check(lib().nudim_synchronize(self._h))
+
+
+def stream_probe(nbytes, kind, reps=5, device=0):
+    """GB/s of the library's plain streaming kernel over nbytes of device
+    memory: kind 0 read only, 1 copy (bytes read + written)"""
+    g = C.c_double()
+    _check(lib().nudim_stream_probe(device, nbytes, kind, reps, C.byref(g)))
+    return g.value
 
 
 # ---- stateless parity entry points ------------------------------------------

@@ -1150,6 +1150,67 @@ extern "C" int nudim_tv_height(int precision, double *u, int nz, int ny, int nx,
 }
 
 // ---------------------------------------------------------------------------
+// bandwidth probes: plain 16-byte vector loads and stores over a buffer of
+// known size (the reference point of achieved GB/s and of the HBM traffic
+// counters)
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(256) k_stream_read(const float4 *__restrict__ a, long long n4, float *__restrict__ out) {
+  float s = 0.f;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (long long)gridDim.x * blockDim.x) {
+    const float4 v = a[i];
+    s += v.x + v.y + v.z + v.w;
+  }
+  __shared__ float red[256];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(256) k_stream_copy(const float4 *__restrict__ a, float4 *__restrict__ b, long long n4) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (long long)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
+extern "C" int nudim_stream_probe(int device, size_t bytes, int kind, int reps, double *gbs) {
+  API_BEGIN
+  NUDIM_CHECK(gbs && bytes >= 16 && reps >= 1 && (kind == 0 || kind == 1), "invalid argument");
+  NUDIM_HIP(hipSetDevice(device));
+  const long long n4 = (long long)(bytes / 16);
+  DevBuf a((size_t)n4 * 16), b(kind == 1 ? (size_t)n4 * 16 : 0), out(2048 * 4 * 8);
+  TmpStream ts;
+  NUDIM_HIP(hipMemsetAsync(a.p, 0, (size_t)n4 * 16, ts.s));
+  hipEvent_t e0, e1;
+  NUDIM_HIP(hipEventCreate(&e0));
+  NUDIM_HIP(hipEventCreate(&e1));
+  const unsigned blocks = 256 * 8 * 4;  // 32 blocks of 256 threads per CU
+  double best = 0;
+  for (int r = 0; r < reps; r++) {
+    NUDIM_HIP(hipEventRecord(e0, ts.s));
+    if (kind == 0)
+      hipLaunchKernelGGL(k_stream_read, dim3(blocks), dim3(256), 0, ts.s, a.as<float4>(), n4, out.as<float>());
+    else
+      hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(256), 0, ts.s, a.as<float4>(), b.as<float4>(), n4);
+    NUDIM_HIP(hipGetLastError());
+    NUDIM_HIP(hipEventRecord(e1, ts.s));
+    NUDIM_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    NUDIM_HIP(hipEventElapsedTime(&ms, e0, e1));
+    const double moved = (kind == 0 ? 1.0 : 2.0) * (double)n4 * 16;
+    best = std::max(best, moved / (ms * 1e-3) / 1e9);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *gbs = best;
+  return NUDIM_OK;
+  API_END
+}
+
+// ---------------------------------------------------------------------------
 // TV-height on caller device memory with a file-scope scratch array (no
 // device allocation): for callers that cannot call hipMalloc
 // ---------------------------------------------------------------------------

@@ -70,6 +70,10 @@ struct NfftGeom {
   LineFft<T> fft;
   DevBuf c0, c1, psi0, psi1, dec;
   DevBuf cell_ptr, ent;  // per-cell gather map (any plane count)
+  // the nodes in the order the interpolation visits them: by 8 x 8-cell
+  // tile of their first tap, tiles in Morton (Z) order, so that the nodes of
+  // a block, and the blocks of an XCD, read the same grid cells from L2
+  DevBuf perm;
   // tiled gather map (built when n % kGatherTile == 0; used when P % 64 == 0)
   bool has_tiles = false;
   int nchunk = 0, ncomb = 0, npart = 0;

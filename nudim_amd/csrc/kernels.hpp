@@ -14,6 +14,13 @@ namespace nudim {
 
 constexpr int kThreads = 256;
 constexpr int kGatherTile = 8;  // cells per tile edge of the tiled gather
+// nodes per chunk of the tiled gather (LDS rows of its staged node values);
+// 32-node chunks in FP64 (four blocks per CU instead of two) measured slower
+// at 256^3 (3.71 against 3.56 ms per adjoint with its combine kernel)
+template <typename T>
+struct GatherChunk {
+  static constexpr int value = 64;
+};
 constexpr int kRedBlocks = 240;  // partial-sum rows of the per-plane reductions
 
 // ---------------------------------------------------------------------------
@@ -204,15 +211,16 @@ struct GEnt {
   T w;
 };
 
-// Tiled gather, one block per node chunk (at most 64 nodes) of a TS x TS
-// tile and 64 planes.  The block stages the chunk's node values in LDS,
-// [slot][64 planes] (65 rows: row 64 is a spare), so each node value is read once per tile it meets
+// Tiled gather, one block per node chunk (at most KN nodes) of a TS x TS
+// tile and 64 planes.  The block stages the chunk's
+// node values in LDS, [slot][64 planes] (KN + 1 rows: row KN is a spare), so
+// each node value is read once per tile it meets
 // instead of once per tap; lane = plane; wave w owns the tile's cells
 // w*CPW .. w*CPW+CPW-1 (row-major in the tile) and sums each cell's entries
 // of the chunk in node order.  The only chunk of a tile writes the grid; the
 // chunks of a tile with several write partial sums part[slot][cell][p] for
 // k_gather_combine.
-template <typename T, int TS>
+template <typename T, int TS, int KN>
 __global__ void __launch_bounds__(kThreads)
 k_gather_chunk(const cx<T> *__restrict__ s, cx<T> *__restrict__ g, cx<T> *__restrict__ part,
                const int *__restrict__ chunk_tile, const int *__restrict__ chunk_part,
@@ -226,20 +234,20 @@ k_gather_chunk(const cx<T> *__restrict__ s, cx<T> *__restrict__ g, cx<T> *__rest
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int pbase = blockIdx.y * 64;
-  const int a = chunk_node[c], nn = chunk_node[c + 1] - a;  // 1 <= nn <= 64
+  const int a = chunk_node[c], nn = chunk_node[c + 1] - a;  // 1 <= nn <= KN
   {
-    // wave w stages slots w, w+4, ..., w+60: sixteen unconditional loads in
-    // flight (slot clamped into the chunk) and sixteen unconditional stores
-    // (slots past the chunk go to the spare row 64), so that no load is
+    // wave w stages slots w, w+4, ..., w+KN-4: KN/4 unconditional loads in
+    // flight (slot clamped into the chunk) and KN/4 unconditional stores
+    // (slots past the chunk go to the spare row KN), so that no load is
     // sunk into a branch of its own
     const cx<T> *sp = s + pbase + lane;
     const int myn = node[a + min(lane, nn - 1)];
-    cx<T> v[16];
+    cx<T> v[KN / 4];
 #pragma unroll
-    for (int j = 0; j < 16; j++)
+    for (int j = 0; j < KN / 4; j++)
       v[j] = sp[(size_t)__builtin_amdgcn_readlane(myn, min(wv + 4 * j, nn - 1)) * P];
 #pragma unroll
-    for (int j = 0; j < 16; j++) lds[(wv + 4 * j < nn ? wv + 4 * j : 64) * 64 + lane] = v[j];
+    for (int j = 0; j < KN / 4; j++) lds[(wv + 4 * j < nn ? wv + 4 * j : KN) * 64 + lane] = v[j];
   }
   __syncthreads();
   const int tile = chunk_tile[c], ps = chunk_part[c];
@@ -258,9 +266,23 @@ k_gather_chunk(const cx<T> *__restrict__ s, cx<T> *__restrict__ g, cx<T> *__rest
 #pragma unroll
     for (int k = 0; k < CPW; k++) {
       const int ka = max(cb[k], base), kb = min(cb[k + 1], base + 64);
-      for (int e = ka; e < kb; e++) {
-        const int j = e - base;
-        acc[k] = acc[k] + lane_bcast(q.w, j) * lds[__builtin_amdgcn_readlane(q.s, j) * 64 + lane];
+      // four entries at a time: their LDS reads in flight together (the
+      // slot of an entry past kb is clamped to the last one and not added;
+      // stepping all the wave's cells together instead, entry i of each
+      // per step, measured 2.5x slower: the steps pad every cell to the
+      // longest one)
+      for (int e = ka; e < kb; e += 4) {
+        cx<T> v[4];
+        T w[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int j = min(e + u, kb - 1) - base;
+          w[u] = lane_bcast(q.w, j);
+          v[u] = lds[__builtin_amdgcn_readlane(q.s, j) * 64 + lane];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+          if (e + u < kb) acc[k] = acc[k] + w[u] * v[u];
       }
     }
   }

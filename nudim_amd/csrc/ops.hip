@@ -184,7 +184,7 @@ void LineFft<T>::run(const cx<T> *in, cx<T> *out, int nlines, const LineIO &li, 
 
 template <typename T>
 size_t NfftGeom<T>::bytes() const {
-  return fft.tw.bytes + c0.bytes + c1.bytes + psi0.bytes + psi1.bytes + dec.bytes + cell_ptr.bytes + ent.bytes +
+  return fft.tw.bytes + c0.bytes + c1.bytes + psi0.bytes + psi1.bytes + dec.bytes + cell_ptr.bytes + ent.bytes + perm.bytes +
          chunk_tile.bytes + chunk_part.bytes + chunk_node.bytes + tnode.bytes + chunk_cell.bytes + tent.bytes +
          comb.bytes;
 }
@@ -209,11 +209,33 @@ static std::shared_ptr<NfftGeom<T>> build_nfft_geometry(int N, int n, int W, con
   upload(g->dec, t.dec);
   upload(g->cell_ptr, t.cell_ptr);
   upload(g->ent, t.ent);
+  {
+    auto wrap = [n](int c) { return ((c % n) + n) % n; };
+    // Morton (Z) order of the 8 x 8-cell tiles, row-major inside a tile
+    auto spread = [](unsigned v) {
+      unsigned long long x = v & 0xFFFFu;
+      x = (x | (x << 8)) & 0x00FF00FFull;
+      x = (x | (x << 4)) & 0x0F0F0F0Full;
+      x = (x | (x << 2)) & 0x33333333ull;
+      x = (x | (x << 1)) & 0x55555555ull;
+      return x;
+    };
+    std::vector<long long> key(M);
+    for (int m = 0; m < M; m++) {
+      const int r = wrap(t.c0[m]), c = wrap(t.c1[m]);
+      const long long z = (long long)(spread((unsigned)(r / 8)) << 1 | spread((unsigned)(c / 8)));
+      key[m] = (z * 8 + r % 8) * 8 + c % 8;
+    }
+    std::vector<int> perm(M);
+    for (int m = 0; m < M; m++) perm[m] = m;
+    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return key[a] < key[b]; });
+    upload(g->perm, perm);
+  }
   g->has_tiles = n % kGatherTile == 0;
   if (g->has_tiles) {
-    // node values of a chunk staged as [slot][64 planes]: 64 nodes per chunk
+    // node values of a chunk staged as [slot][64 planes]
     GatherTiles gt;
-    build_gather_tiles(t, kGatherTile, 64, &gt);
+    build_gather_tiles(t, kGatherTile, GatherChunk<T>::value, &gt);
     g->nchunk = (int)gt.chunk_tile.size();
     g->ncomb = (int)gt.comb.size() / 3;
     g->npart = gt.npart;
@@ -356,6 +378,9 @@ void NfftOp<T>::trafo(const cx<T> *fhat, cx<T> *f, hipStream_t st, const int *se
   grid_fft(fhat, st);
   int PCs = plane_group_log2(P);
   const int Mc = sel ? msel : M;
+  // every node: in the spatial order of the geometry (same values as node order)
+  if (!sel && !(getenv("NUDIM_INTERP_ORDER") && std::string(getenv("NUDIM_INTERP_ORDER")) == "node"))
+    sel = g->perm.template as<int>();
   dim3 gr(cdiv(Mc, kThreads >> PCs), P >> PCs);
   // algorithmic bytes: grid read once, node values written, node coordinates
   ProfScope ps(KC_INTERP, st, ((double)n * n * sizeof(cx<T>) + (double)Mc * sizeof(cx<T>) + 2.0 * Mc * sizeof(T)) * P);
@@ -391,8 +416,8 @@ void NfftOp<T>::adjoint(const cx<T> *f, cx<T> *fhat, hipStream_t st) {
     ProfScope ps(KC_GATHER, st, ((double)M * sizeof(cx<T>) + 4.0 * M + (double)ncell * sizeof(cx<T>)) * P);
     cx<T> *gp = grid.as<cx<T>>(), *pp = part.as<cx<T>>();
     if (g->nchunk > 0)
-      hipLaunchKernelGGL((k_gather_chunk<T, kGatherTile>), dim3(g->nchunk, P / 64), dim3(kThreads),
-                         65 * 64 * sizeof(cx<T>), st, f, gp, pp, g->chunk_tile.template as<int>(), g->chunk_part.template as<int>(),
+      hipLaunchKernelGGL((k_gather_chunk<T, kGatherTile, GatherChunk<T>::value>), dim3(g->nchunk, P / 64),
+                         dim3(kThreads), (GatherChunk<T>::value + 1) * 64 * sizeof(cx<T>), st, f, gp, pp, g->chunk_tile.template as<int>(), g->chunk_part.template as<int>(),
                          g->chunk_node.template as<int>(), g->tnode.template as<int>(), g->chunk_cell.template as<int>(),
                          g->tent.template as<GEnt<T>>(), n, P);
     if (g->ncomb > 0)

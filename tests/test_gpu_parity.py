@@ -592,3 +592,20 @@ def test_z_transform_lengths(hip, orc, np_, nz):
         r = job.iterate(2)
         assert rel(job.volume(), img2) < 1e-4
         assert abs(r[1] - r2) < 1e-5
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_interp_spatial_order_same_values(hip, orc, monkeypatch, prec):
+    """The interpolation visits the nodes sorted by 16 x 16-cell tile (for L2
+    reuse of the grid rows); every node's sum is the same as in node order,
+    bit for bit."""
+    N, P = 64, 64
+    pmin, pmax = orc.pad_range(-60, 3)
+    x = orc.polar_nodes(pmin, pmax, 1.0, N)
+    rng = np.random.default_rng(31)
+    fh = rng.standard_normal((P, N, N)) + 1j * rng.standard_normal((P, N, N))
+    W = 14 if prec == 0 else 8
+    a = hip.nfft_trafo(x, fh, n=2 * N, W=W, precision=prec)
+    monkeypatch.setenv("NUDIM_INTERP_ORDER", "node")
+    b = hip.nfft_trafo(x, fh, n=2 * N, W=W, precision=prec)
+    assert np.array_equal(a, b)
