@@ -59,7 +59,10 @@ typedef struct nudim_params {
   int use_graph;                      /* run passes as hipGraphs: a pass is captured once per
                                          starting state (volume buffers, HIO/ER) and replayed
                                          for later passes that start in it; passes that time
-                                         kernels (nudim_profile) or dump run directly */
+                                         kernels (nudim_profile) or dump run directly.  While a
+                                         pass is being captured, synchronous copies of other
+                                         threads of the process fail: use it where one context
+                                         is driven at a time (the job server ignores it) */
   /* denoising variants of src/reconstruction-parallel.c (0 / 0 for the serial program) */
   int tv_full;                        /* TV over the whole volume even with a height band (:766) */
   float diffusion_lambda;             /* aniso_diffusion_2d lambda; 0: 0.25f (:764 passes 0.4) */

@@ -220,6 +220,11 @@ extern "C" int nudim_server_submit(nudim_server *s, const nudim_params *p, const
     const int nmeas = (int)((p->gam_max - p->gam_min) / p->interval + 0.5) + 1;
     std::unique_ptr<ServerJob> j(new ServerJob());
     j->p = *p;
+    // served jobs launch their passes directly: while one job's stream
+    // captured a pass into a graph, other jobs' synchronous copies in their
+    // own slot threads fail (the runtime refuses a legacy-stream operation
+    // while a stream is capturing); replay gains nothing measurable here
+    j->p.use_graph = 0;
     j->proj.assign(proj, proj + (size_t)nmeas * p->nz * p->nx);
     j->volume = volume;
     j->rfactors = rfactors;

@@ -127,3 +127,24 @@ def test_cu_masked_jobs_identical(hip):
             assert np.array_equal(vol, ref[0]) and np.array_equal(rf, ref[1])
     with pytest.raises(hip.NudimError):
         hip.Server(max_active=1, small_cus=100000)
+
+
+def test_graph_jobs_in_a_mix_run_directly(hip):
+    """A job submitted with use_graph runs with direct launches in the
+    server (capturing one job's pass would break the other slots'
+    synchronous copies): it succeeds beside other jobs and equals its alone
+    run without graphs."""
+    p = job_params(hip, 48, 48, 1, True, 1)
+    proj = job_inputs(hip, 48, 48, 21)
+    v0, r0 = alone(hip, p, proj)
+    pg = job_params(hip, 48, 48, 1, True, 1)
+    pg.use_graph = 1
+    with hip.Server(max_active=4) as srv:
+        ids = {srv.submit(pg, proj): "graph"}
+        for k in range(3):
+            ids[srv.submit(job_params(hip, 32, 32, 1, True, 1), job_inputs(hip, 32, 32, 30 + k))] = "other"
+        for _ in range(4):
+            rep, vol, rf = srv.next(timeout_ms=120000)
+            assert rep["status"] == 0, rep["error"]
+            if ids[rep["job_id"]] == "graph":
+                assert np.array_equal(vol, v0) and np.array_equal(rf, r0)
