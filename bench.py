@@ -487,6 +487,67 @@ def batch(args):
                         "least work first, %d in flight" % args.batch_slots}
 
 
+def stacked(args):
+    """Many small volumes of one geometry (64^3, np 64, +-60 at step 2, SNR 4,
+    ner = 30 with TV over the slab), FP32, end to end (create, load, initial
+    INFFT, passes, volumes back), three ways: one batch context
+    (nudim_batch_create: every kernel of a pass covers all volumes), one job
+    after another, and the job server with 8 jobs in flight."""
+    K, ner = args.stack_vols, 30
+    vols = []
+    for i in range(K):
+        bp = phantom.blob_params(16, (64, 64, 64), 32, 6000 + i)
+        vols.append(phantom.add_noise(phantom.projections(bp, 64, -60, 60, 2), 4.0, 7000 + i))
+    p = nd.make_params(64, 64, 64, -60, 60, 2, padangle=1.0, weight=True, nhio=0, ner=ner, denoise=True,
+                       method=nd.TV, height=32, center=32, precision=nd.FP32)
+
+    def run_batch(projs):
+        with nd.Batch(p, len(projs)) as b:
+            for k, pr in enumerate(projs):
+                b.load_projections(k, pr)
+            b.initial()
+            b.iterate(ner)
+            return [b.volume(k) for k in range(len(projs))]
+
+    def run_one(pr):
+        with nd.Reconstruction(p) as job:
+            job.load_projections(pr)
+            job.initial()
+            job.iterate(ner)
+            return job.volume()
+
+    def run_server(projs, slots=8):
+        out = [None] * len(projs)
+        with nd.Server(max_active=slots) as srv:
+            ids = {srv.submit(p, pr): k for k, pr in enumerate(projs)}
+            for _ in projs:
+                rep, v, _ = srv.next(timeout_ms=-1)
+                if rep["status"] != 0:
+                    raise RuntimeError("stacked leg job failed: %s" % rep["error"])
+                out[ids[rep["job_id"]]] = v
+        return out
+
+    run_batch(vols[:3])  # the geometry's tables and the code paths warm
+    run_one(vols[0])
+    res = {}
+    for name, fn in (("batch", lambda: run_batch(vols)), ("one_at_a_time", lambda: [run_one(v) for v in vols]),
+                     ("server_8_in_flight", lambda: run_server(vols))):
+        t = time.perf_counter()
+        out = fn()
+        res[name] = (time.perf_counter() - t, out)
+        log("  %d volumes of 64^3, %s: %.2f s -> %.0f volume-iter/s"
+            % (K, name, res[name][0], K * ner / res[name][0]))
+    same = all(np.array_equal(a, b) for a, b in zip(res["batch"][1], res["one_at_a_time"][1]))
+    return {"volumes": K, "passes": ner, "shape": [64, 64, 64],
+            "volume_iters_per_s": {k: K * ner / v[0] for k, v in res.items()},
+            "seconds": {k: v[0] for k, v in res.items()},
+            "speedup_batch_over_one_at_a_time": res["one_at_a_time"][0] / res["batch"][0],
+            "speedup_batch_over_server": res["server_8_in_flight"][0] / res["batch"][0],
+            "batch_volumes_bit_equal_to_own_jobs": bool(same),
+            "schedule": "FP32, ner=30 with TV over the slab, end to end per volume (create, load, initial, passes, "
+                        "volume back)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -517,6 +578,8 @@ def main():
     ap.add_argument("--cli", action="store_true", help="run only the drop-in CLI on the 256^3 workload's files")
     ap.add_argument("--batch-jobs", type=int, default=48)
     ap.add_argument("--batch-slots", type=int, default=8)
+    ap.add_argument("--stack-vols", type=int, default=32, help="volumes of the stacked 64^3 leg (0: skip it)")
+    ap.add_argument("--stacked", action="store_true", help="run only the stacked 64^3 leg")
     ap.add_argument("--probe", type=int, default=0, metavar="MIB",
                     help="run one FP32 pass and a MIB-MiB device copy, then exit (for counter runs)")
     args = ap.parse_args()
@@ -532,6 +595,9 @@ def main():
         return
     if args.batch:
         print(json.dumps({"batch": batch(args)}), flush=True)
+        return
+    if args.stacked:
+        print(json.dumps({"stacked_64": stacked(args)}), flush=True)
         return
     if args.cli:
         print(json.dumps({"cli": cli_leg(args)}), flush=True)
@@ -640,6 +706,7 @@ def main():
 
     configs_128 = None if args.no_small else small_configs()
     batch_out = None if args.no_batch else batch(args)
+    stacked_out = stacked(args) if args.stack_vols > 0 else None
     serving = serve(args, nd.FP32) if args.serve_jobs > 0 else None
     below = None
     if args.serve_jobs > 0 and args.below_jobs > 0:
@@ -706,6 +773,7 @@ def main():
                     for k, v in kern.items()},
         "configs_128": configs_128,
         "batch": batch_out,
+        "stacked_64": stacked_out,
         "serving": serving,
         "serving_below_capacity": below,
         "serving_fp32_vs_fp64": fp64_serving,

@@ -91,7 +91,8 @@ int nudim_load_samples(nudim_ctx *ctx, const double *slices, const int *kind);
 int nudim_initial(nudim_ctx *ctx);
 /* `count` passes of the HIO/ER loop body, src/reconstruction.c:545-700.  The pass
  * counter nn continues across calls (HIO while nn < nhio).  rfactor[count] gets
- * R = sqrt(diff1/diff2) of each pass (:603-604); may be NULL. */
+ * R = sqrt(diff1/diff2) of each pass (:603-604), rfactor[count][nvol] for a
+ * batch (R of every volume of each pass); may be NULL. */
 int nudim_iterate(nudim_ctx *ctx, int count, double *rfactor);
 /* current volume img[nz][np][np] (double) */
 int nudim_get_volume(nudim_ctx *ctx, double *img);
@@ -100,6 +101,26 @@ int nudim_set_volume(nudim_ctx *ctx, const double *img, int nn);
 /* at pass `nn` keep copies of img before and after denoising (the
  * grad_before/grad_after dump of src/reconstruction.c:637-658) */
 int nudim_set_dump(nudim_ctx *ctx, int nn, double *before, double *after);
+
+/* ---- batches: nvol volumes of one geometry reconstructed together ----
+ * No reference counterpart: the reference runs one volume per process
+ * (src/reconstruction.c:50-733).  A batch context runs the same passes as
+ * nvol contexts with these parameters would, each volume with its own
+ * projections, CGNR step sizes, denoising norms and R, but every kernel of a
+ * pass covers all volumes (the Fourier side as nvol nz planes), so small
+ * volumes fill the device.  Each volume's result equals its own job's to
+ * rounding (bit for bit where the plane grouping of the reductions is the
+ * same, e.g. nz a multiple of 64 or nvol odd).  Device memory is about nvol
+ * times a single job's.  nudim_initial and nudim_iterate work on the whole
+ * batch (rfactor[count][nvol]); nudim_load_projections, nudim_get_volume,
+ * nudim_load_samples, nudim_set_volume and nudim_set_dump refuse a batch
+ * (NUDIM_ESTATE).  A batch of one volume is a plain job. */
+int nudim_batch_create(const nudim_params *p, int nvol, nudim_ctx **out);
+/* projections of volume vol (0 <= vol < nvol), as nudim_load_projections;
+ * nudim_initial needs every volume loaded */
+int nudim_batch_load_projections(nudim_ctx *ctx, int vol, const double *proj);
+/* current volume vol, as nudim_get_volume */
+int nudim_batch_get_volume(nudim_ctx *ctx, int vol, double *img);
 /* bytes of device memory the context holds */
 size_t nudim_device_bytes(const nudim_ctx *ctx);
 /* bytes a context with these parameters would hold (admission control) */

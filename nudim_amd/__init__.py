@@ -53,7 +53,7 @@ EXPORTS = [
     "nudim_diffusion2d",
     "nudim_constraint", "nudim_projection_ft", "nudim_project_2d1d",
     "nudim_server_create", "nudim_server_submit", "nudim_server_next", "nudim_server_stats",
-    "nudim_server_destroy",
+    "nudim_server_destroy", "nudim_batch_create", "nudim_batch_load_projections", "nudim_batch_get_volume",
 ]
 EAGAIN = -6
 
@@ -125,6 +125,9 @@ def _declare(L):
         "nudim_server_next": (i, [v, i, C.POINTER(JobReport)]),
         "nudim_server_stats": (i, [v, C.POINTER(i), C.POINTER(i), C.POINTER(C.c_longlong)]),
         "nudim_server_destroy": (None, [v]),
+        "nudim_batch_create": (i, [C.POINTER(Params), i, C.POINTER(v)]),
+        "nudim_batch_load_projections": (i, [v, i, dp]),
+        "nudim_batch_get_volume": (i, [v, i, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -191,16 +194,21 @@ def autotune(p):
 class Reconstruction:
     """One reconstruction job on the device (the numerics of reconstruction.c's main)."""
 
+    nvol = 1
+
     def __init__(self, params):
         self.p = params
         h = C.c_void_p()
-        _check(lib().nudim_create(C.byref(params), C.byref(h)))
+        self._create(params, h)
         self._h = h
         nm, M, na = C.c_int(), C.c_int(), C.c_int()
         _check(lib().nudim_sizes(h, C.byref(nm), C.byref(M), C.byref(na)))
         self.nmeas, self.M, self.nang = nm.value, M.value, na.value
         self.shape = (params.nz, params.np, params.np)
         self.nn = 0
+
+    def _create(self, params, h):
+        _check(lib().nudim_create(C.byref(params), C.byref(h)))
 
     def close(self):
         if getattr(self, "_h", None):
@@ -265,6 +273,35 @@ class Reconstruction:
 
     def synchronize(self):
         _check(lib().nudim_synchronize(self._h))
+
+
+class Batch(Reconstruction):
+    """nvol volumes of one geometry reconstructed together
+    (nudim_batch_create): load each volume's projections, then initial() and
+    iterate() run the whole batch; iterate returns R as [count][nvol]."""
+
+    def __init__(self, params, nvol):
+        self.nvol = int(nvol)
+        super().__init__(params)
+
+    def _create(self, params, h):
+        _check(lib().nudim_batch_create(C.byref(params), self.nvol, C.byref(h)))
+
+    def load_projections(self, vol, proj):
+        proj = _f64(proj)
+        assert proj.shape == (self.nmeas, self.p.nz, self.p.nx), proj.shape
+        _check(lib().nudim_batch_load_projections(self._h, vol, _ptr(proj)))
+
+    def iterate(self, count=1):
+        r = np.zeros((max(count, 1), self.nvol))
+        _check(lib().nudim_iterate(self._h, count, _ptr(r)))
+        self.nn += count
+        return r[:count]
+
+    def volume(self, vol):
+        out = np.zeros(self.shape)
+        _check(lib().nudim_batch_get_volume(self._h, vol, _ptr(out)))
+        return out
 
 
 def stream_probe(nbytes, kind, reps=5, device=0):

@@ -72,10 +72,12 @@ __global__ void k_pft_z(const cx<double> *__restrict__ Q, const cx<double> *__re
   tomo[i] = acc;
 }
 
-// sample assembly, reconstruction.c:437-484: meas[m][j]
+// sample assembly, reconstruction.c:437-484: meas[m][off + j] of a sample
+// table with Pt planes per node (off = k nz for volume k of a batch)
 template <typename T>
 __global__ void k_assemble(const cx<double> *__restrict__ tomo, const int *__restrict__ kind, const int *__restrict__ order,
-                           const double *__restrict__ rnd, cx<T> *__restrict__ meas, int M, int np, int nz) {
+                           const double *__restrict__ rnd, cx<T> *__restrict__ meas, int M, int np, int nz, int Pt,
+                           int off) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (long long)M * nz) return;
   int m = (int)(i / nz), j = (int)(i % nz);
@@ -86,7 +88,7 @@ __global__ void k_assemble(const cx<double> *__restrict__ tomo, const int *__res
   } else if (rnd) {
     v = mk((T)rnd[(long long)j * M + m], (T)0);
   }
-  meas[i] = v;
+  meas[(long long)m * Pt + off + j] = v;
 }
 
 // R sums from the quadratic form (reconstruction.c:574-582 over the known
@@ -104,10 +106,14 @@ __global__ void k_rform_final(const double *__restrict__ q, const double *__rest
 }
 
 // R = sqrt(sum_p d1 / sum_p d2) (reconstruction.c:603-604); one block of 256
-// threads, fixed-order sums
+// threads per volume (block k: planes k P .. k P + P - 1 into out[k]),
+// fixed-order sums
 __global__ void k_rfactor_final(const double *__restrict__ d1, const double *__restrict__ d2, int P,
                                 double *__restrict__ out) {
   __shared__ double sa[256], sb[256];
+  d1 += (long long)blockIdx.x * P;
+  d2 += (long long)blockIdx.x * P;
+  out += blockIdx.x;
   double a = 0, b = 0;
   for (int p = threadIdx.x; p < P; p += 256) {
     a += d1[p];
@@ -163,14 +169,23 @@ static void projection_ft_dev(int nx, int nz, int np, int nmeas, const double *d
 // job
 // ---------------------------------------------------------------------------
 
+// A job reconstructs nvol volumes of one geometry together (a batch; 1 for
+// a plain job): volume k's Fourier planes are k nz .. k nz + nz - 1 of every
+// Fourier-side operator, its real-space volume the k-th of nvol consecutive
+// [y][x][z] volumes.  Every operator works plane by plane, so each volume
+// follows the same arithmetic as its own job.
 struct JobBase {
   virtual ~JobBase() = default;
-  virtual void load_meas_planes(const double *slices) = 0;                        // host c128 [nz][M]
-  virtual void load_projections(const double *proj, const double *rnd) = 0;     // host
+  virtual void load_meas_planes(const double *slices) = 0;  // host c128 [nz][M] (nvol = 1)
+  // host projections of volume vol; the known nodes' adjoint terms are formed
+  // once every volume is loaded (finish_load)
+  virtual void load_projections(const double *proj, const double *rnd, int vol) = 0;
+  virtual void finish_load() = 0;
   virtual void initial() = 0;
+  // one pass; R of volume k into rdev_slot[k]
   virtual void iterate(int nn, double *rdev_slot) = 0;
-  virtual void get_volume(double *img) = 0;
-  virtual void set_volume(const double *img) = 0;
+  virtual void get_volume(double *img, int vol) = 0;
+  virtual void set_volume(const double *img) = 0;  // nvol = 1
   virtual size_t bytes() const = 0;
   // the host-side state a pass reads and changes: which buffer holds the
   // current, the previous and the spare volume
@@ -309,8 +324,11 @@ struct Job : JobBase {
   nudim_params prm;
   const Geometry &geo;
   hipStream_t st;
-  int N, P, M, n, W, form;
-  long long V, NN, MM;
+  // P: planes (nz) of one volume; K volumes, Pt = K P Fourier planes; V and
+  // Vt: voxels of one volume and of all; NN, MM: Fourier and sample sizes
+  int N, P, M, n, W, form, K, Pt;
+  long long V, Vt, NN, MM;
+  bool adjoint_done = false;
   NfftOp<T> op;
   ToeplitzOp<T> toep;
   Cgnr<T> cg;
@@ -335,9 +353,11 @@ struct Job : JobBase {
   T *img = nullptr, *img_last = nullptr, *spare = nullptr;
   static constexpr int kTvBlocks = 1024;
 
-  Job(const nudim_params &p, const Geometry &g, hipStream_t s) : prm(p), geo(g), st(s) {
+  Job(const nudim_params &p, const Geometry &g, hipStream_t s, int nvol = 1) : prm(p), geo(g), st(s) {
     N = p.np;
     P = p.nz;
+    K = nvol;
+    Pt = K * P;
     M = g.M;
     resolve_knobs(&p, &n, &W, &form);
     if (autotune_enabled(&p) && passes_run_nfft(form, p.np)) {
@@ -346,15 +366,16 @@ struct Job : JobBase {
       W = t.W;
     }
     V = (long long)N * N * P;
-    NN = V;
-    MM = (long long)M * P;
-    op.init(N, n, W, g.x.data(), M, P);
+    Vt = V * K;
+    NN = (long long)N * N * Pt;
+    MM = (long long)M * Pt;
+    op.init(N, n, W, g.x.data(), M, Pt);
     if (!g.w.empty()) {
       dw.alloc(g.w.size() * 8);
       NUDIM_HIP(hipMemcpy(dw.p, g.w.data(), g.w.size() * 8, hipMemcpyHostToDevice));
     }
-    if (form == NUDIM_CG_TOEPLITZ) toep.init(N, g.x.data(), g.w.empty() ? nullptr : g.w.data(), M, P, st);
-    cg.init(N, M, P, form, &op, form == NUDIM_CG_TOEPLITZ ? &toep : nullptr, dw.as<double>());
+    if (form == NUDIM_CG_TOEPLITZ) toep.init(N, g.x.data(), g.w.empty() ? nullptr : g.w.data(), M, Pt, st);
+    cg.init(N, M, Pt, form, &op, form == NUDIM_CG_TOEPLITZ ? &toep : nullptr, dw.as<double>());
     proj_start = projected_start(form, N);
     if (proj_start) {
       if (keep_on((size_t)10 * NN * sizeof(cx<T>))) cg.keep_directions(10);
@@ -364,7 +385,7 @@ struct Job : JobBase {
       for (int m = 0; m < M; m++) wu[m] = g.kind[m] == 1 ? 0.0 : (g.w.empty() ? 1.0 : g.w[m]);
       toep.init_second(g.x.data(), wu.data(), M, st);
       bk.alloc(NN * sizeof(cx<T>));
-      rpart.alloc((size_t)2 * kRBlocks * P * sizeof(double));
+      rpart.alloc((size_t)2 * kRBlocks * Pt * sizeof(double));
       std::vector<int> kn;
       for (int m = 0; m < M; m++)
         if (g.kind[m] == 1) kn.push_back(m);
@@ -380,10 +401,10 @@ struct Job : JobBase {
         for (int m = 0; m < M; m++) wk[m] = g.kind[m] == 1 ? 1.0 : 0.0;
         toep.init_rform(g.x.data(), wk.data(), M, st);
         ck.alloc(NN * sizeof(cx<T>));
-        m2.alloc(P * sizeof(double));
-        qpart.alloc((size_t)2 * N * P * sizeof(double));
-        qsum.alloc(P * sizeof(double));
-        xr.alloc(P * sizeof(double));
+        m2.alloc(Pt * sizeof(double));
+        qpart.alloc((size_t)2 * N * Pt * sizeof(double));
+        qsum.alloc(Pt * sizeof(double));
+        xr.alloc(Pt * sizeof(double));
       }
     }
     dkind.alloc(M * sizeof(int));
@@ -394,21 +415,21 @@ struct Job : JobBase {
     y.alloc(MM * sizeof(cx<T>));
     Fh.alloc(NN * sizeof(cx<T>));
     F0.alloc(NN * sizeof(cx<T>));
-    img_a.alloc(V * sizeof(T));
-    img_b.alloc(V * sizeof(T));
+    img_a.alloc(Vt * sizeof(T));
+    img_b.alloc(Vt * sizeof(T));
     if (p.denoise && p.method == NUDIM_TV) {
-      img_c.alloc(V * sizeof(T));
-      grad.alloc(V * sizeof(T));
-      grad2.alloc(V * sizeof(T));
+      img_c.alloc(Vt * sizeof(T));
+      grad.alloc(Vt * sizeof(T));
+      grad2.alloc(Vt * sizeof(T));
       // zero outside the TV band for the life of the job
-      NUDIM_HIP(hipMemset(grad.p, 0, V * sizeof(T)));
-      NUDIM_HIP(hipMemset(grad2.p, 0, V * sizeof(T)));
+      NUDIM_HIP(hipMemset(grad.p, 0, Vt * sizeof(T)));
+      NUDIM_HIP(hipMemset(grad2.p, 0, Vt * sizeof(T)));
     }
-    d1.alloc(P * 8);
-    d2.alloc(P * 8);
-    tvpart_g.alloc((size_t)tv_blocks(N, N, P) * 8);
-    tvpart_a.alloc((size_t)tv_blocks(N, N, P) * 8);
-    tvsums.alloc(2 * 8);
+    d1.alloc(Pt * 8);
+    d2.alloc(Pt * 8);
+    tvpart_g.alloc((size_t)K * tv_blocks(N, N, P) * 8);
+    tvpart_a.alloc((size_t)K * tv_blocks(N, N, P) * 8);
+    tvsums.alloc(2 * K * 8);
     fdz = make_fft_desc(P);
     std::vector<double> t;
     fft_twiddles(P, &t);
@@ -419,7 +440,7 @@ struct Job : JobBase {
     img = img_a.as<T>();
     img_last = img_b.as<T>();
     spare = img_c.as<T>();
-    NUDIM_HIP(hipMemsetAsync(img, 0, V * sizeof(T), st));
+    NUDIM_HIP(hipMemsetAsync(img, 0, Vt * sizeof(T), st));
   }
 
   // b_K = A^H W K m (the known nodes' measured values, weighted); with
@@ -428,12 +449,12 @@ struct Job : JobBase {
     NUDIM_CHECK(proj_start && bk.p, "b_K requested without the projected CG start");
     cx<T> *t = cg.t.template as<cx<T>>();
     hipLaunchKernelGGL(k_known_weighted<T>, dim3(cdiv(MM, 256)), dim3(256), 0, st, meas.as<cx<T>>(), dkind.as<int>(),
-                       dw.as<double>(), t, MM, P);
+                       dw.as<double>(), t, MM, Pt);
     NUDIM_HIP(hipGetLastError());
     op.adjoint(t, bk.as<cx<T>>(), st);
     if (rform) {
       hipLaunchKernelGGL(k_known_weighted<T>, dim3(cdiv(MM, 256)), dim3(256), 0, st, meas.as<cx<T>>(),
-                         dkind.as<int>(), (const double *)nullptr, t, MM, P);
+                         dkind.as<int>(), (const double *)nullptr, t, MM, Pt);
       NUDIM_HIP(hipGetLastError());
       op.adjoint(t, ck.as<cx<T>>(), st);
       // |0 - m|^2 over the known nodes
@@ -451,30 +472,42 @@ struct Job : JobBase {
            tvpart_a.bytes + tvsums.bytes + twz.bytes;
   }
 
+  void finish_load() override {
+    if (proj_start && !adjoint_done) {
+      known_adjoint();
+      NUDIM_HIP(hipStreamSynchronize(st));
+    }
+    adjoint_done = true;
+  }
+
   void load_meas_planes(const double *slices) override {
+    NUDIM_CHECK(K == 1, "sample tables load into single-volume jobs only");
     DevBuf tmp((size_t)MM * 16);
     NUDIM_HIP(hipMemcpyAsync(tmp.p, slices, (size_t)MM * 16, hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_cx_from_planes<T>, dim3(cdiv(MM, 256)), dim3(256), 0, st, tmp.as<double>(), meas.as<cx<T>>(),
                        (long long)M, P);
     NUDIM_HIP(hipGetLastError());
-    if (proj_start) known_adjoint();
-    NUDIM_HIP(hipStreamSynchronize(st));
+    adjoint_done = false;
+    finish_load();
   }
 
-  void load_projections(const double *proj, const double *rnd) override {
+  void load_projections(const double *proj, const double *rnd, int vol) override {
+    NUDIM_CHECK(vol >= 0 && vol < K, "volume index out of range");
     int nx = prm.nx, nmeas = geo.nmeas;
     size_t pb = (size_t)nmeas * P * nx * 8;
+    const long long Mv = (long long)M * P;  // samples of one volume
     DevBuf dproj(pb), dtomo((size_t)nmeas * P * N * sizeof(cx<double>)), drnd;
     NUDIM_HIP(hipMemcpyAsync(dproj.p, proj, pb, hipMemcpyHostToDevice, st));
     if (rnd) {
-      drnd.alloc((size_t)MM * 8);
-      NUDIM_HIP(hipMemcpyAsync(drnd.p, rnd, (size_t)MM * 8, hipMemcpyHostToDevice, st));
+      drnd.alloc((size_t)Mv * 8);
+      NUDIM_HIP(hipMemcpyAsync(drnd.p, rnd, (size_t)Mv * 8, hipMemcpyHostToDevice, st));
     }
     projection_ft_dev(nx, P, N, nmeas, dproj.as<double>(), dtomo.as<cx<double>>(), st);
-    hipLaunchKernelGGL(k_assemble<T>, dim3(cdiv(MM, 256)), dim3(256), 0, st, dtomo.as<cx<double>>(),
-                       dkind.as<int>(), dorder.as<int>(), drnd.as<double>(), meas.as<cx<T>>(), M, N, P);
+    hipLaunchKernelGGL(k_assemble<T>, dim3(cdiv(Mv, 256)), dim3(256), 0, st, dtomo.as<cx<double>>(),
+                       dkind.as<int>(), dorder.as<int>(), drnd.as<double>(), meas.as<cx<T>>(), M, N, P, Pt, vol * P);
     NUDIM_HIP(hipGetLastError());
-    if (proj_start) known_adjoint();
+    adjoint_done = false;
+    if (K == 1) finish_load();
     NUDIM_HIP(hipStreamSynchronize(st));
   }
 
@@ -482,36 +515,39 @@ struct Job : JobBase {
   // f += alpha p, folded into the loads)
   // the z IFFT of g + sum_{s < nk} alpha_s p_s (p_s at pv + s N^2 P, alpha_s
   // per plane at alpha + s P)
+  // (lines: N^2 K, line (y x) K + k of the Fourier side from volume k's
+  // line (y x))
   void zfft_inverse(const cx<T> *g, T *out, const cx<T> *pv = nullptr, const double *alpha = nullptr, int nk = 1) {
     int Gs = line_chunk_log2(P, 1 << 30, (int)sizeof(cx<T>), FftEpt<T>::value);
-    long long nlines = (long long)N * N;
+    long long nlines = (long long)N * N * K;
     size_t lds = (size_t)P * ((1 << Gs) + 1) * sizeof(cx<T>);
     if (!pv) nk = 0;
-    ProfScope ps(KC_ZFFT, st, (double)V * sizeof(cx<T>) * (1.0 + nk) + (double)V * sizeof(T));
+    ProfScope ps(KC_ZFFT, st, (double)Vt * sizeof(cx<T>) * (1.0 + nk) + (double)Vt * sizeof(T));
     if (zfft2_launch<T>(P, true, nlines, nullptr, g, pv, alpha, nullptr, nullptr, out, twz.as<cx<T>>(), st, nk,
-                        NN)) {
+                        NN, K, V)) {
       NUDIM_HIP(hipGetLastError());
       return;
     }
     for (int s = 0; s < nk; s++)
       hipLaunchKernelGGL(k_axpy_planes<T>, dim3(cdiv(NN, 256)), dim3(256), 0, st, const_cast<cx<T> *>(g),
-                         pv + (long long)s * NN, alpha + (long long)s * P, 1.0, NN, P);
+                         pv + (long long)s * NN, alpha + (long long)s * Pt, 1.0, NN, Pt);
     hipLaunchKernelGGL(k_zfft_inv<T>, dim3(cdiv(nlines, 1 << Gs)), dim3(kFftThreads), lds, st, g, out, fdz,
-                       twz.as<cx<T>>(), Gs, nlines);
+                       twz.as<cx<T>>(), Gs, nlines, K, V);
     NUDIM_HIP(hipGetLastError());
   }
 
   void zfft_forward(const T *in, cx<T> *fh, cx<T> *f0) {
     int Gs = line_chunk_log2(P, 1 << 30, (int)sizeof(cx<T>), FftEpt<T>::value);
-    long long nlines = (long long)N * N;
+    long long nlines = (long long)N * N * K;
     size_t lds = (size_t)P * ((1 << Gs) + 1) * sizeof(cx<T>);
-    ProfScope ps(KC_ZFFT, st, (double)V * sizeof(T) + (f0 ? 2.0 : 1.0) * V * sizeof(cx<T>));
-    if (zfft2_launch<T>(P, false, nlines, in, nullptr, nullptr, nullptr, fh, f0, nullptr, twz.as<cx<T>>(), st)) {
+    ProfScope ps(KC_ZFFT, st, (double)Vt * sizeof(T) + (f0 ? 2.0 : 1.0) * Vt * sizeof(cx<T>));
+    if (zfft2_launch<T>(P, false, nlines, in, nullptr, nullptr, nullptr, fh, f0, nullptr, twz.as<cx<T>>(), st, 0, 0,
+                        K, V)) {
       NUDIM_HIP(hipGetLastError());
       return;
     }
     hipLaunchKernelGGL(k_zfft_fwd<T>, dim3(cdiv(nlines, 1 << Gs)), dim3(kFftThreads), lds, st, in, fh, f0, fdz,
-                       twz.as<cx<T>>(), Gs, nlines);
+                       twz.as<cx<T>>(), Gs, nlines, K, V);
     NUDIM_HIP(hipGetLastError());
   }
 
@@ -531,7 +567,7 @@ struct Job : JobBase {
     if (yhi < ylo) return;
     // total_variation_discrete(_height)(img, 0.2f, 40, ...), reconstruction.c:630-632
     T *r = tv_sweeps<T>(u, spare, grad.as<T>(), grad2.as<T>(), N, N, P, ylo, yhi, 0.2f, 40, tvpart_g.as<double>(),
-                        tvpart_a.as<double>(), tvsums.as<double>(), tv_blocks(N, N, P), st);
+                        tvpart_a.as<double>(), tvsums.as<double>(), tv_blocks(N, N, P), st, K);
     if (r != img) std::swap(img, spare);
   }
 
@@ -539,7 +575,7 @@ struct Job : JobBase {
     // aniso_diffusion_2d(img, 0.25f, 0.005f, 80, ...), reconstruction.c:627
     // (reconstruction-parallel.c:764 passes 0.4f)
     const double lam = (double)(prm.diffusion_lambda > 0 ? prm.diffusion_lambda : 0.25f), kappa = (double)0.005f;
-    diffusion_sweeps<T>(u, N, N, P, 80, lam, kappa * kappa, st);
+    diffusion_sweeps<T>(u, N, N, P, 80, lam, kappa * kappa, st, K);
   }
 
   void copy_to_host_ref(const T *u, double *out) {
@@ -560,18 +596,18 @@ struct Job : JobBase {
     } else if (proj_start) {
       // R needs A fh only at the known nodes: the interpolation sums the R
       // terms itself (no A fh stored)
-      double *p1 = rpart.as<double>(), *p2 = p1 + (size_t)kRBlocks * P;
+      double *p1 = rpart.as<double>(), *p2 = p1 + (size_t)kRBlocks * Pt;
       op.trafo_rsums(Fh.as<cx<T>>(), meas.as<cx<T>>(), dknown.as<int>(), nknown, p1, p2, kRBlocks, st);
-      hipLaunchKernelGGL(k_sum_partials, dim3(cdiv(P, 8)), dim3(256), 0, st, p1, kRBlocks, P, d1.as<double>());
-      hipLaunchKernelGGL(k_sum_partials, dim3(cdiv(P, 8)), dim3(256), 0, st, p2, kRBlocks, P, d2.as<double>());
+      hipLaunchKernelGGL(k_sum_partials, dim3(cdiv(Pt, 8)), dim3(256), 0, st, p1, kRBlocks, Pt, d1.as<double>());
+      hipLaunchKernelGGL(k_sum_partials, dim3(cdiv(Pt, 8)), dim3(256), 0, st, p2, kRBlocks, Pt, d2.as<double>());
     } else {
       op.trafo(Fh.as<cx<T>>(), v, st);
       cg.red.template rfactor<T>(v, meas.as<cx<T>>(), dkind.as<int>(), M, d1.as<double>(), d2.as<double>(), st);
       hipLaunchKernelGGL(k_projector<T>, dim3(cdiv(MM, 256)), dim3(256), 0, st, v, meas.as<cx<T>>(),
-                         dkind.as<int>(), y.as<cx<T>>(), MM, P);
+                         dkind.as<int>(), y.as<cx<T>>(), MM, Pt);
     }
     if (!rform) {
-      hipLaunchKernelGGL(k_rfactor_final, dim3(1), dim3(256), 0, st, d1.as<double>(), d2.as<double>(), P, rslot);
+      hipLaunchKernelGGL(k_rfactor_final, dim3(K), dim3(256), 0, st, d1.as<double>(), d2.as<double>(), P, rslot);
       NUDIM_HIP(hipGetLastError());
     }
     if (proj_start) {
@@ -579,11 +615,11 @@ struct Job : JobBase {
       toep.apply_start(Fh.as<cx<T>>(), F0.as<cx<T>>(), bk.as<cx<T>>(), cg.z.template as<cx<T>>(),
                        cg.rowpart.template as<double>(), st, rform ? qpart.as<double>() : nullptr);
       if (rform) {
-        hipLaunchKernelGGL(k_sum_partials, dim3(cdiv(P, 8)), dim3(256), 0, st, qpart.as<double>(), 2 * N, P,
+        hipLaunchKernelGGL(k_sum_partials, dim3(cdiv(Pt, 8)), dim3(256), 0, st, qpart.as<double>(), 2 * N, Pt,
                            qsum.as<double>());
-        hipLaunchKernelGGL(k_rform_final, dim3(cdiv(P, 256)), dim3(256), 0, st, qsum.as<double>(), xr.as<double>(),
-                           m2.as<double>(), d1.as<double>(), d2.as<double>(), P);
-        hipLaunchKernelGGL(k_rfactor_final, dim3(1), dim3(256), 0, st, d1.as<double>(), d2.as<double>(), P, rslot);
+        hipLaunchKernelGGL(k_rform_final, dim3(cdiv(Pt, 256)), dim3(256), 0, st, qsum.as<double>(), xr.as<double>(),
+                           m2.as<double>(), d1.as<double>(), d2.as<double>(), Pt);
+        hipLaunchKernelGGL(k_rfactor_final, dim3(K), dim3(256), 0, st, d1.as<double>(), d2.as<double>(), P, rslot);
         NUDIM_HIP(hipGetLastError());
       }
       // f += sum_k alpha_k p_k happens in the z transform's loads (f itself
@@ -599,19 +635,22 @@ struct Job : JobBase {
       zfft_inverse(F0.as<cx<T>>(), img);
     }
     if (prm.denoise) {
-      if (nn == dump_nn && dump_before) copy_to_host_ref(img, dump_before);
+      if (nn == dump_nn && dump_before) copy_to_host_ref(img, dump_before);  // (dumps: nvol = 1)
       if (prm.method == NUDIM_DIFFUSION)
         diffusion(img);
       else
         tv(img);
       if (nn == dump_nn && dump_after) copy_to_host_ref(img, dump_after);
     }
-    hipLaunchKernelGGL(k_constraint<T>, dim3(cdiv(V, 256)), dim3(256), 0, st, img, img_last, N, P, prm.nx,
-                       nn < prm.nhio ? 1 : 0, prm.config_height, prm.nheight, prm.ncenter);
+    hipLaunchKernelGGL(k_constraint<T>, dim3(cdiv(Vt, 256)), dim3(256), 0, st, img, img_last, N, P, prm.nx,
+                       nn < prm.nhio ? 1 : 0, prm.config_height, prm.nheight, prm.ncenter, K);
     NUDIM_HIP(hipGetLastError());
   }
 
-  void get_volume(double *out) override { copy_to_host_ref(img, out); }
+  void get_volume(double *out, int vol) override {
+    NUDIM_CHECK(vol >= 0 && vol < K, "volume index out of range");
+    copy_to_host_ref(img + (long long)vol * V, out);
+  }
 
   void get_state(void *s[3]) const override {
     s[0] = img;
@@ -625,6 +664,7 @@ struct Job : JobBase {
   }
 
   void set_volume(const double *in) override {
+    NUDIM_CHECK(K == 1, "set_volume applies to single-volume jobs");
     DevBuf tmp(V * 8);
     NUDIM_HIP(hipMemcpyAsync(tmp.p, in, V * 8, hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_vol_from_ref<T>, dim3(cdiv(V, 256)), dim3(256), 0, st, tmp.as<double>(), img, N, N, P);
@@ -648,11 +688,13 @@ struct nudim_ctx {
   Geometry geo;
   hipStream_t st = nullptr;
   std::unique_ptr<JobBase> job;
+  int nvol = 1;              // volumes of the job (a batch when > 1)
+  std::vector<char> vloaded;  // which volumes have projections
   int nn = 0;
   bool loaded = false, initialized = false;
   DevBuf rtrace;
   int rcap = 0;
-  DevBuf rcur;  // R of a replayed pass, copied to its trace slot after the graph
+  DevBuf rcur;  // R of a replayed pass (nvol values), copied to its trace slot after the graph
   std::vector<PassGraph> graphs;
   KernelProfiler prof;
   // also the clean-up of a context whose creation failed part way
@@ -675,7 +717,7 @@ static void iterate_graph(nudim_ctx *c, int nn, double *rslot) {
   for (const auto &g : c->graphs)
     if (g.hio == hio && std::equal(pre, pre + 3, g.pre)) hit = &g;
   if (!hit) {
-    if (!c->rcur.p) c->rcur.alloc(sizeof(double));
+    if (!c->rcur.p) c->rcur.alloc(c->nvol * sizeof(double));
     PassGraph g{};
     std::copy(pre, pre + 3, g.pre);
     g.hio = hio;
@@ -702,7 +744,7 @@ static void iterate_graph(nudim_ctx *c, int nn, double *rslot) {
   }
   NUDIM_HIP(hipGraphLaunch(hit->exec, c->st));
   c->job->set_state(hit->post);
-  NUDIM_HIP(hipMemcpyAsync(rslot, c->rcur.p, sizeof(double), hipMemcpyDeviceToDevice, c->st));
+  NUDIM_HIP(hipMemcpyAsync(rslot, c->rcur.p, c->nvol * sizeof(double), hipMemcpyDeviceToDevice, c->st));
 }
 
 // makes the context's profiler current for the calling thread
@@ -724,13 +766,16 @@ extern "C" void nudim_default_params(nudim_params *p) {
   p->precision = NUDIM_FP64_PLAIN;
 }
 
-extern "C" int nudim_create(const nudim_params *p, nudim_ctx **out) {
+static int create_job(const nudim_params *p, int nvol, nudim_ctx **out) {
   API_BEGIN
   NUDIM_CHECK(out, "null output pointer");
   *out = nullptr;
   validate(p);
+  NUDIM_CHECK(nvol >= 1 && nvol <= 4096, "a batch holds 1 to 4096 volumes");
   std::unique_ptr<nudim_ctx> c(new nudim_ctx());
   c->p = *p;
+  c->nvol = nvol;
+  c->vloaded.assign(nvol, 0);
   NUDIM_HIP(hipSetDevice(p->device));
   bool masked = false;
   for (unsigned w : p->cu_mask) masked |= w != 0;
@@ -743,13 +788,19 @@ extern "C" int nudim_create(const nudim_params *p, nudim_ctx **out) {
     NUDIM_HIP(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
   make_geometry(p, &c->geo);
   if (p->precision == NUDIM_FP64_PLAIN)
-    c->job.reset(new Job<double>(c->p, c->geo, c->st));
+    c->job.reset(new Job<double>(c->p, c->geo, c->st, nvol));
   else
-    c->job.reset(new Job<float>(c->p, c->geo, c->st));
+    c->job.reset(new Job<float>(c->p, c->geo, c->st, nvol));
   NUDIM_HIP(hipStreamSynchronize(c->st));
   *out = c.release();
   return NUDIM_OK;
   API_END
+}
+
+extern "C" int nudim_create(const nudim_params *p, nudim_ctx **out) { return create_job(p, 1, out); }
+
+extern "C" int nudim_batch_create(const nudim_params *p, int nvol, nudim_ctx **out) {
+  return create_job(p, nvol, out);
 }
 
 extern "C" void nudim_destroy(nudim_ctx *c) { delete c; }
@@ -762,9 +813,10 @@ extern "C" int nudim_sizes(const nudim_ctx *c, int *nmeas, int *M, int *nang) {
   return NUDIM_OK;
 }
 
-extern "C" int nudim_load_projections(nudim_ctx *c, const double *proj) {
+static int load_volume(nudim_ctx *c, int vol, const double *proj) {
   API_BEGIN
   NUDIM_CHECK(c && proj, "null argument");
+  NUDIM_CHECK(vol >= 0 && vol < c->nvol, "volume index out of range");
   NUDIM_HIP(hipSetDevice(c->p.device));
   std::vector<double> rnd;
   if (c->p.random) {
@@ -776,10 +828,20 @@ extern "C" int nudim_load_projections(nudim_ctx *c, const double *proj) {
       for (int i = 0; i < c->geo.M; i++)
         if (c->geo.kind[i] != 1) rnd[(size_t)j * c->geo.M + i] = mt.uniform() * 10;
   }
-  c->job->load_projections(proj, rnd.empty() ? nullptr : rnd.data());
-  c->loaded = true;
+  c->job->load_projections(proj, rnd.empty() ? nullptr : rnd.data(), vol);
+  c->vloaded[vol] = 1;
+  c->loaded = std::all_of(c->vloaded.begin(), c->vloaded.end(), [](char v) { return v != 0; });
   return NUDIM_OK;
   API_END
+}
+
+extern "C" int nudim_load_projections(nudim_ctx *c, const double *proj) {
+  if (c && c->nvol > 1) return fail(NUDIM_ESTATE, "a batch loads each volume with nudim_batch_load_projections");
+  return load_volume(c, 0, proj);
+}
+
+extern "C" int nudim_batch_load_projections(nudim_ctx *c, int vol, const double *proj) {
+  return load_volume(c, vol, proj);
 }
 
 extern "C" int nudim_load_samples(nudim_ctx *c, const double *slices, const int *kind) {
@@ -799,8 +861,9 @@ extern "C" int nudim_load_samples(nudim_ctx *c, const double *slices, const int 
 extern "C" int nudim_initial(nudim_ctx *c) {
   API_BEGIN
   NUDIM_CHECK(c, "null context");
-  if (!c->loaded) return fail(NUDIM_ESTATE, "load projections before nudim_initial");
+  if (!c->loaded) return fail(NUDIM_ESTATE, "load projections (of every volume) before nudim_initial");
   NUDIM_HIP(hipSetDevice(c->p.device));
+  c->job->finish_load();
   c->job->initial();
   c->initialized = true;
   c->nn = 0;
@@ -813,23 +876,26 @@ extern "C" int nudim_iterate(nudim_ctx *c, int count, double *rfactor) {
   NUDIM_CHECK(c && count >= 0, "invalid argument");
   if (!c->initialized) return fail(NUDIM_ESTATE, "call nudim_initial (or nudim_set_volume) first");
   NUDIM_HIP(hipSetDevice(c->p.device));
-  if (c->rcap < count) {
-    c->rtrace.alloc((size_t)count * 8);
-    c->rcap = count;
+  const int nr = count * c->nvol;  // R values: [count][nvol]
+  if (c->rcap < nr) {
+    c->rtrace.alloc((size_t)nr * 8);
+    c->rcap = nr;
   }
+  if (c->loaded) c->job->finish_load();  // volumes reloaded since the last pass
   {
     ProfBind pb(c);
     // passes that time kernels or dump volumes run their launches directly
     for (int i = 0; i < count; i++) {
       const int nn = c->nn + i;
+      double *slot = c->rtrace.as<double>() + (size_t)i * c->nvol;
       if (c->p.use_graph && !c->prof.cls && nn != c->job->dump_nn)
-        iterate_graph(c, nn, c->rtrace.as<double>() + i);
+        iterate_graph(c, nn, slot);
       else
-        c->job->iterate(nn, c->rtrace.as<double>() + i);
+        c->job->iterate(nn, slot);
     }
   }
   if (rfactor && count > 0)
-    NUDIM_HIP(hipMemcpyAsync(rfactor, c->rtrace.p, (size_t)count * 8, hipMemcpyDeviceToHost, c->st));
+    NUDIM_HIP(hipMemcpyAsync(rfactor, c->rtrace.p, (size_t)nr * 8, hipMemcpyDeviceToHost, c->st));
   NUDIM_HIP(hipStreamSynchronize(c->st));
   c->prof.collect();
   c->nn += count;
@@ -838,10 +904,16 @@ extern "C" int nudim_iterate(nudim_ctx *c, int count, double *rfactor) {
 }
 
 extern "C" int nudim_get_volume(nudim_ctx *c, double *img) {
+  if (c && c->nvol > 1) return fail(NUDIM_ESTATE, "a batch returns each volume with nudim_batch_get_volume");
+  return nudim_batch_get_volume(c, 0, img);
+}
+
+extern "C" int nudim_batch_get_volume(nudim_ctx *c, int vol, double *img) {
   API_BEGIN
   NUDIM_CHECK(c && img, "null argument");
+  NUDIM_CHECK(vol >= 0 && vol < c->nvol, "volume index out of range");
   NUDIM_HIP(hipSetDevice(c->p.device));
-  c->job->get_volume(img);
+  c->job->get_volume(img, vol);
   return NUDIM_OK;
   API_END
 }
@@ -849,6 +921,7 @@ extern "C" int nudim_get_volume(nudim_ctx *c, double *img) {
 extern "C" int nudim_set_volume(nudim_ctx *c, const double *img, int nn) {
   API_BEGIN
   NUDIM_CHECK(c && img && nn >= 0, "invalid argument");
+  if (c->nvol > 1) return fail(NUDIM_ESTATE, "nudim_set_volume applies to single-volume jobs");
   NUDIM_HIP(hipSetDevice(c->p.device));
   c->job->set_volume(img);
   c->nn = nn;
@@ -859,6 +932,7 @@ extern "C" int nudim_set_volume(nudim_ctx *c, const double *img, int nn) {
 
 extern "C" int nudim_set_dump(nudim_ctx *c, int nn, double *before, double *after) {
   if (!c) return fail(NUDIM_EINVAL, "null context");
+  if (c->nvol > 1) return fail(NUDIM_ESTATE, "volume dumps apply to single-volume jobs");
   c->job->dump_nn = nn;
   c->job->dump_before = before;
   c->job->dump_after = after;
@@ -1323,7 +1397,7 @@ static void constraint_parity(double *img, const double *last, int nz, int np, i
   VolTmp<T> a(img, nz, np, np), b(last, nz, np, np);
   NUDIM_HIP(hipStreamSynchronize(b.ts.s));
   hipLaunchKernelGGL(k_constraint<T>, dim3(cdiv(a.V, 256)), dim3(256), 0, a.ts.s, a.u.template as<T>(),
-                     b.u.template as<T>(), np, nz, nx, is_hio, ch, nh, nc);
+                     b.u.template as<T>(), np, nz, nx, is_hio, ch, nh, nc, 1);
   NUDIM_HIP(hipGetLastError());
   a.out(img);
 }
@@ -1416,7 +1490,7 @@ extern "C" int nudim_project_2d1d(int nx, int ny, int nz, int np, double gam_min
     long long nlines = (long long)nx * ny;
     hipLaunchKernelGGL(k_zfft_fwd<double>, dim3(cdiv(nlines, 1 << Gs)), dim3(kFftThreads), lds, ts.s,
                        vol.as<double>(), fh.as<cx<double>>(), f0.as<cx<double>>(), fdz, twz.as<cx<double>>(), Gs,
-                       nlines);
+                       nlines, 1, 0LL);
     NUDIM_HIP(hipGetLastError());
   }
   NUDIM_HIP(hipStreamSynchronize(ts.s));

@@ -30,11 +30,13 @@ void toep_launch(int kind, int n, int nlines, const cx<T> *in, cx<T> *out, long 
 
 // z-axis transforms of [line][z] volumes on the fixed-length kernels
 // (k_zfft2, fft2.hpp); false when n has no specialisation.  The inverse
-// adds sum_{s < nk} alpha[s n + z] pv[s pstride + i] to its input.
+// adds sum_{s < nk} alpha[s n K + plane] pv[s pstride + i] to its input.
+// With K > 1 the real-space side is K volumes vimg elements apart
+// (zimg_base) and the Fourier side has n K planes.
 template <typename T>
 bool zfft2_launch(int n, bool inv, long long nlines, const T *img_in, const cx<T> *g_in, const cx<T> *pv,
                   const double *alpha, cx<T> *fh, cx<T> *f0, T *img_out, const cx<T> *tw, hipStream_t st,
-                  int nk = 0, long long pstride = 0);
+                  int nk = 0, long long pstride = 0, int K = 1, long long vimg = 0);
 
 // FP32 NFFT sizing for a node set (tune.hip): the fastest measured (n, W)
 // whose trafo and adjoint stay within 2e-6 relative L2 of the FP64 NFFT at
@@ -153,18 +155,21 @@ struct ToeplitzOp {
 // a [ny][nx][nz] volume held in u0; u1, g0, g1 are volume-sized scratch, and
 // g0, g1 must hold zeros outside rows ylo..yhi (clear them once when they are
 // allocated).  part_g and part_a hold nb >= tv_blocks(ny, nx, nz) partial
-// sums.  The result is left in u0 (returned); u1's rows are scratch.
+// sums.  The result is left in u0 (returned); u1's rows are scratch.  With
+// K > 1 the buffers hold K volumes one after another (ny nx nz elements
+// apart), part_g / part_a K x nb sums and sums 2 K: every launch covers all
+// K volumes, each with its own norms, exactly as K separate calls would.
 int tv_blocks(int ny, int nx, int nz);
 template <typename T>
 T *tv_sweeps(T *u0, T *u1, T *g0, T *g1, int ny, int nx, int nz, int ylo, int yhi, float lam, int niter,
-             double *part_g, double *part_a, double *sums, int nb, hipStream_t st);
+             double *part_g, double *part_a, double *sums, int nb, hipStream_t st, int K = 1);
 
 // aniso_diffusion_2d (denoise.c:320-378) on a [ny][nx][nz] volume: niter
 // in-place Gauss-Seidel sweeps of every z plane, each plane in LDS when it
 // fits (k_diffusion_plane), else one launch per hyperplane (k_diffusion_tau);
-// both give the same result bit for bit
+// both give the same result bit for bit; K volumes of a batch one after another
 template <typename T>
-void diffusion_sweeps(T *u, int ny, int nx, int nz, int niter, double lam, double ksq, hipStream_t st);
+void diffusion_sweeps(T *u, int ny, int nx, int nz, int niter, double lam, double ksq, hipStream_t st, int K = 1);
 
 // per-plane reduction helpers
 struct Reducer {

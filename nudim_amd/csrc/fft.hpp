@@ -278,6 +278,13 @@ k_fft_lines(const cx<T> *in, cx<T> *out, FftDesc fd, LineIO li, LineIO lo,
   }
 }
 
+// Start of line `line` of the real-space side of the z transforms: K volumes
+// of a batch, vimg elements apart, each [y][x][z]; line = (y x) K + k reads
+// volume k's line (y x), and the Fourier side is [y][x][K][z] (plane k nz + p)
+__device__ __forceinline__ long long zimg_base(long long line, int n, int K, long long vimg) {
+  return K == 1 ? line * n : (line % K) * vimg + (line / K) * n;
+}
+
 // z-axis transforms on volumes stored [y][x][z] (z innermost, lines contiguous).
 // S(z) = (z + nz/2) mod nz is the reference's z shift (reconstruction.c:536,
 // 550, 568, 599, 618).
@@ -289,7 +296,7 @@ k_fft_lines(const cx<T> *in, cx<T> *out, FftDesc fd, LineIO li, LineIO lo,
 template <typename T>
 __global__ void __launch_bounds__(kFftThreads)
 k_zfft_fwd(const T *__restrict__ img, cx<T> *__restrict__ fh, cx<T> *__restrict__ f0, FftDesc fd,
-           const cx<T> *__restrict__ tw, int Gs, long long nlines) {
+           const cx<T> *__restrict__ tw, int Gs, long long nlines, int K, long long vimg) {
   extern __shared__ unsigned char smem_raw[];
   cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
   constexpr int EPT = FftEpt<T>::value;
@@ -306,7 +313,7 @@ k_zfft_fwd(const T *__restrict__ img, cx<T> *__restrict__ fh, cx<T> *__restrict_
     if (line < nlines) {
       int sz = z + h;
       sz -= sz >= nz ? nz : 0;
-      v = img[line * nz + sz];
+      v = img[zimg_base(line, nz, K, vimg) + sz];
       if (f0) f0[line * nz + z] = mk(v, (T)0);
     }
     lds[(z << Gs) + g] = mk(v, (T)0);
@@ -328,7 +335,7 @@ k_zfft_fwd(const T *__restrict__ img, cx<T> *__restrict__ fh, cx<T> *__restrict_
 template <typename T>
 __global__ void __launch_bounds__(kFftThreads)
 k_zfft_inv(const cx<T> *__restrict__ gin, T *__restrict__ img, FftDesc fd, const cx<T> *__restrict__ tw,
-           int Gs, long long nlines) {
+           int Gs, long long nlines, int K, long long vimg) {
   extern __shared__ unsigned char smem_raw[];
   cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
   constexpr int EPT = FftEpt<T>::value;
@@ -358,7 +365,7 @@ k_zfft_inv(const cx<T> *__restrict__ gin, T *__restrict__ img, FftDesc fd, const
     if (line < nlines) {
       int sz = z + h;
       sz -= sz >= nz ? nz : 0;
-      img[line * nz + z] = lds[(sz << Gs) + g].x / (T)nz;
+      img[zimg_base(line, nz, K, vimg) + z] = lds[(sz << Gs) + g].x / (T)nz;
     }
   }
 }

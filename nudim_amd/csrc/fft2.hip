@@ -52,10 +52,10 @@ template void toep_launch<double>(int, int, int, const cx<double> *, cx<double> 
 template <typename T>
 bool zfft2_launch(int n, bool inv, long long nlines, const T *img_in, const cx<T> *g_in, const cx<T> *pv,
                   const double *alpha, cx<T> *fh, cx<T> *f0, T *img_out, const cx<T> *tw, hipStream_t st, int nk,
-                  long long pstride) {
+                  long long pstride, int K, long long vimg) {
 #define NUDIM_ZFFT(NN, A, B)                                                                             \
   case NN:                                                                                               \
-    launch_zfft2<T, A, B>(inv, nlines, img_in, g_in, pv, alpha, fh, f0, img_out, tw, st, nk, pstride); \
+    launch_zfft2<T, A, B>(inv, nlines, img_in, g_in, pv, alpha, fh, f0, img_out, tw, st, nk, pstride, K, vimg); \
     return true;
   if (!g_use_fft2) return false;
   switch (n) {
@@ -63,17 +63,19 @@ bool zfft2_launch(int n, bool inv, long long nlines, const T *img_in, const cx<T
     NUDIM_ZFFT(128, 16, 8)
     NUDIM_ZFFT(256, 16, 16)
     NUDIM_ZFFT(512, 32, 16)
-    default: return zfft2_launch_more<T>(n, inv, nlines, img_in, g_in, pv, alpha, fh, f0, img_out, tw, st, nk, pstride);
+    default:
+      return zfft2_launch_more<T>(n, inv, nlines, img_in, g_in, pv, alpha, fh, f0, img_out, tw, st, nk, pstride, K,
+                                  vimg);
   }
 #undef NUDIM_ZFFT
 }
 
 template bool zfft2_launch<float>(int, bool, long long, const float *, const cx<float> *, const cx<float> *,
                                   const double *, cx<float> *, cx<float> *, float *, const cx<float> *, hipStream_t,
-                                  int, long long);
+                                  int, long long, int, long long);
 template bool zfft2_launch<double>(int, bool, long long, const double *, const cx<double> *, const cx<double> *,
                                    const double *, cx<double> *, cx<double> *, double *, const cx<double> *,
-                                   hipStream_t, int, long long);
+                                   hipStream_t, int, long long, int, long long);
 
 // Returns false when no fixed-length kernel exists for n (the caller then
 // uses the generic Stockham kernel).

@@ -771,15 +771,16 @@ k_toep(const cx<T> *in, cx<T> *out, long long in_so, long long in_sk, long long 
 // r + N2 ((j1 + N1/2) mod N1), S^-1(k1 + N1 k2) = k1 + N1 ((k2 + N2/2) mod N2).
 //   forward (INV false): d[z] = img[S(z)];  D = DFT_-(d);  fh[p] = D[S(p)];
 //                        f0[z] = d[z] (when f0 is non-null)
-//   inverse (INV true):  d[k] = g[S(k)] + sum_{s < nk} alpha[s n + S(k)]
+//   inverse (INV true):  d[k] = g[S(k)] + sum_{s < nk} alpha[(s K + k') n + S(k)]
 //                        pv[s pstride + S(k)] (nk = 0 when pv is null);
 //                        e = DFT_+(d);  img[z] = Re e[S(z)] / n
-// tw holds exp(-2 pi i e / n), e < n.
+// tw holds exp(-2 pi i e / n), e < n.  K volumes of a batch: line (y x) K + k'
+// of the Fourier side is line (y x) of real-space volume k' (zimg_base).
 template <typename T, int N1, int N2, bool INV>
 __global__ void __launch_bounds__(256)
 k_zfft2(const T *__restrict__ img_in, const cx<T> *__restrict__ g_in, const cx<T> *__restrict__ pv,
         const double *__restrict__ alpha, cx<T> *__restrict__ fh, cx<T> *__restrict__ f0, T *__restrict__ img_out,
-        const cx<T> *__restrict__ tw, long long nlines, int nk, long long pstride) {
+        const cx<T> *__restrict__ tw, long long nlines, int nk, long long pstride, int K, long long vimg) {
   constexpr int n = N1 * N2, G = 256 / N1, RS = G * N2 + 1;
   constexpr int S = INV ? +1 : -1;
   static_assert(256 % N1 == 0 && N1 >= N2 && N2 % 2 == 0 && N1 % 8 == 0, "lane layout");
@@ -790,14 +791,15 @@ k_zfft2(const T *__restrict__ img_in, const cx<T> *__restrict__ g_in, const cx<T
   const long long line = (long long)xcd_block(blockIdx.x, gridDim.x) * G + g;
   const bool act = line < nlines;
   const bool actA = act && r < N2;
-  const long long base = line * n;
+  const long long base = line * n;                     // Fourier side
+  const long long ibase = zimg_base(line, n, K, vimg);  // real-space side
   cx<T> a[N1];
   if (actA) {
 #pragma unroll
     for (int j1 = 0; j1 < N1; j1++) {
       const int zs = r + N2 * ((j1 + N1 / 2) % N1);
       if constexpr (!INV) {
-        const T v = img_in[base + zs];
+        const T v = img_in[ibase + zs];
         a[j1] = mk(v, (T)0);
       } else {
         a[j1] = g_in[base + zs];
@@ -810,7 +812,8 @@ k_zfft2(const T *__restrict__ img_in, const cx<T> *__restrict__ g_in, const cx<T
 #pragma unroll 1
       for (int s = 0; s < nk; s++) {
         const cx<T> *ps = pv + s * pstride + base;
-        const double *as = alpha + s * n;
+        // alpha_s holds one value per Fourier plane, n K of them
+        const double *as = alpha + (s * (long long)K + line % K) * n;
 #pragma unroll
         for (int c = 0; c < N1; c += 8) {
           cx<T> pl[8];
@@ -852,7 +855,7 @@ k_zfft2(const T *__restrict__ img_in, const cx<T> *__restrict__ g_in, const cx<T
     for (int k2 = 0; k2 < N2; k2++) {
       const int p = r + N1 * ((k2 + N2 / 2) % N2);
       if constexpr (!INV) fh[base + p] = b[k2];
-      else img_out[base + p] = b[k2].x / (T)n;
+      else img_out[ibase + p] = b[k2].x / (T)n;
     }
   }
 }

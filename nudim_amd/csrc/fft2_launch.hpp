@@ -135,14 +135,14 @@ void launch_toep(int kind, int nlines, const cx<T> *in, cx<T> *out, long long in
 template <typename T, int A, int B>
 void launch_zfft2(bool inv, long long nlines, const T *img_in, const cx<T> *g_in, const cx<T> *pv,
                   const double *alpha, cx<T> *fh, cx<T> *f0, T *img_out, const cx<T> *tw, hipStream_t st, int nk,
-                  long long pstride) {
+                  long long pstride, int K, long long vimg) {
   const dim3 grid((unsigned)((nlines + 256 / A - 1) / (256 / A)));
   if (inv)
     hipLaunchKernelGGL((k_zfft2<T, A, B, true>), grid, dim3(256), 0, st, img_in, g_in, pv, alpha, fh, f0, img_out, tw,
-                       nlines, nk, pstride);
+                       nlines, nk, pstride, K, vimg);
   else
     hipLaunchKernelGGL((k_zfft2<T, A, B, false>), grid, dim3(256), 0, st, img_in, g_in, pv, alpha, fh, f0, img_out,
-                       tw, nlines, nk, pstride);
+                       tw, nlines, nk, pstride, K, vimg);
 }
 
 // the lengths instantiated in fft2_more.hip (line FFT / Toeplitz lengths
@@ -158,7 +158,7 @@ bool toep_launch_more(int kind, int n, int nlines, const cx<T> *in, cx<T> *out, 
 template <typename T>
 bool zfft2_launch_more(int n, bool inv, long long nlines, const T *img_in, const cx<T> *g_in, const cx<T> *pv,
                        const double *alpha, cx<T> *fh, cx<T> *f0, T *img_out, const cx<T> *tw, hipStream_t st,
-                       int nk, long long pstride);
+                       int nk, long long pstride, int K, long long vimg);
 bool toep_supported_more(int n);
 
 }  // namespace nudim

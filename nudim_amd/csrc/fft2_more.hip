@@ -57,10 +57,14 @@ bool toep_launch_more(int kind, int n, int nlines, const cx<T> *in, cx<T> *out, 
 template <typename T>
 bool zfft2_launch_more(int n, bool inv, long long nlines, const T *img_in, const cx<T> *g_in, const cx<T> *pv,
                        const double *alpha, cx<T> *fh, cx<T> *f0, T *img_out, const cx<T> *tw, hipStream_t st,
-                       int nk, long long pstride) {
+                       int nk, long long pstride, int K, long long vimg) {
   switch (n) {
-    case 96: launch_zfft2<T, 16, 6>(inv, nlines, img_in, g_in, pv, alpha, fh, f0, img_out, tw, st, nk, pstride); return true;
-    case 192: launch_zfft2<T, 16, 12>(inv, nlines, img_in, g_in, pv, alpha, fh, f0, img_out, tw, st, nk, pstride); return true;
+    case 96:
+      launch_zfft2<T, 16, 6>(inv, nlines, img_in, g_in, pv, alpha, fh, f0, img_out, tw, st, nk, pstride, K, vimg);
+      return true;
+    case 192:
+      launch_zfft2<T, 16, 12>(inv, nlines, img_in, g_in, pv, alpha, fh, f0, img_out, tw, st, nk, pstride, K, vimg);
+      return true;
     default: return false;
   }
 }
@@ -72,7 +76,8 @@ bool zfft2_launch_more(int n, bool inv, long long nlines, const T *img_in, const
                                     long long, int, hipStream_t, const cx<T> *, const T *, const ToepCg<T> &, int, \
                                     bool);                                                                         \
   template bool zfft2_launch_more<T>(int, bool, long long, const T *, const cx<T> *, const cx<T> *, const double *, \
-                                     cx<T> *, cx<T> *, T *, const cx<T> *, hipStream_t, int, long long);
+                                     cx<T> *, cx<T> *, T *, const cx<T> *, hipStream_t, int, long long, int,         \
+                                     long long);
 NUDIM_MORE(float)
 NUDIM_MORE(double)
 #undef NUDIM_MORE

@@ -621,7 +621,12 @@ __device__ __forceinline__ void st_at(T *__restrict__ p, unsigned b, T v) {
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
 k_tv_grad(const T *__restrict__ u, T *__restrict__ grad, int ny, int nx, int nz, int ylo, int yhi,
-          double *__restrict__ part_g, double *__restrict__ part_a) {
+          double *__restrict__ part_g, double *__restrict__ part_a, long long vs, int pst) {
+  // volume blockIdx.z of a batch: volumes vs elements apart, partial sums pst apart
+  u += blockIdx.z * vs;
+  grad += blockIdx.z * vs;
+  part_g += blockIdx.z * pst;
+  part_a += blockIdx.z * pst;
   __shared__ double sg[kThreads], sa[kThreads];
   const T eps = (T)1e-8;
   double accg = 0, acca = 0;
@@ -682,7 +687,12 @@ k_tv_grad(const T *__restrict__ u, T *__restrict__ grad, int ny, int nx, int nz,
 // over the band rows (dst may be u)
 template <typename T>
 __global__ void k_tv_update(T *dst, const T *u, const T *__restrict__ grad, int nx, int nz, int ylo, int yhi,
-                            const double *__restrict__ sums, double lam) {
+                            const double *__restrict__ sums, double lam, long long vs) {
+  // volume blockIdx.y of a batch, its norm pair at sums + 2 blockIdx.y
+  dst += blockIdx.y * vs;
+  u += blockIdx.y * vs;
+  grad += blockIdx.y * vs;
+  sums += 2 * blockIdx.y;
   const long long rowlen = (long long)nx * nz;
   const long long total = (long long)(yhi - ylo + 1) * rowlen;
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -697,7 +707,7 @@ __global__ void k_tv_update(T *dst, const T *u, const T *__restrict__ grad, int 
 }
 
 __global__ void k_sum_scalars(const double *__restrict__ part_g, const double *__restrict__ part_a, int nb,
-                              double *__restrict__ sums);
+                              double *__restrict__ sums, int pst);
 
 // One fused TV sweep s -> s+1 (denoise.c:83-145 with the next sweep's
 // gradient):  reads u_s and grad_s, forms u_{s+1} = u_s - c_s grad_s/|grad_s|
@@ -710,7 +720,14 @@ template <typename T>
 __global__ void __launch_bounds__(kThreads)
 k_tv_fused(const T *__restrict__ u, const T *__restrict__ g, T *__restrict__ un, T *__restrict__ gn, int ny, int nx,
            int nz, int ylo, int yhi, const double *__restrict__ sums, double lam, double *__restrict__ part_g,
-           double *__restrict__ part_a) {
+           double *__restrict__ part_a, long long vs, int pst) {
+  u += blockIdx.z * vs;
+  g += blockIdx.z * vs;
+  un += blockIdx.z * vs;
+  gn += blockIdx.z * vs;
+  sums += 2 * blockIdx.z;
+  part_g += blockIdx.z * pst;
+  part_a += blockIdx.z * pst;
   __shared__ double sg[kThreads], sa[kThreads];
   const T eps = (T)1e-8;
   const double s1d = sqrt(sums[0]);
@@ -793,7 +810,14 @@ template <typename T, int Q>
 __global__ void __launch_bounds__(kThreads)
 k_tv_fused_q(const T *__restrict__ u, const T *__restrict__ g, T *__restrict__ un, T *__restrict__ gn, int ny,
              int nx, int nz, int ylo, int yhi, const double *__restrict__ sums, double lam,
-             double *__restrict__ part_g, double *__restrict__ part_a) {
+             double *__restrict__ part_g, double *__restrict__ part_a, long long vs, int pst) {
+  u += blockIdx.z * vs;
+  g += blockIdx.z * vs;
+  un += blockIdx.z * vs;
+  gn += blockIdx.z * vs;
+  sums += 2 * blockIdx.z;
+  part_g += blockIdx.z * pst;
+  part_a += blockIdx.z * pst;
   static_assert(Q == 4 || Q == 8, "runs of 4 or 8");
   __shared__ double sg[kThreads], sa[kThreads];
   const T eps = (T)1e-8;
@@ -921,7 +945,14 @@ template <typename T>
 __global__ void __launch_bounds__(kThreads)
 k_tv_march(const T *__restrict__ u, const T *__restrict__ g, T *__restrict__ un, T *__restrict__ gn, int ny, int nx,
            int nz, int ylo, int yhi, int R, const double *__restrict__ sums, double lam, double *__restrict__ part_g,
-           double *__restrict__ part_a) {
+           double *__restrict__ part_a, long long vs, int pst) {
+  u += blockIdx.z * vs;
+  g += blockIdx.z * vs;
+  un += blockIdx.z * vs;
+  gn += blockIdx.z * vs;
+  sums += 2 * blockIdx.z;
+  part_g += blockIdx.z * pst;
+  part_a += blockIdx.z * pst;
   constexpr int Q = 4;
   __shared__ double sg[kThreads], sa[kThreads];
   const T eps = (T)1e-8;
@@ -1050,7 +1081,9 @@ k_tv_march(const T *__restrict__ u, const T *__restrict__ g, T *__restrict__ un,
 }
 
 template <typename T>
-__global__ void k_copy_real(T *__restrict__ d, const T *__restrict__ s, long long total) {
+__global__ void k_copy_real(T *__restrict__ d, const T *__restrict__ s, long long total, long long vs = 0) {
+  d += blockIdx.y * vs;  // volume blockIdx.y of a batch
+  s += blockIdx.y * vs;
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < total) d[i] = s[i];
 }
@@ -1124,8 +1157,10 @@ __global__ void __launch_bounds__(1024)
 k_diffusion_plane(T *__restrict__ u, int ny, int nx, int nz, int niter, double lam, double ksq) {
   extern __shared__ unsigned char smem_raw[];
   T *pl = reinterpret_cast<T *>(smem_raw);
-  const int z = blockIdx.x;
+  // block b: plane z = b mod nz of volume b / nz of a batch
+  const int z = blockIdx.x % nz;
   const int nyx = ny * nx;
+  u += (long long)(blockIdx.x / nz) * nyx * nz;
   for (int i = threadIdx.x; i < nyx; i += blockDim.x) pl[i] = u[(long long)i * nz + z];
   __syncthreads();
   // thread t takes sweeps s = smin + (t >> 7) + 8 k and rows y = ylo + (t & 127) + 128 k
@@ -1156,11 +1191,11 @@ k_diffusion_plane(T *__restrict__ u, int ny, int nx, int nz, int niter, double l
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void k_constraint(T *__restrict__ u, const T *__restrict__ last, int np, int nz, int nx, int is_hio,
-                             int config_height, int nheight, int ncenter) {
+                             int config_height, int nheight, int ncenter, int K = 1) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  long long total = (long long)np * np * nz;
-  if (i >= total) return;
-  long long yx = i / nz;
+  const long long V = (long long)np * np * nz;  // K volumes of a batch one after another
+  if (i >= V * K) return;
+  long long yx = (i % V) / nz;
   int yi = (int)(yx / np), xi = (int)(yx % np);
   bool inside;
   if (config_height)

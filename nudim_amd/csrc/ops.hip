@@ -62,7 +62,11 @@ __global__ void k_cg_scalars(double *__restrict__ dot_z, double *__restrict__ co
 
 // one block of 256 threads; fixed-order tree
 __global__ void k_sum_scalars(const double *__restrict__ part_g, const double *__restrict__ part_a, int nb,
-                              double *__restrict__ sums) {
+                              double *__restrict__ sums, int pst) {
+  // volume blockIdx.x of a batch: its partial sums pst apart, its pair at sums + 2 blockIdx.x
+  part_g += blockIdx.x * pst;
+  part_a += blockIdx.x * pst;
+  sums += 2 * blockIdx.x;
   __shared__ double sa[256], sb[256];
   double a = 0, b = 0;
   // eight strided values in flight, added in index order (the order of a
@@ -916,13 +920,13 @@ int tv_blocks(int ny, int nx, int nz) { return ny * 128; }
 
 template <typename T>
 T *tv_sweeps(T *u0, T *u1, T *g0, T *g1, int ny, int nx, int nz, int ylo, int yhi, float lam, int niter,
-             double *part_g, double *part_a, double *sums, int nb, hipStream_t st) {
+             double *part_g, double *part_a, double *sums, int nb, hipStream_t st, int K) {
   if (niter <= 0 || yhi < ylo) return u0;
   const long long rowlen = (long long)nx * nz;
   const long long V = (long long)ny * rowlen;
   const long long band = (long long)(yhi - ylo + 1) * rowlen;
   const double lamd = (double)lam;
-  const double vb = (double)band * sizeof(T);  // algorithmic: read u once + write u once per sweep
+  const double vb = (double)band * sizeof(T) * K;  // algorithmic: read u once + write u once per sweep
   // grid: (chunks of a row, band rows); one partial sum per block
   const int nrows = yhi - ylo + 1;
   const int gx = (int)std::min<long long>(cdiv(rowlen, kThreads), 64);
@@ -936,85 +940,89 @@ T *tv_sweeps(T *u0, T *u1, T *g0, T *g1, int ny, int nx, int nz, int ylo, int yh
   static const int march_rows = getenv("NUDIM_TV_ROWS") ? atoi(getenv("NUDIM_TV_ROWS")) : 8;
   const bool march = quads && march_rows > 0;
   const int R = std::max(1, march_rows);
-  const dim3 mgrid(march ? (unsigned)cdiv(rowlen / 4, kThreads) : 1u, (unsigned)cdiv(nrows, R));
+  const dim3 mgrid(march ? (unsigned)cdiv(rowlen / 4, kThreads) : 1u, (unsigned)cdiv(nrows, R), K);
   const int npm = (int)(mgrid.x * mgrid.y);
   NUDIM_CHECK(np_ <= nb && nrows * gx4 <= nb && npm <= nb, "TV partial-sum buffer too small");
   NUDIM_CHECK((double)V * sizeof(T) < 4294967296.0, "TV kernels address volumes of fewer than 2^32 bytes");
-  const dim3 grid(gx, nrows);
+  const dim3 grid(gx, nrows, K);
   // the sweeps write band rows only and read one row on each side of the
   // band, so u1 needs those two rows of u0; the last update writes into u0
   if (ylo > 0)
-    hipLaunchKernelGGL(k_copy_real<T>, dim3(cdiv(rowlen, 256)), dim3(256), 0, st, u1 + (ylo - 1) * rowlen,
-                       u0 + (ylo - 1) * rowlen, rowlen);
+    hipLaunchKernelGGL(k_copy_real<T>, dim3(cdiv(rowlen, 256), K), dim3(256), 0, st, u1 + (ylo - 1) * rowlen,
+                       u0 + (ylo - 1) * rowlen, rowlen, V);
   if (yhi < ny - 1)
-    hipLaunchKernelGGL(k_copy_real<T>, dim3(cdiv(rowlen, 256)), dim3(256), 0, st, u1 + (yhi + 1) * rowlen,
-                       u0 + (yhi + 1) * rowlen, rowlen);
+    hipLaunchKernelGGL(k_copy_real<T>, dim3(cdiv(rowlen, 256), K), dim3(256), 0, st, u1 + (yhi + 1) * rowlen,
+                       u0 + (yhi + 1) * rowlen, rowlen, V);
   {
     ProfScope ps(KC_TV, st, vb);
-    hipLaunchKernelGGL(k_tv_grad<T>, grid, dim3(kThreads), 0, st, u0, g0, ny, nx, nz, ylo, yhi, part_g, part_a);
+    hipLaunchKernelGGL(k_tv_grad<T>, grid, dim3(kThreads), 0, st, u0, g0, ny, nx, nz, ylo, yhi, part_g, part_a, V,
+                       nb);
   }
-  hipLaunchKernelGGL(k_sum_scalars, dim3(1), dim3(256), 0, st, part_g, part_a, np_, sums);
+  hipLaunchKernelGGL(k_sum_scalars, dim3(K), dim3(256), 0, st, part_g, part_a, np_, sums, nb);
   T *uc = u0, *un = u1, *gc = g0, *gn = g1;
   for (int s = 0; s + 1 < niter; s++) {
     {
       ProfScope ps(KC_TV, st, 2.0 * vb);
       if (march)
         hipLaunchKernelGGL(k_tv_march<T>, mgrid, dim3(kThreads), 0, st, uc, gc, un, gn, ny, nx, nz, ylo, yhi, R, sums,
-                           lamd, part_g, part_a);
+                           lamd, part_g, part_a, V, nb);
       else if (quads)
-        hipLaunchKernelGGL((k_tv_fused_q<T, 4>), dim3(gx4, nrows), dim3(kThreads), 0, st, uc, gc, un, gn, ny, nx, nz,
-                           ylo, yhi, sums, lamd, part_g, part_a);
+        hipLaunchKernelGGL((k_tv_fused_q<T, 4>), dim3(gx4, nrows, K), dim3(kThreads), 0, st, uc, gc, un, gn, ny, nx,
+                           nz, ylo, yhi, sums, lamd, part_g, part_a, V, nb);
       else
         hipLaunchKernelGGL(k_tv_fused<T>, grid, dim3(kThreads), 0, st, uc, gc, un, gn, ny, nx, nz, ylo, yhi, sums,
-                           lamd, part_g, part_a);
+                           lamd, part_g, part_a, V, nb);
     }
-    hipLaunchKernelGGL(k_sum_scalars, dim3(1), dim3(256), 0, st, part_g, part_a,
-                       march ? npm : (quads ? nrows * gx4 : np_), sums);
+    hipLaunchKernelGGL(k_sum_scalars, dim3(K), dim3(256), 0, st, part_g, part_a,
+                       march ? npm : (quads ? nrows * gx4 : np_), sums, nb);
     std::swap(uc, un);
     std::swap(gc, gn);
   }
   {
     ProfScope ps(KC_TV, st, vb);
-    hipLaunchKernelGGL(k_tv_update<T>, dim3(cdiv(band, 256)), dim3(256), 0, st, u0, uc, gc, nx, nz, ylo, yhi, sums,
-                       lamd);
+    hipLaunchKernelGGL(k_tv_update<T>, dim3(cdiv(band, 256), K), dim3(256), 0, st, u0, uc, gc, nx, nz, ylo, yhi, sums,
+                       lamd, V);
   }
   NUDIM_HIP(hipGetLastError());
   return u0;
 }
 
 template <typename T>
-void diffusion_sweeps(T *u, int ny, int nx, int nz, int niter, double lam, double ksq, hipStream_t st) {
+void diffusion_sweeps(T *u, int ny, int nx, int nz, int niter, double lam, double ksq, hipStream_t st, int K) {
   if (niter <= 0) return;
   // algorithmic: read and write the volume once per sweep
-  ProfScope ps(KC_DIFFUSION, st, 2.0 * niter * (double)ny * nx * nz * sizeof(T));
+  ProfScope ps(KC_DIFFUSION, st, 2.0 * niter * (double)ny * nx * nz * sizeof(T) * K);
   const size_t lds = (size_t)ny * nx * sizeof(T);
   const char *e = getenv("NUDIM_DIFFUSION_LDS");  // "0": the hyperplane launches only
   const bool lds_on = !(e && std::string(e) == "0");
   if (lds_on && lds <= (size_t)160 * 1024) {
-    hipLaunchKernelGGL(k_diffusion_plane<T>, dim3(nz), dim3(1024), lds, st, u, ny, nx, nz, niter, lam, ksq);
+    hipLaunchKernelGGL(k_diffusion_plane<T>, dim3(nz * K), dim3(1024), lds, st, u, ny, nx, nz, niter, lam, ksq);
     NUDIM_HIP(hipGetLastError());
     return;
   }
   const int taumax = 2 * (niter - 1) + (ny - 1) + (nx - 1);
-  for (int tau = 0; tau <= taumax; tau++) {
-    int smin = tau - (ny - 1) - (nx - 1);
-    smin = smin > 0 ? (smin + 1) / 2 : 0;
-    const int smax = std::min(niter - 1, tau / 2);
-    if (smax < smin) continue;
-    const long long cnt = (long long)(smax - smin + 1) * ny * nz;
-    hipLaunchKernelGGL(k_diffusion_tau<T>, dim3(cdiv(cnt, 256)), dim3(256), 0, st, u, ny, nx, nz, niter, tau, lam,
-                       ksq);
+  for (int k = 0; k < K; k++) {
+    T *uk = u + (long long)k * ny * nx * nz;
+    for (int tau = 0; tau <= taumax; tau++) {
+      int smin = tau - (ny - 1) - (nx - 1);
+      smin = smin > 0 ? (smin + 1) / 2 : 0;
+      const int smax = std::min(niter - 1, tau / 2);
+      if (smax < smin) continue;
+      const long long cnt = (long long)(smax - smin + 1) * ny * nz;
+      hipLaunchKernelGGL(k_diffusion_tau<T>, dim3(cdiv(cnt, 256)), dim3(256), 0, st, uk, ny, nx, nz, niter, tau,
+                         lam, ksq);
+    }
   }
   NUDIM_HIP(hipGetLastError());
 }
 
-template void diffusion_sweeps<float>(float *, int, int, int, int, double, double, hipStream_t);
-template void diffusion_sweeps<double>(double *, int, int, int, int, double, double, hipStream_t);
+template void diffusion_sweeps<float>(float *, int, int, int, int, double, double, hipStream_t, int);
+template void diffusion_sweeps<double>(double *, int, int, int, int, double, double, hipStream_t, int);
 
 template float *tv_sweeps<float>(float *, float *, float *, float *, int, int, int, int, int, float, int, double *,
-                                   double *, double *, int, hipStream_t);
+                                   double *, double *, int, hipStream_t, int);
 template double *tv_sweeps<double>(double *, double *, double *, double *, int, int, int, int, int, float, int,
-                                     double *, double *, double *, int, hipStream_t);
+                                     double *, double *, double *, int, hipStream_t, int);
 
 template struct LineFft<float>;
 template struct LineFft<double>;

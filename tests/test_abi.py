@@ -94,3 +94,16 @@ def test_tv_volume_limit_refused_up_front(hip):
                                               precision=hip.FP32)) > 0
     # without denoising the size is not limited by the TV kernels
     assert hip.estimate_bytes(hip.make_params(1024, 1024, 1024, -60, 60, 3)) > 0
+
+
+def test_batch_arguments_checked_before_device_work(hip):
+    """The batch entry points refuse a volume count outside 1..4096 and null
+    contexts with an error code and message (no GPU needed)."""
+    p = hip.make_params(32, 32, 32, -60, 60, 5)
+    for nvol in (0, -2, 4097):
+        with pytest.raises(hip.NudimError, match="1 to 4096 volumes"):
+            hip.Batch(p, nvol)
+    L = hip.lib()
+    assert L.nudim_batch_load_projections(None, 0, None) != 0
+    assert b"null argument" in L.nudim_last_error()
+    assert L.nudim_batch_get_volume(None, 0, None) != 0
