@@ -347,7 +347,7 @@ def serve(args, precision, jobs=None, rate=None, seed=None, classes=None):
     policy = nd.SERVE_FIFO if args.serve_fifo else nd.SERVE_SHORTEST
     # small jobs: the 128^3 class (np 192, 60 passes: 2.8e8 work units)
     with nd.Server(max_active=args.serve_slots, policy=policy, small_cus=args.serve_small_cus,
-                   small_work=5e8) as srv:
+                   small_work=5e8, max_stack=args.serve_stack) as srv:
         meta = {}
         start = time.perf_counter()
         k = 0
@@ -392,6 +392,9 @@ def serve(args, precision, jobs=None, rate=None, seed=None, classes=None):
            "p50_latency_s_by_class": {k: float(np.percentile(v, 50)) for k, v in sorted(by.items())},
            "p99_latency_s_by_class": {k: float(np.percentile(v, 99)) for k, v in sorted(by.items())},
            "calibration_s": calib_s, "small_job_cus": args.serve_small_cus,
+           "max_stack": args.serve_stack if args.serve_stack > 0 else 8,
+           "jobs_run_in_batches": int(sum(1 for r in reps.values() if r["stacked"] > 1)),
+           "largest_batch": int(max(r["stacked"] for r in reps.values())),
            "precision": "f32" if precision == nd.FP32 else "f64",
            "policy": "fifo" if args.serve_fifo else "shortest work first, 30 s aging",
            "mix": "%s; seed %d" % (", ".join("%s %.0f%%" % (n, 100 * f / frac.sum()) for n, f in mix), seed)}
@@ -490,9 +493,10 @@ def batch(args):
 def stacked(args):
     """Many small volumes of one geometry (64^3, np 64, +-60 at step 2, SNR 4,
     ner = 30 with TV over the slab), FP32, end to end (create, load, initial
-    INFFT, passes, volumes back), three ways: one batch context
+    INFFT, passes, volumes back), four ways: one batch context
     (nudim_batch_create: every kernel of a pass covers all volumes), one job
-    after another, and the job server with 8 jobs in flight."""
+    after another, and the job server with 8 slots, every job alone
+    (max_stack 1) or queued jobs batched (its default, up to 8 a batch)."""
     K, ner = args.stack_vols, 30
     vols = []
     for i in range(K):
@@ -516,9 +520,9 @@ def stacked(args):
             job.iterate(ner)
             return job.volume()
 
-    def run_server(projs, slots=8):
+    def run_server(projs, max_stack, slots=8):
         out = [None] * len(projs)
-        with nd.Server(max_active=slots) as srv:
+        with nd.Server(max_active=slots, max_stack=max_stack) as srv:
             ids = {srv.submit(p, pr): k for k, pr in enumerate(projs)}
             for _ in projs:
                 rep, v, _ = srv.next(timeout_ms=-1)
@@ -531,7 +535,8 @@ def stacked(args):
     run_one(vols[0])
     res = {}
     for name, fn in (("batch", lambda: run_batch(vols)), ("one_at_a_time", lambda: [run_one(v) for v in vols]),
-                     ("server_8_in_flight", lambda: run_server(vols))):
+                     ("server_8_slots_alone", lambda: run_server(vols, 1)),
+                     ("server_8_slots_batched", lambda: run_server(vols, 0))):
         t = time.perf_counter()
         out = fn()
         res[name] = (time.perf_counter() - t, out)
@@ -542,7 +547,7 @@ def stacked(args):
             "volume_iters_per_s": {k: K * ner / v[0] for k, v in res.items()},
             "seconds": {k: v[0] for k, v in res.items()},
             "speedup_batch_over_one_at_a_time": res["one_at_a_time"][0] / res["batch"][0],
-            "speedup_batch_over_server": res["server_8_in_flight"][0] / res["batch"][0],
+            "speedup_batch_over_server_alone": res["server_8_slots_alone"][0] / res["batch"][0],
             "batch_volumes_bit_equal_to_own_jobs": bool(same),
             "schedule": "FP32, ner=30 with TV over the slab, end to end per volume (create, load, initial, passes, "
                         "volume back)"}
@@ -569,6 +574,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=4.0, help="mean arrivals per second (Poisson)")
     ap.add_argument("--serve-slots", type=int, default=4, help="jobs in flight at once")
     ap.add_argument("--serve-seed", type=int, default=5)
+    ap.add_argument("--serve-stack", type=int, default=0,
+                    help="most same-parameter jobs the server runs as one batch (0: its default, 8; 1: none)")
     ap.add_argument("--serve-fifo", action="store_true", help="admit in arrival order (default: least work first)")
     ap.add_argument("--serve-small-cus", type=int, default=0,
                     help="CUs reserved for small jobs (128^3 class) in the serving leg (0: none)")

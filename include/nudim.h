@@ -214,7 +214,12 @@ int nudim_project_2d1d(int nx, int ny, int nz, int np, double gam_min, double ga
  * src/reconstruction.c main.)  Jobs are admitted by the server's policy
  * while their device memory (nudim_estimate_bytes) fits the budget; each
  * runs on its own stream and host slot thread, so jobs overlap on the
- * device.  A job's result does not depend on what else runs. */
+ * device.  Pending jobs with the same parameters as an admitted small one
+ * (np^2 nz <= 2^21) join it as one batch (nudim_batch_create; up to
+ * max_stack volumes, 2^23 np^2 nz in all, that fit the budget, in an odd
+ * count unless nz is a multiple of 64), which takes one slot: larger volumes
+ * fill the device alone.  A job's result does not depend on what else
+ * runs. */
 typedef struct nudim_server nudim_server;
 enum { NUDIM_SERVE_FIFO = 0,      /* first pending job (arrival order) that fits */
        NUDIM_SERVE_SHORTEST = 1 }; /* least work (np^2 nz passes) that fits; a job waiting
@@ -230,6 +235,7 @@ typedef struct {
                            their own cu_mask keep it), and small jobs have max_active slots of
                            their own */
   double small_work;    /* np^2 nz passes of the largest small job */
+  int max_stack;        /* most jobs run as one batch; <= 0: 8; 1: every job alone */
 } nudim_server_opts;
 typedef struct {
   long long job_id;
@@ -237,6 +243,7 @@ typedef struct {
   double t_submit, t_start, t_end; /* seconds since the server was created */
   long long device_bytes;          /* the admission estimate */
   char error[256];
+  int stacked;                     /* jobs of the batch it ran in (1: alone) */
 } nudim_job_report;
 int nudim_server_create(const nudim_server_opts *opts, nudim_server **out);
 /* Copies proj [nmeas][nz][nx]; volume [nz][np][np] and rfactors [nhio+ner]

@@ -65,14 +65,14 @@ class ServerOpts(C.Structure):
     """nudim_server_opts (include/nudim.h)"""
     _fields_ = [("hbm_budget", C.c_longlong), ("max_active", C.c_int), ("device", C.c_int),
                 ("policy", C.c_int), ("max_wait_s", C.c_double), ("small_cus", C.c_int),
-                ("small_work", C.c_double)]
+                ("small_work", C.c_double), ("max_stack", C.c_int)]
 
 
 class JobReport(C.Structure):
     """nudim_job_report (include/nudim.h)"""
     _fields_ = [("job_id", C.c_longlong), ("status", C.c_int), ("t_submit", C.c_double),
                 ("t_start", C.c_double), ("t_end", C.c_double), ("device_bytes", C.c_longlong),
-                ("error", C.c_char * 256)]
+                ("error", C.c_char * 256), ("stacked", C.c_int)]
 
 
 def lib():
@@ -429,11 +429,12 @@ class Server:
     """Continuous-batching job server (nudim_server_*): jobs of mixed size are
     admitted under an HBM budget (least work first with an aging bound, or
     arrival order) and run concurrently, one stream and host slot thread
-    each."""
+    each; pending jobs with the same parameters run together as one batch of
+    up to max_stack volumes (1: every job alone)."""
 
     def __init__(self, max_active=4, hbm_budget=0, device=0, policy=SERVE_SHORTEST, max_wait_s=30.0,
-                 small_cus=0, small_work=0.0):
-        o = ServerOpts(hbm_budget, max_active, device, policy, max_wait_s, small_cus, small_work)
+                 small_cus=0, small_work=0.0, max_stack=0):
+        o = ServerOpts(hbm_budget, max_active, device, policy, max_wait_s, small_cus, small_work, max_stack)
         h = C.c_void_p()
         _check(lib().nudim_server_create(C.byref(o), C.byref(h)))
         self._h = h
@@ -461,7 +462,8 @@ class Server:
         _check(rc)
         vol, rf, p = self._jobs.pop(r.job_id)
         rep = {"job_id": r.job_id, "status": r.status, "t_submit": r.t_submit, "t_start": r.t_start,
-               "t_end": r.t_end, "device_bytes": r.device_bytes, "error": r.error.decode(errors="replace")}
+               "t_end": r.t_end, "device_bytes": r.device_bytes, "error": r.error.decode(errors="replace"),
+               "stacked": r.stacked}
         return rep, vol.reshape(p.nz, p.np, p.np), rf[: p.nhio + p.ner]
 
     def stats(self):

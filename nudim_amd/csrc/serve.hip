@@ -6,9 +6,12 @@
 // (geometry and window tables) and transfers overlap other jobs' kernels,
 // and small jobs' kernels overlap large ones'.  Admission follows the
 // server's policy (arrival order, or least work with an aging bound) among
-// the jobs that fit an HBM budget (nudim_estimate_bytes per job).  A
-// job's arithmetic does not depend on what else runs: its result is the same
-// alone and in any mix.
+// the jobs that fit an HBM budget (nudim_estimate_bytes per job).  Pending
+// jobs with the same parameters as the one admitted join it as one batch
+// (nudim_batch_create, up to max_stack volumes, in counts that keep each
+// volume's arithmetic that of its own job), so many small volumes of one
+// geometry share every launch.  A job's arithmetic does not depend on what
+// else runs: its result is the same alone and in any mix.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -38,8 +41,20 @@ struct ServerJob {
   double work = 0;  // np^2 nz passes
   double t_submit = 0, t_start = 0, t_end = 0;
   int status = NUDIM_OK;
+  int stacked = 1;  // volumes of the batch the job ran in
   std::string error;
 };
+
+// the same job apart from its data: every parameter equal
+bool same_params(const nudim_params &a, const nudim_params &b) {
+  return a.nx == b.nx && a.nz == b.nz && a.np == b.np && a.gam_min == b.gam_min && a.gam_max == b.gam_max &&
+         a.interval == b.interval && a.padangle == b.padangle && a.weight == b.weight && a.random == b.random &&
+         a.seed == b.seed && a.config_height == b.config_height && a.nheight == b.nheight &&
+         a.ncenter == b.ncenter && a.denoise == b.denoise && a.method == b.method && a.nhio == b.nhio &&
+         a.ner == b.ner && a.precision == b.precision && a.device == b.device && a.nfft_n == b.nfft_n &&
+         a.nfft_w == b.nfft_w && a.cg_form == b.cg_form && a.use_graph == b.use_graph && a.tv_full == b.tv_full &&
+         a.diffusion_lambda == b.diffusion_lambda && std::memcmp(a.cu_mask, b.cu_mask, sizeof(a.cu_mask)) == 0;
+}
 
 }  // namespace
 
@@ -64,8 +79,49 @@ struct nudim_server {
   // slot of their kind (caller holds mu): FIFO takes the first in arrival
   // order; SHORTEST takes the least work, except that a job waiting longer
   // than max_wait_s (oldest first) goes before it.  With the small-job
-  // partition, small and other jobs have max_active slots each.
-  ServerJob *take() {
+  // partition, small and other jobs have max_active slots each.  Pending
+  // jobs with the same parameters (in arrival order, while they fit the
+  // budget) join a small volume, up to max_stack volumes of kStackElems
+  // Fourier elements in all and in a count for which batch_bit_exact holds;
+  // the group takes one slot.
+  //
+  // Measured per-volume gain of a batch over one job at a time (FP32, TV,
+  // K = 3 / 5 / 8): 2.7 / 3.9 / 5.1x at 64^3, 2.0 / 2.3 / 2.6x at 96^3,
+  // 1.5x at 128^3 (np 128), 1.2x at 128^3 with np 192 and 1.0-1.1x at
+  // 192^3, whose kernels fill the device alone; there a batch only makes
+  // all its jobs finish with the last one.
+  static constexpr double kStackVolume = 2097152.0;  // np^2 nz of a volume that may join a batch
+  static constexpr double kStackElems = 8388608.0;   // np^2 nz of a whole batch
+  std::vector<ServerJob *> take() {
+    std::vector<ServerJob *> group;
+    ServerJob *j = take_one();
+    if (!j) return group;
+    group.push_back(j);
+    const double elems = (double)j->p.np * j->p.np * j->p.nz;
+    const int cap = elems > kStackVolume ? 1 : std::min(o.max_stack, std::max(1, (int)(kStackElems / elems)));
+    for (auto it = pending.begin(); it != pending.end() && (int)group.size() < cap;) {
+      if (same_params((*it)->p, j->p) && used + (*it)->bytes <= (size_t)o.hbm_budget) {
+        used += (*it)->bytes;
+        group.push_back(*it);
+        it = pending.erase(it);
+      } else {
+        ++it;
+      }
+    }
+    // an even count at nz not a multiple of 64 gives its last job back, to
+    // its place in arrival order
+    if (!nudim::batch_bit_exact(j->p.nz, (int)group.size())) {
+      ServerJob *back = group.back();
+      group.pop_back();
+      used -= back->bytes;
+      auto at = std::find_if(pending.begin(), pending.end(), [&](const ServerJob *x) { return x->id > back->id; });
+      pending.insert(at, back);
+    }
+    for (ServerJob *g : group) g->stacked = (int)group.size();
+    return group;
+  }
+
+  ServerJob *take_one() {
     auto best = pending.end();
     const double t = now();
     const bool room_small = active_small < o.max_active, room_large = active - active_small < o.max_active;
@@ -116,24 +172,64 @@ struct nudim_server {
     std::vector<double>().swap(j->proj);
   }
 
+  // a group of jobs with the same parameters as one batch context
+  void run_batch(const std::vector<ServerJob *> &g) {
+    ServerJob *j = g[0];
+    if (partition) {
+      bool own = false;
+      for (unsigned w : j->p.cu_mask) own |= w != 0;
+      if (!own) {
+        const unsigned *m = j->work <= o.small_work ? small_mask : large_mask;
+        for (ServerJob *x : g) std::memcpy(x->p.cu_mask, m, sizeof(small_mask));
+      }
+    }
+    const int K = (int)g.size(), npass = j->p.nhio + j->p.ner;
+    nudim_ctx *c = nullptr;
+    int rc = nudim_batch_create(&j->p, K, &c);
+    for (int k = 0; k < K && rc == NUDIM_OK; k++) rc = nudim_batch_load_projections(c, k, g[k]->proj.data());
+    if (rc == NUDIM_OK) rc = nudim_initial(c);
+    std::vector<double> r((size_t)std::max(npass, 1) * K);
+    if (rc == NUDIM_OK) rc = nudim_iterate(c, npass, r.data());
+    for (int k = 0; k < K && rc == NUDIM_OK; k++) rc = nudim_batch_get_volume(c, k, g[k]->volume);
+    const std::string err = rc == NUDIM_OK ? std::string() : std::string(nudim_last_error());
+    for (int k = 0; k < K; k++) {
+      ServerJob *x = g[k];
+      if (rc != NUDIM_OK) {
+        x->status = rc;
+        x->error = err;
+      } else if (x->rfactors) {
+        for (int i = 0; i < npass; i++) x->rfactors[i] = r[(size_t)i * K + k];
+      }
+      std::vector<double>().swap(x->proj);
+    }
+    nudim_destroy(c);
+  }
+
   void slot_main() {
     for (;;) {
-      ServerJob *j = nullptr;
+      std::vector<ServerJob *> g;
       {
         std::unique_lock<std::mutex> lk(mu);
         // a stopping server still runs what is pending
-        cv_work.wait(lk, [&] { return (j = take()) != nullptr || (stop && pending.empty()); });
-        if (!j) return;
+        cv_work.wait(lk, [&] { return !(g = take()).empty() || (stop && pending.empty()); });
+        if (g.empty()) return;
       }
-      j->t_start = now();
-      run(j);
-      j->t_end = now();
+      const double t0 = now();
+      for (ServerJob *x : g) x->t_start = t0;
+      if (g.size() == 1)
+        run(g[0]);
+      else
+        run_batch(g);
+      const double t1 = now();
       {
         std::lock_guard<std::mutex> lk(mu);
-        used -= j->bytes;
+        for (ServerJob *x : g) {
+          x->t_end = t1;
+          used -= x->bytes;
+          done.push_back(x);
+        }
         active--;
-        if (small(j)) active_small--;
-        done.push_back(j);
+        if (small(g[0])) active_small--;
       }
       cv_done.notify_all();
       cv_work.notify_all();
@@ -151,6 +247,7 @@ extern "C" int nudim_server_create(const nudim_server_opts *o, nudim_server **ou
     std::unique_ptr<nudim_server> s(new nudim_server());
     s->o = *o;
     if (s->o.max_wait_s <= 0) s->o.max_wait_s = 30.0;
+    if (s->o.max_stack <= 0) s->o.max_stack = 8;
     if (s->o.policy != NUDIM_SERVE_FIFO && s->o.policy != NUDIM_SERVE_SHORTEST) {
       nudim::set_last_error("unknown server policy");
       return NUDIM_EINVAL;
@@ -268,6 +365,7 @@ extern "C" int nudim_server_next(nudim_server *s, int timeout_ms, nudim_job_repo
   rep->t_start = j->t_start;
   rep->t_end = j->t_end;
   rep->device_bytes = (long long)j->bytes;
+  rep->stacked = j->stacked;
   std::strncpy(rep->error, j->error.c_str(), sizeof(rep->error) - 1);
   delete j;
   return NUDIM_OK;

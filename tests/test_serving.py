@@ -148,3 +148,42 @@ def test_graph_jobs_in_a_mix_run_directly(hip):
             assert rep["status"] == 0, rep["error"]
             if ids[rep["job_id"]] == "graph":
                 assert np.array_equal(vol, v0) and np.array_equal(rf, r0)
+
+
+def blocker(hip):
+    """a job that keeps a one-slot server busy while the test queues others"""
+    p = hip.make_params(128, 128, 128, -60, 60, 3, padangle=1.0, weight=True, nhio=0, ner=10)
+    return p, job_inputs(hip, 128, 128, 99, interval=3)
+
+
+@pytest.mark.parametrize("nz,njobs,max_stack,expect", [(64, 5, 0, [5]), (32, 4, 0, [3, 1]), (32, 5, 0, [5]),
+                                                        (64, 5, 1, [1] * 5), (64, 5, 2, [2, 2, 1])])
+def test_same_parameter_jobs_run_as_one_batch(hip, nz, njobs, max_stack, expect):
+    """Jobs with the same parameters queued behind a running one run
+    together as one batch context (reported in 'stacked'): all of them at
+    nz = 64; at nz = 32 an odd count (four queued jobs run as three and then
+    one, since an even count at nz = 32 would group the reductions' planes
+    differently); max_stack bounds the batch.  Every job's volume and R
+    trace equal its own job's bit for bit."""
+    p = job_params(hip, 64, nz, 1, True, 1)
+    projs = [job_inputs(hip, 64, nz, 300 + k) for k in range(njobs)]
+    ref = [alone(hip, p, pr) for pr in projs]
+    bp, bproj = blocker(hip)
+    with hip.Server(max_active=1, max_stack=max_stack) as srv:
+        srv.submit(bp, bproj)
+        ids = {srv.submit(p, pr): k for k, pr in enumerate(projs)}
+        got = {}
+        for _ in range(njobs + 1):
+            rep, vol, rf = srv.next(timeout_ms=120000)
+            assert rep["status"] == 0, rep["error"]
+            if rep["job_id"] in ids:
+                got[ids[rep["job_id"]]] = (rep, vol, rf)
+    groups = {}
+    for k, (rep, vol, rf) in got.items():
+        assert np.array_equal(vol, ref[k][0]), "job %d differs in its batch" % k
+        assert np.array_equal(rf, ref[k][1])
+        groups.setdefault(rep["t_start"], []).append(rep["stacked"])
+    sizes = sorted((len(v) for v in groups.values()), reverse=True)
+    assert sizes == expect, groups
+    for v in groups.values():
+        assert all(s == len(v) for s in v)
