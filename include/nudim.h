@@ -111,7 +111,8 @@ int nudim_set_dump(nudim_ctx *ctx, int nn, double *before, double *after);
  * volumes fill the device.  Each volume's result equals its own job's to
  * rounding (bit for bit where the plane grouping of the reductions is the
  * same, e.g. nz a multiple of 64 or nvol odd).  Device memory is about nvol
- * times a single job's.  nudim_initial and nudim_iterate work on the whole
+ * times a single job's; a batch whose NFFT grid n^2 nz nvol reaches 2^31
+ * elements is refused (NUDIM_EINVAL).  nudim_initial and nudim_iterate work on the whole
  * batch (rfactor[count][nvol]); nudim_load_projections, nudim_get_volume,
  * nudim_load_samples, nudim_set_volume and nudim_set_dump refuse a batch
  * (NUDIM_ESTATE).  A batch of one volume is a plain job. */

@@ -772,6 +772,15 @@ static int create_job(const nudim_params *p, int nvol, nudim_ctx **out) {
   *out = nullptr;
   validate(p);
   NUDIM_CHECK(nvol >= 1 && nvol <= 4096, "a batch holds 1 to 4096 volumes");
+  if (nvol > 1) {
+    // the largest single jobs run (np = 768, nz = 512: n^2 nz = 1.2e9) stay
+    // below 2^31 grid elements; a batch is held to the same bound
+    int n, W, form;
+    resolve_knobs(p, &n, &W, &form);
+    NUDIM_CHECK((double)n * n * p->nz * nvol < 2147483648.0,
+                "a batch's NFFT grid (n^2 nz nvol = " + std::to_string((double)n * n * p->nz * nvol) +
+                    " elements) must stay below 2^31");
+  }
   std::unique_ptr<nudim_ctx> c(new nudim_ctx());
   c->p = *p;
   c->nvol = nvol;

@@ -103,6 +103,10 @@ def test_batch_arguments_checked_before_device_work(hip):
     for nvol in (0, -2, 4097):
         with pytest.raises(hip.NudimError, match="1 to 4096 volumes"):
             hip.Batch(p, nvol)
+    # two 512^3 volumes (np 768, n 1536): 2.4e9 grid elements, over 2^31
+    big = hip.make_params(768, 512, 768, -60, 60, 1, precision=hip.FP32)
+    with pytest.raises(hip.NudimError, match="below 2\\^31"):
+        hip.Batch(big, 2)
     L = hip.lib()
     assert L.nudim_batch_load_projections(None, 0, None) != 0
     assert b"null argument" in L.nudim_last_error()
