@@ -654,6 +654,7 @@ def main():
 
     rate64 = None
     acc = {}
+    spreading = {}
     if not args.no_plain:
         log("FP64 plain path:")
         j64 = make_job(prm, proj, nd.FP64_PLAIN)
@@ -690,6 +691,22 @@ def main():
                               "rfactor_fp64": [float(v) for v in r64], "rfactor_fp32": [float(v) for v in r32a]}}
             log("  rel L2 FP32 vs FP64: initial %.2e; after %d HIO+TV passes %.2e (FP64 vs itself after a 1e-7 "
                 "perturbation: %.2e)" % (acc["rel_l2_initial"], npass, acc["hio_tv"]["rel_l2_fp32_vs_fp64"], floor))
+        # the FP64 plain pass's spreading kernels (12 interpolations and 11
+        # gathers per pass): HIP events on the job's stream, one pass each
+        n64, w64 = NFFT_CHOICE["f64"]["n"], NFFT_CHOICE["f64"]["W"]
+        for kc, what in ((2, "interpolation: k_interp_block<double> + k_interp_sum<double>, W=%d, n=%d"),
+                         (3, "gather adjoint: k_gather_patch<double>, W=%d, n=%d")):
+            _, _, (kms, klaunch, kbytes) = timed(j64, 1, kc)
+            if not klaunch:
+                continue
+            avg_us = 1e3 * kms / klaunch
+            gbs = kbytes / klaunch / (avg_us * 1e-6) / 1e9
+            spreading[("interp", "gather")[kc - 2]] = {
+                "kernels": what % (w64, n64), "launches_per_pass": klaunch, "avg_us": avg_us,
+                "bytes_per_launch": kbytes / klaunch, "achieved_GBs": gbs, "frac_of_peak": gbs / HBM_PEAK_GBS,
+                "ms_per_pass": kms}
+            log("  FP64 %s: %d launches, avg %.1f us, %.0f GB/s algorithmic (%.2f of peak)"
+                % (what % (w64, n64), klaunch, avg_us, gbs, gbs / HBM_PEAK_GBS))
         j64.close()
     j32.close()
     if not args.no_plain and not args.no_accuracy:
@@ -768,6 +785,7 @@ def main():
                    "grid": [prm["nz"], prm["np_"], prm["np_"]], "nfft": NFFT_CHOICE or None},
         "graph": graph,
         "fp64_plain_passes_per_s": rate64,
+        "fp64_spreading": spreading or None,
         "speedup_vs_fp64_plain": (rate32 / rate64) if rate64 else None,
         "accuracy": acc or None,
         "roofline": {"kernel": kd["name"], "bound": "hbm", "achieved": kd["achieved_GBs"], "peak": HBM_PEAK_GBS,

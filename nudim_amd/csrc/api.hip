@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "engine.hpp"
+#include "spread.hpp"
 #include "nudim.h"
 
 using namespace nudim;
@@ -986,6 +987,17 @@ extern "C" int nudim_estimate_bytes(const nudim_params *p, size_t *bytes) {
   size_t b = 0;
   b += (size_t)N * n * P * cb + (size_t)n * n * P * cb;           // NFFT X, grid
   b += (size_t)M * W * W * (4 + rb) + (size_t)n * n * 4 + M * W * 16;  // gather maps, window tables
+  {
+    // patch lists (records of both kernels) and the interpolation's partial
+    // sums, at most one entry per patch band pair a window meets
+    auto entries = [&](int pr, int pc) { return M * (size_t)((W + pr - 2) / pr + 1) * ((W + pc - 2) / pc + 1); };
+    const int pr = kPatchR, pc = kPatchC;
+    if (P % 64 == 0 && n % pc == 0 && n >= W + pc)
+      b += entries(pr, pc) * (cb == 16 ? sizeof(PatchRec<double>) : sizeof(PatchRec<float>)) + (size_t)n * n / (pr * pc) * 8;
+    if (P % 64 == 0 && n % (2 * pc) == 0 && n >= W + 2 * pc)
+      b += entries(2 * pr, 2 * pc) * (P * cb + 4 + 4 * (cb == 16 ? sizeof(PatchRec<double>) : sizeof(PatchRec<float>))) +
+           (size_t)n * n / (4 * pr * pc) * 8;
+  }
   if (form == NUDIM_CG_TOEPLITZ) b += (size_t)4 * N * N * (cb + rb) + (size_t)N * 2 * N * P * cb + NN * cb;
   if (projected_start(form, (int)N))  // T_U, X2, b_K, R partials, known list; c_K and the quadratic form's partials
     b += (size_t)4 * N * N * rb + (size_t)N * 2 * N * P * cb + NN * cb + (size_t)2 * 1024 * P * 8 + M * 4 +

@@ -101,9 +101,10 @@ inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 // Whether a batch of nvol volumes of nz planes runs each volume with the same
 // arithmetic as its own job.  The per-plane reductions group up to 64 planes
 // (the largest power of two dividing the plane count, at most 64) and the
-// interpolation takes its tiled form at plane counts that are multiples of
-// 64; nvol nz gives the same choices as nz when nvol is odd or nz is a
-// multiple of 64.  (Other batches agree with their own jobs to rounding.)
+// NFFT interpolation takes its patch form (spread.hpp) at plane counts that
+// are multiples of 64; nvol nz gives the same choices as nz when nvol is odd
+// or nz is a multiple of 64.  (Other batches agree with their own jobs to
+// rounding.)
 inline bool batch_bit_exact(int nz, int nvol) { return nvol % 2 == 1 || nz % 64 == 0; }
 
 // Kernel classes that can be timed with HIP events on the launching stream.

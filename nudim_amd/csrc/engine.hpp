@@ -76,10 +76,14 @@ struct NfftGeom {
   // tile of their first tap, tiles in Morton (Z) order, so that the nodes of
   // a block, and the blocks of an XCD, read the same grid cells from L2
   DevBuf perm;
-  // tiled gather map (built when n % kGatherTile == 0; used when P % 64 == 0)
-  bool has_tiles = false;
-  int nchunk = 0, ncomb = 0, npart = 0;
-  DevBuf chunk_tile, chunk_part, chunk_node, tnode, chunk_cell, tent, comb;
+  // patch lists of the patch-form kernels (spread.hpp; used when P % 64 == 0):
+  // the gather's 4 x 8-cell patches (order, heaviest first, entry ranges,
+  // PatchRec<T> records) and the interpolation's 8 x 16-cell block patches
+  // (order of the nonempty ones, heaviest first, entry ranges, four
+  // PatchRec<T> quarter records per entry, each node's entries)
+  bool has_patches = false, has_blocks = false;
+  int npatch = 0, nbitem = 0, nbentry = 0;
+  DevBuf porder, pptr, grec, border, bptr, irec, nptr, nent;
   size_t bytes() const;
 };
 template <typename T>
@@ -90,8 +94,9 @@ template <typename T>
 struct NfftOp {
   int N = 0, n = 0, W = 0, M = 0, P = 0;
   std::shared_ptr<const NfftGeom<T>> g;  // shared node tables
-  bool tiled = false;                    // tiled gather map in use (P % 64 == 0)
-  DevBuf part;                           // its partial sums [npart][tile cells][P]
+  bool patched = false;                  // patch-form gather in use (P % 64 == 0)
+  bool blocked = false;                  // block-patch interpolation in use (P % 64 == 0)
+  DevBuf part;                           // the interpolation's partial sums [entry][P]
   DevBuf X, grid;                        // [N][n][P], [n][n][P]
   size_t bytes() const;
   void init(int N, int n, int W, const double *x, int M, int P);

@@ -172,82 +172,73 @@ void build_nfft_tables(int N, int n, int W, const double *x, int M, bool with_ga
     }
 }
 
-void build_gather_tiles(const NfftTables &t, int ts, int k, GatherTiles *g) {
+bool build_patch_lists(const NfftTables &t, int pr, int pc, PatchLists *pl) {
   const int n = t.n, W = t.W, M = t.M;
-  if (n % ts != 0) throw std::runtime_error("gather tiles: grid size not a multiple of the tile size");
-  if (k < 1 || k > 65536) throw std::runtime_error("gather tiles: chunk size out of range");
-  const int nt = n / ts, ntile = nt * nt, tt = ts * ts;
-  g->ts = ts; g->k = k; g->ntile = ntile;
-  // tile -> nodes meeting it, ascending (nodes are visited in order, and a
-  // node is appended to a tile at most once)
-  std::vector<std::vector<int>> tn(ntile);
-  std::vector<int> mark(ntile, -1);
+  if (pr < 1 || pc < 1 || n % pr != 0 || n % pc != 0 || n < W + std::max(pr, pc)) return false;
+  if (W + std::max(pr, pc) - 2 >= 64 || M >= (1 << 20)) return false;
+  const int npr = n / pr, npc = n / pc, l0 = W + 2 * (pr - 1), l1 = W + 2 * (pc - 1);
+  const size_t npatch = (size_t)npr * npc;
+  // patch rows (columns) a node's window meets, with the window tap of the
+  // patch's first row (column): d in [-(p-1), W-1]
+  auto bands = [n, W](int c0, int p, std::vector<std::pair<int, int>> *out) {
+    out->clear();
+    for (int k = 0; k < W; k++) {
+      const int band = pmod(c0 + k, n) / p;
+      bool seen = false;
+      for (auto &b : *out) seen = seen || b.first == band;
+      if (seen) continue;
+      int d = pmod(band * p - c0, n);
+      if (d > W - 1) d -= n;
+      out->push_back({band, d});
+    }
+  };
+  std::vector<int> cnt(npatch + 1, 0);
+  std::vector<std::pair<int, int>> rb, cb;
+  for (int m = 0; m < M; m++) {
+    bands(t.c0[m], pr, &rb);
+    bands(t.c1[m], pc, &cb);
+    for (auto &r : rb)
+      for (auto &c : cb) cnt[(size_t)r.first * npc + c.first + 1]++;
+  }
+  for (size_t q = 0; q < npatch; q++) cnt[q + 1] += cnt[q];
+  if (cnt[npatch] > INT32_MAX / 2) return false;
+  pl->pr = pr; pl->pc = pc; pl->npr = npr; pl->npc = npc; pl->l0 = l0; pl->l1 = l1;
+  pl->ptr.assign(cnt.begin(), cnt.end());
+  pl->ent.assign((size_t)cnt[npatch], 0);
+  std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+  std::vector<int> ecount(M + 1, 0);
+  for (int m = 0; m < M; m++) {
+    bands(t.c0[m], pr, &rb);
+    bands(t.c1[m], pc, &cb);
+    for (auto &r : rb)
+      for (auto &c : cb) {
+        const int k0 = r.second + pr - 1, k1 = c.second + pc - 1;
+        pl->ent[fill[(size_t)r.first * npc + c.first]++] = (uint32_t)m | ((uint32_t)k0 << 20) | ((uint32_t)k1 << 26);
+      }
+    ecount[m + 1] = (int)(rb.size() * cb.size());
+  }
+  // each node's entries in increasing patch id (patches are visited in order)
+  for (int m = 0; m < M; m++) ecount[m + 1] += ecount[m];
+  pl->node_ptr = ecount;
+  pl->node_ent.assign((size_t)cnt[npatch], 0);
+  std::vector<int> nfill(ecount.begin(), ecount.end() - 1);
+  for (size_t q = 0; q < npatch; q++)
+    for (int e = pl->ptr[q]; e < pl->ptr[q + 1]; e++) pl->node_ent[nfill[pl->ent[e] & 0xFFFFFu]++] = e;
+  pl->order.resize(npatch);
+  for (size_t q = 0; q < npatch; q++) pl->order[q] = (int)q;
+  std::stable_sort(pl->order.begin(), pl->order.end(), [&](int a, int b) {
+    return pl->ptr[a + 1] - pl->ptr[a] > pl->ptr[b + 1] - pl->ptr[b];
+  });
+  pl->nonempty = 0;
+  for (size_t q = 0; q < npatch; q++) pl->nonempty += pl->ptr[q + 1] > pl->ptr[q];
+  pl->psi0p.assign((size_t)M * l0, 0.0);
+  pl->psi1p.assign((size_t)M * l1, 0.0);
   for (int m = 0; m < M; m++)
-    for (int a = 0; a < W; a++) {
-      int ty = pmod(t.c0[m] + a, n) / ts;
-      for (int b = 0; b < W; b++) {
-        int tl = ty * nt + pmod(t.c1[m] + b, n) / ts;
-        if (mark[tl] != m) { mark[tl] = m; tn[tl].push_back(m); }
-      }
+    for (int k = 0; k < W; k++) {
+      pl->psi0p[(size_t)m * l0 + k + pr - 1] = t.psi0[(size_t)m * W + k];
+      pl->psi1p[(size_t)m * l1 + k + pc - 1] = t.psi1[(size_t)m * W + k];
     }
-  g->tile_chunk.assign(ntile + 1, 0);
-  g->chunk_node.assign(1, 0);
-  g->node.clear();
-  for (int tl = 0; tl < ntile; tl++) {
-    const std::vector<int> &v = tn[tl];
-    int nch = ((int)v.size() + k - 1) / k;
-    for (int c = 0; c < nch; c++) {
-      int a = c * k, b = std::min((int)v.size(), a + k);
-      g->node.insert(g->node.end(), v.begin() + a, v.begin() + b);
-      g->chunk_node.push_back((int)g->node.size());
-    }
-    g->tile_chunk[tl + 1] = g->tile_chunk[tl] + nch;
-  }
-  const int nchunk = g->tile_chunk[ntile];
-  g->chunk_tile.assign(nchunk, 0);
-  g->chunk_part.assign(nchunk, -1);
-  g->comb.clear();
-  g->npart = 0;
-  for (int tl = 0; tl < ntile; tl++) {
-    const int cb = g->tile_chunk[tl], ce = g->tile_chunk[tl + 1];
-    for (int c = cb; c < ce; c++) g->chunk_tile[c] = tl;
-    if (ce - cb != 1) {
-      g->comb.push_back(tl);
-      g->comb.push_back(g->npart);
-      g->comb.push_back(ce - cb);
-      for (int c = cb; c < ce; c++) g->chunk_part[c] = g->npart++;
-    }
-  }
-  g->chunk_cell.assign((size_t)nchunk * (tt + 1), 0);
-  g->slot.clear();
-  g->weight.clear();
-  g->max_chunk_entries = 0;
-  // per chunk and cell: the cell's taps (already in node order in the
-  // per-cell map) whose node lies in the chunk
-  for (int tl = 0; tl < ntile; tl++) {
-    const int ty = tl / nt, tx = tl % nt;
-    const int cb = g->tile_chunk[tl], ce = g->tile_chunk[tl + 1];
-    for (int c = cb; c < ce; c++) {
-      const int *nb = g->node.data() + g->chunk_node[c], *ne = g->node.data() + g->chunk_node[c + 1];
-      const size_t e_start = g->slot.size();
-      int *cc = g->chunk_cell.data() + (size_t)c * (tt + 1);
-      for (int cl = 0; cl < tt; cl++) {
-        cc[cl] = (int)g->slot.size();
-        size_t cell = (size_t)(ty * ts + cl / ts) * n + (size_t)(tx * ts + cl % ts);
-        for (int e = t.cell_ptr[cell]; e < t.cell_ptr[cell + 1]; e++) {
-          uint32_t u = t.ent[e];
-          int m = (int)(u & 0xFFFFFu), t0 = (int)((u >> 20) & 63u), t1 = (int)(u >> 26);
-          if (m < *nb || m > *(ne - 1)) continue;
-          const int *it = std::lower_bound(nb, ne, m);
-          g->slot.push_back((uint16_t)(it - nb));
-          g->weight.push_back(t.psi0[(size_t)m * W + t0] * t.psi1[(size_t)m * W + t1]);
-        }
-      }
-      cc[tt] = (int)g->slot.size();
-      g->max_chunk_entries = std::max(g->max_chunk_entries, (int)(g->slot.size() - e_start));
-    }
-  }
-  if (g->slot.size() > (size_t)INT32_MAX) throw std::runtime_error("gather tiles: too many entries");
+  return true;
 }
 
 int next_pow2(int v) {

@@ -49,28 +49,26 @@ struct NfftTables {
 void build_nfft_tables(int N, int n, int W, const double *x, int M, bool with_gather,
                        NfftTables *t);
 
-// Tiled form of the gather inverse map.  The n x n grid is cut into tiles of
-// ts x ts cells (n % ts == 0), tile (ty, tx) = ty * (n/ts) + tx.  The nodes
-// whose W x W footprint meets a tile, in increasing node order, are cut into
-// chunks of at most k nodes; a chunk's node values are what one block stages
-// in LDS.  Within a chunk the entries are grouped by cell (row-major within
-// the tile) and, per cell, kept in node order.  A tile with one chunk is
-// summed straight into the grid; the chunks of a tile with several write
-// partial sums (slot chunk_part[c]) that a second pass adds in chunk order,
-// and that pass also clears the tiles no node meets.
-struct GatherTiles {
-  int ts = 0, k = 0, ntile = 0, max_chunk_entries = 0, npart = 0;
-  std::vector<int> tile_chunk;      // [ntile+1]: chunk ids of each tile
-  std::vector<int> chunk_node;      // [nchunk+1]: range of each chunk in node
-  std::vector<int> node;            // node ids
-  std::vector<int> chunk_cell;      // [nchunk][ts*ts+1]: absolute entry ranges per cell
-  std::vector<uint16_t> slot;       // entry -> node slot within its chunk
-  std::vector<double> weight;       // entry -> psi0[node][t0] * psi1[node][t1]
-  std::vector<int> chunk_tile;      // [nchunk]
-  std::vector<int> chunk_part;      // [nchunk]: partial-sum slot, -1 for a tile's only chunk
-  std::vector<int> comb;            // tiles with != 1 chunk: (tile, first slot, count) triples
+// Patch lists of the spreading kernels.  The n x n grid is cut into patches
+// of pr x pc cells (n % pr == n % pc == 0), patch (a, b) = a * (n/pc) + b,
+// each held in registers by one wavefront.  A patch's entries are the nodes
+// whose W x W window meets it, in increasing node order, each with the
+// offsets k0, k1 of the patch's first row and column in the node's
+// zero-padded window tables: row i of the patch takes psi0p[node][k0 + i],
+// where psi0p[node][k] = psi0[node][k - (pr - 1)] inside the window and 0
+// outside (length W + 2 (pr - 1)); columns likewise with pc.
+struct PatchLists {
+  int pr = 0, pc = 0, npr = 0, npc = 0, l0 = 0, l1 = 0, nonempty = 0;
+  std::vector<int> ptr;             // [npatch + 1]: entry range of each patch
+  std::vector<uint32_t> ent;        // node | k0 << 20 | k1 << 26
+  std::vector<int> order;           // patches by entry count, largest first (ties: patch id)
+  std::vector<double> psi0p, psi1p; // [M][l0], [M][l1]
+  std::vector<int> node_ptr;        // [M + 1]
+  std::vector<int> node_ent;        // each node's entry ids, in increasing patch id
 };
-void build_gather_tiles(const NfftTables &t, int ts, int k, GatherTiles *g);
+// false (and pl untouched) when the sizes do not allow patches: n not a
+// multiple of pr and pc, n < W + max(pr, pc), offsets past 6 bits, M >= 2^20
+bool build_patch_lists(const NfftTables &t, int pr, int pc, PatchLists *pl);
 
 // smallest even 2^a 3^b 5^c >= v
 int nice_fft_size(int v);

@@ -13,14 +13,6 @@
 namespace nudim {
 
 constexpr int kThreads = 256;
-constexpr int kGatherTile = 8;  // cells per tile edge of the tiled gather
-// nodes per chunk of the tiled gather (LDS rows of its staged node values);
-// 32-node chunks in FP64 (four blocks per CU instead of two) measured slower
-// at 256^3 (3.71 against 3.56 ms per adjoint with its combine kernel)
-template <typename T>
-struct GatherChunk {
-  static constexpr int value = 64;
-};
 constexpr int kRedBlocks = 240;  // partial-sum rows of the per-plane reductions
 
 // ---------------------------------------------------------------------------
@@ -192,130 +184,6 @@ k_gather_w(const cx<T> *__restrict__ s, cx<T> *__restrict__ g, const int *__rest
       if (e + j < e1) acc = acc + (T)(w0[j] * w1[j]) * sv[j];
   }
   g[(size_t)cell * P + p] = acc;
-}
-
-__device__ __forceinline__ float lane_bcast(float v, int j) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
-}
-__device__ __forceinline__ double lane_bcast(double v, int j) {
-  long long b = __double_as_longlong(v);
-  int lo = __builtin_amdgcn_readlane((int)b, j), hi = __builtin_amdgcn_readlane((int)(b >> 32), j);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-
-// One entry of the tiled gather map: the node's slot in its chunk and the
-// window weight psi0[node][t0] psi1[node][t1].
-template <typename T>
-struct GEnt {
-  int s;
-  T w;
-};
-
-// Tiled gather, one block per node chunk (at most KN nodes) of a TS x TS
-// tile and 64 planes.  The block stages the chunk's
-// node values in LDS, [slot][64 planes] (KN + 1 rows: row KN is a spare), so
-// each node value is read once per tile it meets
-// instead of once per tap; lane = plane; wave w owns the tile's cells
-// w*CPW .. w*CPW+CPW-1 (row-major in the tile) and sums each cell's entries
-// of the chunk in node order.  The only chunk of a tile writes the grid; the
-// chunks of a tile with several write partial sums part[slot][cell][p] for
-// k_gather_combine.
-template <typename T, int TS, int KN>
-__global__ void __launch_bounds__(kThreads)
-k_gather_chunk(const cx<T> *__restrict__ s, cx<T> *__restrict__ g, cx<T> *__restrict__ part,
-               const int *__restrict__ chunk_tile, const int *__restrict__ chunk_part,
-               const int *__restrict__ chunk_node, const int *__restrict__ node, const int *__restrict__ chunk_cell,
-               const GEnt<T> *__restrict__ ent, int n, int P) {
-  static_assert(kThreads == 256, "four wavefronts per block");
-  constexpr int TT = TS * TS, CPW = TT / 4;
-  extern __shared__ unsigned char smem_raw[];
-  cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
-  const int c = (int)xcd_block(blockIdx.x, gridDim.x);
-  const int lane = threadIdx.x & 63;
-  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  const int pbase = blockIdx.y * 64;
-  const int a = chunk_node[c], nn = chunk_node[c + 1] - a;  // 1 <= nn <= KN
-  {
-    // wave w stages slots w, w+4, ..., w+KN-4: KN/4 unconditional loads in
-    // flight (slot clamped into the chunk) and KN/4 unconditional stores
-    // (slots past the chunk go to the spare row KN), so that no load is
-    // sunk into a branch of its own
-    const cx<T> *sp = s + pbase + lane;
-    const int myn = node[a + min(lane, nn - 1)];
-    cx<T> v[KN / 4];
-#pragma unroll
-    for (int j = 0; j < KN / 4; j++)
-      v[j] = sp[(size_t)__builtin_amdgcn_readlane(myn, min(wv + 4 * j, nn - 1)) * P];
-#pragma unroll
-    for (int j = 0; j < KN / 4; j++) lds[(wv + 4 * j < nn ? wv + 4 * j : KN) * 64 + lane] = v[j];
-  }
-  __syncthreads();
-  const int tile = chunk_tile[c], ps = chunk_part[c];
-  const int nt = n / TS, ty = tile / nt, tx = tile - ty * nt;
-  // the wave's cells hold entries [E0, E1) of the chunk, cell k from cc[k]:
-  // read 64 entries at a time lane-parallel, broadcast each with readlane
-  const int *cc = chunk_cell + (size_t)c * (TT + 1) + wv * CPW;
-  int cb[CPW + 1];
-#pragma unroll
-  for (int k = 0; k <= CPW; k++) cb[k] = __builtin_amdgcn_readfirstlane(cc[k]);
-  cx<T> acc[CPW];
-#pragma unroll
-  for (int k = 0; k < CPW; k++) acc[k] = mk((T)0, (T)0);
-  for (int base = cb[0]; base < cb[CPW]; base += 64) {
-    const GEnt<T> q = ent[min(base + lane, cb[CPW] - 1)];
-#pragma unroll
-    for (int k = 0; k < CPW; k++) {
-      const int ka = max(cb[k], base), kb = min(cb[k + 1], base + 64);
-      // four entries at a time: their LDS reads in flight together (the
-      // slot of an entry past kb is clamped to the last one and not added;
-      // stepping all the wave's cells together instead, entry i of each
-      // per step, measured 2.5x slower: the steps pad every cell to the
-      // longest one)
-      for (int e = ka; e < kb; e += 4) {
-        cx<T> v[4];
-        T w[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-          const int j = min(e + u, kb - 1) - base;
-          w[u] = lane_bcast(q.w, j);
-          v[u] = lds[__builtin_amdgcn_readlane(q.s, j) * 64 + lane];
-        }
-#pragma unroll
-        for (int u = 0; u < 4; u++)
-          if (e + u < kb) acc[k] = acc[k] + w[u] * v[u];
-      }
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < CPW; k++) {
-    const int cl = wv * CPW + k;
-    if (ps < 0) {
-      const size_t cell = (size_t)(ty * TS + cl / TS) * n + (size_t)(tx * TS + cl % TS);
-      g[cell * P + pbase + lane] = acc[k];
-    } else {
-      part[((size_t)ps * TT + cl) * P + pbase + lane] = acc[k];
-    }
-  }
-}
-
-// Tiles with several node chunks (or none): grid cell = the chunks' partial
-// sums added in chunk order (zero for a tile no node meets).  comb holds
-// (tile, first slot, count) triples; one block per tile and 64 planes.
-template <typename T, int TS>
-__global__ void __launch_bounds__(kThreads)
-k_gather_combine(const cx<T> *__restrict__ part, cx<T> *__restrict__ g, const int *__restrict__ comb, int n, int P) {
-  constexpr int TT = TS * TS;
-  const int b = (int)xcd_block(blockIdx.x, gridDim.x);
-  const int tile = comb[3 * b], p0 = comb[3 * b + 1], cnt = comb[3 * b + 2];
-  const int nt = n / TS, ty = tile / nt, tx = tile - ty * nt;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int p = blockIdx.y * 64 + lane;
-  for (int cl = wv; cl < TT; cl += 4) {
-    cx<T> acc = mk((T)0, (T)0);
-    for (int i = 0; i < cnt; i++) acc = acc + part[((size_t)(p0 + i) * TT + cl) * P + p];
-    const size_t cell = (size_t)(ty * TS + cl / TS) * n + (size_t)(tx * TS + cl % TS);
-    g[cell * P + p] = acc;
-  }
 }
 
 // g[cell][p] = sum over the cell's taps (node, t0, t1) of

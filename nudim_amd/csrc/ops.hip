@@ -1,6 +1,7 @@
 // ops.hip -- batched NFFT, Toeplitz normal operator, per-plane reductions
 // and the CGNR recurrence of the NFFT3 solver, for T = float and double.
 #include "engine.hpp"
+#include "spread.hpp"
 
 #include <algorithm>
 #include <cstring>
@@ -12,6 +13,9 @@ namespace nudim {
 
 thread_local KernelProfiler *g_prof = nullptr;
 bool g_use_fft2 = getenv("NUDIM_GENERIC_FFT") == nullptr;
+// patch-form spreading kernels (spread.hpp) where the sizes allow them;
+// NUDIM_SPREAD=cell selects the per-node interpolation and per-cell gather
+static const bool g_spread_patch = !(getenv("NUDIM_SPREAD") && std::string(getenv("NUDIM_SPREAD")) == "cell");
 
 // ---------------------------------------------------------------------------
 // small non-template kernels
@@ -189,8 +193,7 @@ void LineFft<T>::run(const cx<T> *in, cx<T> *out, int nlines, const LineIO &li, 
 template <typename T>
 size_t NfftGeom<T>::bytes() const {
   return fft.tw.bytes + c0.bytes + c1.bytes + psi0.bytes + psi1.bytes + dec.bytes + cell_ptr.bytes + ent.bytes + perm.bytes +
-         chunk_tile.bytes + chunk_part.bytes + chunk_node.bytes + tnode.bytes + chunk_cell.bytes + tent.bytes +
-         comb.bytes;
+         porder.bytes + pptr.bytes + grec.bytes + border.bytes + bptr.bytes + irec.bytes + nptr.bytes + nent.bytes;
 }
 
 template <typename T>
@@ -235,23 +238,53 @@ static std::shared_ptr<NfftGeom<T>> build_nfft_geometry(int N, int n, int W, con
     std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return key[a] < key[b]; });
     upload(g->perm, perm);
   }
-  g->has_tiles = n % kGatherTile == 0;
-  if (g->has_tiles) {
-    // node values of a chunk staged as [slot][64 planes]
-    GatherTiles gt;
-    build_gather_tiles(t, kGatherTile, GatherChunk<T>::value, &gt);
-    g->nchunk = (int)gt.chunk_tile.size();
-    g->ncomb = (int)gt.comb.size() / 3;
-    g->npart = gt.npart;
-    upload(g->chunk_tile, gt.chunk_tile);
-    upload(g->chunk_part, gt.chunk_part);
-    upload(g->chunk_node, gt.chunk_node);
-    upload(g->tnode, gt.node);
-    upload(g->chunk_cell, gt.chunk_cell);
-    upload(g->comb, gt.comb);
-    std::vector<GEnt<T>> ev(gt.slot.size());
-    for (size_t e = 0; e < ev.size(); e++) ev[e] = GEnt<T>{(int)gt.slot[e], (T)gt.weight[e]};
-    upload(g->tent, ev);
+  {
+    // gather: 4 x 8-cell patches, one wavefront each
+    PatchLists pl;
+    g->has_patches = g_spread_patch && build_patch_lists(t, kPatchR, kPatchC, &pl);
+    if (g->has_patches) {
+      g->npatch = (int)pl.order.size();
+      upload(g->porder, pl.order);
+      upload(g->pptr, pl.ptr);
+      // one spare record at the end (RecChunk reads pieces past the last)
+      std::vector<PatchRec<T>> gr(pl.ent.size() + 1);
+      for (size_t e = 0; e < pl.ent.size(); e++) {
+        const int m = (int)(pl.ent[e] & 0xFFFFFu), k0 = (int)((pl.ent[e] >> 20) & 63u), k1 = (int)(pl.ent[e] >> 26);
+        const double *a0 = pl.psi0p.data() + (size_t)m * pl.l0 + k0, *a1 = pl.psi1p.data() + (size_t)m * pl.l1 + k1;
+        gr[e].node = m;
+        for (int i = 0; i < kPatchR; i++) gr[e].a0[i] = (T)a0[i];
+        for (int j = 0; j < kPatchC; j++) gr[e].w1[j] = (T)a1[j];
+      }
+      upload(g->grec, gr);
+    }
+    // interpolation: 8 x 16-cell block patches, one block each; quarter q of
+    // an entry is the 4 x 8 part at rows 4 (q / 2) .., columns 8 (q % 2) ..
+    PatchLists bl;
+    g->has_blocks = g->has_patches && build_patch_lists(t, 2 * kPatchR, 2 * kPatchC, &bl);
+    if (g->has_blocks) {
+      g->nbitem = bl.nonempty;
+      g->nbentry = (int)bl.ent.size();
+      upload(g->border, bl.order);
+      upload(g->bptr, bl.ptr);
+      std::vector<PatchRec<T>> ir((bl.ent.size() + 1) * 4);
+      for (size_t e = 0; e < bl.ent.size(); e++) {
+        const int m = (int)(bl.ent[e] & 0xFFFFFu), k0 = (int)((bl.ent[e] >> 20) & 63u), k1 = (int)(bl.ent[e] >> 26);
+        for (int q = 0; q < 4; q++) {
+          // window taps of the quarter's first row and column
+          const int t0 = k0 + kPatchR * (q >> 1) - (2 * kPatchR - 1), t1 = k1 + kPatchC * (q & 1) - (2 * kPatchC - 1);
+          const bool meets = t0 + kPatchR > 0 && t0 < W && t1 + kPatchC > 0 && t1 < W;
+          PatchRec<T> &r = ir[e * 4 + q];
+          r.node = meets ? m : -1;
+          const double *a0 = bl.psi0p.data() + (size_t)m * bl.l0 + k0 + kPatchR * (q >> 1);
+          const double *a1 = bl.psi1p.data() + (size_t)m * bl.l1 + k1 + kPatchC * (q & 1);
+          for (int i = 0; i < kPatchR; i++) r.a0[i] = (T)a0[i];
+          for (int j = 0; j < kPatchC; j++) r.w1[j] = (T)a1[j];
+        }
+      }
+      upload(g->irec, ir);
+      upload(g->nptr, bl.node_ptr);
+      upload(g->nent, bl.node_ent);
+    }
   }
   return g;
 }
@@ -314,8 +347,9 @@ template <typename T>
 void NfftOp<T>::init(int N_, int n_, int W_, const double *x, int M_, int P_) {
   N = N_; n = n_; W = W_; M = M_; P = P_;
   g = nfft_geometry<T>(N, n, W, x, M);
-  tiled = g->has_tiles && (P % 64 == 0);
-  if (tiled) part.alloc((size_t)std::max(g->npart, 1) * kGatherTile * kGatherTile * P * sizeof(cx<T>));
+  patched = g->has_patches && (P % 64 == 0);
+  blocked = g->has_blocks && (P % 64 == 0);
+  if (blocked) part.alloc((size_t)std::max(g->nbentry, 1) * P * sizeof(cx<T>));
   X.alloc((size_t)N * n * P * sizeof(cx<T>));
   grid.alloc((size_t)n * n * P * sizeof(cx<T>));
 }
@@ -382,6 +416,19 @@ void NfftOp<T>::trafo(const cx<T> *fhat, cx<T> *f, hipStream_t st, const int *se
   grid_fft(fhat, st);
   int PCs = plane_group_log2(P);
   const int Mc = sel ? msel : M;
+  if (blocked && !sel) {
+    // algorithmic bytes: grid read once, node values written, node coordinates
+    ProfScope ps(KC_INTERP, st, ((double)n * n * sizeof(cx<T>) + (double)M * sizeof(cx<T>) + 2.0 * M * sizeof(T)) * P);
+    const int G = P / 64;
+    if (g->nbitem > 0)
+      hipLaunchKernelGGL(k_interp_block<T>, dim3((unsigned)((size_t)(g->nbitem + 7) / 8 * 8 * G)), dim3(kThreads), 0, st,
+                         grid.as<cx<T>>(), part.as<cx<T>>(), g->border.template as<int>(), g->bptr.template as<int>(),
+                         g->irec.template as<PatchRec<T>>(), n, P, g->nbitem);
+    hipLaunchKernelGGL(k_interp_sum<T>, dim3(cdiv((size_t)M * G, 4)), dim3(kThreads), 0, st, part.as<cx<T>>(), f,
+                       g->nptr.template as<int>(), g->nent.template as<int>(), M, P);
+    NUDIM_HIP(hipGetLastError());
+    return;
+  }
   // every node: in the spatial order of the geometry (same values as node order)
   if (!sel && !(getenv("NUDIM_INTERP_ORDER") && std::string(getenv("NUDIM_INTERP_ORDER")) == "node"))
     sel = g->perm.template as<int>();
@@ -416,40 +463,35 @@ void NfftOp<T>::adjoint(const cx<T> *f, cx<T> *fhat, hipStream_t st) {
   int PCs = plane_group_log2(P, g_gather_lanes);
   int ncell = n * n;
   dim3 gr(cdiv(ncell, kThreads >> PCs), P >> PCs);
-  if (tiled) {
-    ProfScope ps(KC_GATHER, st, ((double)M * sizeof(cx<T>) + 4.0 * M + (double)ncell * sizeof(cx<T>)) * P);
-    cx<T> *gp = grid.as<cx<T>>(), *pp = part.as<cx<T>>();
-    if (g->nchunk > 0)
-      hipLaunchKernelGGL((k_gather_chunk<T, kGatherTile, GatherChunk<T>::value>), dim3(g->nchunk, P / 64),
-                         dim3(kThreads), (GatherChunk<T>::value + 1) * 64 * sizeof(cx<T>), st, f, gp, pp, g->chunk_tile.template as<int>(), g->chunk_part.template as<int>(),
-                         g->chunk_node.template as<int>(), g->tnode.template as<int>(), g->chunk_cell.template as<int>(),
-                         g->tent.template as<GEnt<T>>(), n, P);
-    if (g->ncomb > 0)
-      hipLaunchKernelGGL((k_gather_combine<T, kGatherTile>), dim3(g->ncomb, P / 64), dim3(kThreads), 0, st, pp, gp,
-                         g->comb.template as<int>(), n, P);
-  } else {
+  {
   // algorithmic bytes: node values read, node ids of the inverse map, grid written
   ProfScope ps(KC_GATHER, st, ((double)M * sizeof(cx<T>) + 4.0 * M + (double)ncell * sizeof(cx<T>)) * P);
-  cx<T> *gp = grid.as<cx<T>>();
-  const int *cp = g->cell_ptr.template as<int>();
-  const uint32_t *ep = g->ent.template as<uint32_t>();
-  const double *p0 = g->psi0.template as<double>(), *p1 = g->psi1.template as<double>();
+  if (patched) {
+    hipLaunchKernelGGL(k_gather_patch<T>, dim3(cdiv((size_t)g->npatch * (P / 64), 4)), dim3(kThreads), 0, st, f,
+                       grid.as<cx<T>>(), g->porder.template as<int>(), g->pptr.template as<int>(),
+                       g->grec.template as<PatchRec<T>>(), n, P, g->npatch);
+  } else {
+    cx<T> *gp = grid.as<cx<T>>();
+    const int *cp = g->cell_ptr.template as<int>();
+    const uint32_t *ep = g->ent.template as<uint32_t>();
+    const double *p0 = g->psi0.template as<double>(), *p1 = g->psi1.template as<double>();
 #define NUDIM_GATHER_W(WW)                                                                                 \
   case WW:                                                                                                 \
     hipLaunchKernelGGL((k_gather_w<T, WW>), gr, dim3(kThreads), 0, st, f, gp, cp, ep, p0, p1, ncell, P, PCs); \
     break;
-  switch (W) {
-    NUDIM_GATHER_W(4)
-    NUDIM_GATHER_W(6)
-    NUDIM_GATHER_W(8)
-    NUDIM_GATHER_W(10)
-    NUDIM_GATHER_W(12)
-    NUDIM_GATHER_W(14)
-    NUDIM_GATHER_W(16)
-    default:
-      hipLaunchKernelGGL(k_gather<T>, gr, dim3(kThreads), 0, st, f, gp, cp, ep, p0, p1, ncell, W, P, PCs);
-  }
+    switch (W) {
+      NUDIM_GATHER_W(4)
+      NUDIM_GATHER_W(6)
+      NUDIM_GATHER_W(8)
+      NUDIM_GATHER_W(10)
+      NUDIM_GATHER_W(12)
+      NUDIM_GATHER_W(14)
+      NUDIM_GATHER_W(16)
+      default:
+        hipLaunchKernelGGL(k_gather<T>, gr, dim3(kThreads), 0, st, f, gp, cp, ep, p0, p1, ncell, W, P, PCs);
+    }
 #undef NUDIM_GATHER_W
+  }
   }
   NUDIM_HIP(hipGetLastError());
   const double *d = g->dec.template as<double>();

@@ -60,26 +60,34 @@ def test_nfft_fp32(hip, orc, polar32, n, W, tol):
         assert rel(a[p], orc.ndft_adjoint(polar32, g[p], N)) < tol
 
 
-@pytest.mark.parametrize("fp64,n,W,tol", [(True, 128, 14, 1e-12), (False, 128, 8, 2e-6), (False, 96, 6, 1e-4)])
-def test_nfft_adjoint_tiled(hip, orc, fp64, n, W, tol):
-    """64 planes on a grid whose edge is a multiple of 8 take the LDS-tiled
-    gather (the dense centre tiles need several node chunks, summed as
-    per-chunk partial sums added in chunk order); 63 planes take the
-    per-cell gather (one running sum per cell).  The shared planes agree to
-    rounding, and both match the NDFT."""
+@pytest.mark.parametrize("fp64,n,W,tol", [(True, 128, 14, 1e-12), (False, 128, 8, 2e-6), (False, 96, 6, 1e-4),
+                                          (True, 120, 14, 1e-12)])
+def test_nfft_patch_forms(hip, orc, fp64, n, W, tol):
+    """64 planes on a grid whose edge is a multiple of 16 take the patch-form
+    kernels (4 x 8-cell patches in registers: the gather writes each cell
+    once, the interpolation adds per-block-patch partial sums per node); 63
+    planes take the per-cell gather and the per-node interpolation.  The two
+    forms agree to rounding and both match the NDFT (the dense centre
+    patches hold hundreds of nodes, several record chunks; n = 120 takes the
+    patch gather and the per-node interpolation)."""
     N = 64
     pmin, pmax = orc.pad_range(-60, 3)
     x = orc.polar_nodes(pmin, pmax, 1.0, N)
     M = x.shape[0]
     rng = np.random.default_rng(7)
     g = rng.standard_normal((64, M)) + 1j * rng.standard_normal((64, M))
+    fh = rng.standard_normal((64, N, N)) + 1j * rng.standard_normal((64, N, N))
     prec = hip.FP64_PLAIN if fp64 else hip.FP32
     a64 = hip.nfft_adjoint(x, g, N, n=n, W=W, precision=prec)
     a63 = hip.nfft_adjoint(x, g[:63], N, n=n, W=W, precision=prec)
     assert rel(a64[:63], a63) < (1e-14 if fp64 else 1e-6)
+    f64 = hip.nfft_trafo(x, fh, n=n, W=W, precision=prec)
+    f63 = hip.nfft_trafo(x, fh[:63], n=n, W=W, precision=prec)
+    assert rel(f64[:63], f63) < (1e-14 if fp64 else 1e-6)
     # tol: FP64 rounding, FP32 rounding, window error of sigma 1.5 and W 6 (5.3e-5 in FP64)
     for p in (0, 37, 63):
         assert rel(a64[p], orc.ndft_adjoint(x, g[p], N)) < tol
+        assert rel(f64[p], orc.ndft_trafo(x, fh[p])) < tol
 
 
 @pytest.mark.parametrize("N", [64, 192])
@@ -483,7 +491,8 @@ def test_end_to_end_z_lengths(hip, orc, np_, nz):
 def test_end_to_end_er_fp32(hip, orc, denoise, method, np_):
     """ER passes only (continuous): FP32 form within 1e-4 relative L2.  At
     np = nz = 64 the Toeplitz CGNR runs on the fused fixed-length passes
-    (2N = 128) and the adjoint on the tiled gather (64 planes).
+    (2N = 128) and the NFFT of the initial volume on the patch-form kernels
+    (64 planes).
 
     With DIFFUSION at 64^3 the denoiser amplifies the ~2e-6 per-pass FP32
     rounding with a heavy tail: the oracle fed relative perturbations of 2e-6
@@ -596,10 +605,10 @@ def test_z_transform_lengths(hip, orc, np_, nz):
 
 @pytest.mark.parametrize("prec", [0, 1])
 def test_interp_spatial_order_same_values(hip, orc, monkeypatch, prec):
-    """The interpolation visits the nodes sorted by 16 x 16-cell tile (for L2
-    reuse of the grid rows); every node's sum is the same as in node order,
-    bit for bit."""
-    N, P = 64, 64
+    """The per-node interpolation (plane counts that are not multiples of 64)
+    visits the nodes sorted by 8 x 8-cell tile (for L2 reuse of the grid
+    rows); every node's sum is the same as in node order, bit for bit."""
+    N, P = 64, 48
     pmin, pmax = orc.pad_range(-60, 3)
     x = orc.polar_nodes(pmin, pmax, 1.0, N)
     rng = np.random.default_rng(31)

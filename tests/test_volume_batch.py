@@ -91,7 +91,7 @@ def test_batch_graph_replay_equals_direct(hip):
 
 def test_batch_other_plane_grouping_agrees_to_rounding(hip):
     """nvol = 2 at nz = 32: the per-plane reductions group 64 planes, not 32,
-    and the interpolation takes its tiled form, so sums are added in another
+    and the interpolation takes its patch form, so sums are added in another
     order; on ER passes without the denoiser (a contracting map: with HIO
     and TV such differences grow ~3e3 times a pass, as any rounding
     difference does there) FP64 volumes and R agree with the single jobs to
