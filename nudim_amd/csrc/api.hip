@@ -998,7 +998,7 @@ extern "C" int nudim_estimate_bytes(const nudim_params *p, size_t *bytes) {
       b += entries(2 * pr, 2 * pc) * (P * cb + 4 + 4 * (cb == 16 ? sizeof(PatchRec<double>) : sizeof(PatchRec<float>))) +
            (size_t)n * n / (4 * pr * pc) * 8;
   }
-  if (form == NUDIM_CG_TOEPLITZ) b += (size_t)4 * N * N * (cb + rb) + (size_t)N * 2 * N * P * cb + NN * cb;
+  if (form == NUDIM_CG_TOEPLITZ) b += (size_t)4 * N * N * (cb + rb) + (size_t)N * 2 * N * P * cb + NN * cb + 2 * N * P * 8;
   if (projected_start(form, (int)N))  // T_U, X2, b_K, R partials, known list; c_K and the quadratic form's partials
     b += (size_t)4 * N * N * rb + (size_t)N * 2 * N * P * cb + NN * cb + (size_t)2 * 1024 * P * 8 + M * 4 +
          NN * cb + (size_t)2 * N * P * 8;

@@ -200,6 +200,7 @@ struct Cgnr {
   DevBuf z, pv, q, r, v, t;   // z, p, q: [N^2][P]; r, v, t: [M][P]
   DevBuf dot_z, coef, sc;     // [P] each
   DevBuf zz2, alpha, rowpart; // fused Toeplitz steps: [P], [P], [N][P]
+  DevBuf colpart;             // column sums of <p, T p>: [2N][P]
   // kept directions: p_k of every step [kept][N^2][P] and alpha_k [kept][P]
   DevBuf pk, alk;
   int kept = 0;

@@ -218,6 +218,12 @@ struct ToepCg {
   const T *mid3;
   double *qpart;
   const cx<T> *pin;  // p_{k-1} when the solve keeps each step's p apart (p_k written to p)
+  // CG step with the update in the row inverse pass: the column pass leaves
+  // the row sums of <p, T p> (sum over the column of mid |P^|^2) in qpart,
+  // and the row inverse pass takes z -= alz[plane] q into zu in place of
+  // writing q, with the row sums of |z|^2 in part
+  cx<T> *zu;
+  const double *alz;
 };
 
 // Batched line FFT over a plane-innermost array.  Block b (renumbered so

@@ -631,8 +631,21 @@ k_toep(const cx<T> *in, cx<T> *out, long long in_so, long long in_sk, long long 
 #pragma unroll
         for (int k2 = 0; k2 < H2; k2++) b[k2] = av[k2] + b[k2];
       }
-      // row sum: Re <p, q> with p, |q|^2 without (when part is given)
-      if (cg.p) {
+      if (cg.zu) {
+        // z -= alpha q, row sum of |z|^2
+        cx<T> *zp = cg.zu + ob;
+        const T na = (T)(-cg.alz[planeB]);
+        cx<T> zv[H2];
+#pragma unroll
+        for (int k2 = 0; k2 < H2; k2++) zv[k2] = zp[(long long)(kb + N1 * k2) * out_sk];
+#pragma unroll
+        for (int k2 = 0; k2 < H2; k2++) {
+          b[k2] = zv[k2] + na * b[k2];
+          s += (double)b[k2].x * b[k2].x + (double)b[k2].y * b[k2].y;
+        }
+        dst = zp;
+      } else if (cg.p) {
+        // row sum: Re <p, q> with p, |q|^2 without (when part is given)
         const cx<T> *pp = cg.p + ob;
         cx<T> pv[H2];
 #pragma unroll
@@ -662,7 +675,7 @@ k_toep(const cx<T> *in, cx<T> *out, long long in_so, long long in_sk, long long 
     // forward N2-point DFT, real product, inverse N2-point DFT, twiddle, exchange
     const T *mrow = mid + (long long)o * n;
     T mv[N2];
-    double qf = 0;  // TK_COL2 with mid3: this thread's part of sum mid3 |FFT(in)|^2
+    double qf = 0;  // this thread's part of sum mid3 |FFT(in)|^2 (TK_COL2) or mid |FFT(in)|^2 (TK_COL)
     if (actB) {
       dft_reg<T, N2, -1>(b);
       if constexpr (KIND == TK_COL2) {
@@ -676,6 +689,13 @@ k_toep(const cx<T> *in, cx<T> *out, long long in_so, long long in_sk, long long 
       }
 #pragma unroll
       for (int k2 = 0; k2 < N2; k2++) mv[k2] = mrow[kb + N1 * k2];
+      if constexpr (KIND == TK_COL) {
+        // <p, T p> = sum mid |P^|^2 over the spectrum (Parseval on the embedding)
+        if (cg.qpart) {
+#pragma unroll
+          for (int k2 = 0; k2 < N2; k2++) qf += (double)mv[k2] * ((double)b[k2].x * b[k2].x + (double)b[k2].y * b[k2].y);
+        }
+      }
 #pragma unroll
       for (int k2 = 0; k2 < N2; k2++) b[k2] = mv[k2] * b[k2];
     }
@@ -745,7 +765,7 @@ k_toep(const cx<T> *in, cx<T> *out, long long in_so, long long in_sk, long long 
         for (int i = 0; i < Q4; i++) dst[(long long)(r + N2 * (Q4 * h + i)) * out_sk] = a[i];
       }
     }
-    if constexpr (KIND == TK_COL2) {
+    if constexpr (KIND == TK_COL2 || KIND == TK_COL) {
       if (cg.qpart) {
         // fixed-order sum over the N1 step-B threads of each plane
         __syncthreads();  // exchange buffer free

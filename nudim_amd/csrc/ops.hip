@@ -15,6 +15,9 @@ thread_local KernelProfiler *g_prof = nullptr;
 bool g_use_fft2 = getenv("NUDIM_GENERIC_FFT") == nullptr;
 // patch-form spreading kernels (spread.hpp) where the sizes allow them;
 // NUDIM_SPREAD=cell selects the per-node interpolation and per-cell gather
+// the CG update z -= alpha q inside the Toeplitz row inverse pass, alpha
+// from the column pass (NUDIM_CG_UPDATE=kernel: a kernel of its own after q)
+static const bool g_cg_in_rowi = !(getenv("NUDIM_CG_UPDATE") && std::string(getenv("NUDIM_CG_UPDATE")) == "kernel");
 static const bool g_spread_patch = !(getenv("NUDIM_SPREAD") && std::string(getenv("NUDIM_SPREAD")) == "cell");
 
 // ---------------------------------------------------------------------------
@@ -47,6 +50,34 @@ __global__ void k_sum_partials(const double *__restrict__ part, int nb, int P, d
     __syncthreads();
   }
   if (lane == 0 && p < P) out[p] = sh[pl];
+}
+
+// alpha[p] = zz[p] / (sum of the nb rows of part[row][stride] at plane p),
+// p < np; the rows added as k_sum_partials adds them
+__global__ void k_cg_alpha(const double *__restrict__ part, int nb, int stride, int np, const double *__restrict__ zz,
+                           double *__restrict__ alpha) {
+  __shared__ double sh[256];
+  const int pl = threadIdx.x & 7, lane = threadIdx.x >> 3;
+  const int p = blockIdx.x * 8 + pl;
+  double s = 0;
+  if (p < np)
+    for (int b0 = lane; b0 < nb; b0 += 32 * 4) {
+      double v[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int b = b0 + 32 * k;
+        v[k] = b < nb ? part[(size_t)b * stride + p] : 0.0;
+      }
+#pragma unroll
+      for (int k = 0; k < 4; k++) s += v[k];
+    }
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 16; h > 0; h >>= 1) {
+    if (lane < h) sh[threadIdx.x] += sh[threadIdx.x + h * 8];
+    __syncthreads();
+  }
+  if (lane == 0 && p < np) alpha[p] = zz[p] / sh[pl];
 }
 
 __global__ void k_cg_scalars(double *__restrict__ dot_z, double *__restrict__ coef, const double *__restrict__ s,
@@ -734,6 +765,29 @@ void ToeplitzOp<T>::apply_cg(const ToepCg<T> &cg, cx<T> *q, hipStream_t st) {
       ProfScope ps(KC_TOEPLITZ_ROWF, st, (nvec * N + L) * N * pc * sizeof(cx<T>));
       toep_launch<T>(TK_ROWF, L, N, nullptr, x, NP, P, LC, pc, pc, st, tw, nullptr, c, P, pairs);
     }
+    if (c.zu) {
+      // <p, T p> from the column pass's spectrum, alpha, then the row
+      // inverse pass updates z in place of writing q
+      c.zu += p0;
+      c.qpart += p0;
+      {
+        ProfScope ps(KC_TOEPLITZ_COL, st, 2.0 * N * L * pc * sizeof(cx<T>) + (double)L * L * sizeof(T));
+        toep_launch<T>(TK_COL, L, L, x, x, pc, LC, pc, LC, pc, st, tw, kr, c, P, pairs);
+      }
+      hipLaunchKernelGGL(k_cg_alpha, dim3(cdiv(pc, 8)), dim3(256), 0, st, c.qpart, L, P, pc, c.zz,
+                         const_cast<double *>(c.alz) + p0);
+      c.alz += p0;
+      {
+        // algorithmic bytes: X read (N x L), z read and written (N x N each)
+        ProfScope ps(KC_TOEPLITZ_ROWI, st, (2.0 * N + L) * N * pc * sizeof(cx<T>));
+        ToepCg<T> ci{};
+        ci.zu = c.zu;
+        ci.alz = c.alz;
+        ci.part = c.part;
+        toep_launch<T>(TK_ROWI, L, N, x, c.zu, LC, pc, NP, P, pc, st, tw, nullptr, ci, P, pairs);
+      }
+      continue;
+    }
     {
       ProfScope ps(KC_TOEPLITZ_COL, st, 2.0 * N * L * pc * sizeof(cx<T>) + (double)L * L * sizeof(T));
       toep_launch<T>(TK_COL, L, L, x, x, pc, LC, pc, LC, pc, st, tw, kr, c, P, pairs);
@@ -796,7 +850,8 @@ void Reducer::rfactor(const cx<T> *f, const cx<T> *meas, const int *kind, long l
 template <typename T>
 size_t Cgnr<T>::bytes() const {
   return z.bytes + pv.bytes + q.bytes + r.bytes + v.bytes + t.bytes + dot_z.bytes + coef.bytes + sc.bytes +
-         zz2.bytes + alpha.bytes + rowpart.bytes + pk.bytes + alk.bytes + red.part.bytes + red.part2.bytes;
+         zz2.bytes + alpha.bytes + rowpart.bytes + colpart.bytes + pk.bytes + alk.bytes + red.part.bytes +
+         red.part2.bytes;
 }
 
 template <typename T>
@@ -819,6 +874,7 @@ void Cgnr<T>::init(int N_, int M_, int P_, int form_, NfftOp<T> *op_, ToeplitzOp
       zz2.alloc(P * sizeof(double));
       alpha.alloc(P * sizeof(double));
       rowpart.alloc((size_t)N * P * sizeof(double));
+      colpart.alloc((size_t)2 * N * P * sizeof(double));
     }
   }
   r.alloc(MM);
@@ -863,6 +919,18 @@ void Cgnr<T>::solve_from_residual(cx<T> *fhat, int nsteps, hipStream_t st, bool 
       cg.f = nullptr;
       cg.alpha = nullptr;
       al = alk.as<double>() + (long long)it * P;
+    }
+    if (g_cg_in_rowi) {
+      // alpha from the column pass's <p, T p>; the row inverse pass takes
+      // z -= alpha q and leaves the row sums of |z|^2 (q is never stored)
+      cg.zu = zp;
+      cg.alz = al;
+      cg.qpart = colpart.as<double>();
+      toep->apply_cg(cg, qp, st);
+      hipLaunchKernelGGL(k_sum_partials, dim3(cdiv(P, 8)), dim3(256), 0, st, rowpart.as<double>(), N, P, zz_old);
+      NUDIM_HIP(hipGetLastError());
+      std::swap(zz, zz_old);
+      continue;
     }
     toep->apply_cg(cg, qp, st);
     hipLaunchKernelGGL(k_sum_partials, dim3(cdiv(P, 8)), dim3(256), 0, st, rowpart.as<double>(), N, P, pq);

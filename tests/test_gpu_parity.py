@@ -219,30 +219,38 @@ def test_kept_directions_job(hip, monkeypatch, nz):
     (f + sum_k alpha_k p_k applied once, in the z transform's loads; nz = 64
     on the register z transform, nz = 48 on the generic one with separate
     updates) equals the solve that updates f in every row pass to FP32
-    rounding, and the FP64 plain path to the FP32 bar, over HIO and ER
-    passes without the denoiser; the admission estimate counts the ten
+    rounding, and the FP64 plain path to the FP32 bar: over ER passes
+    without the denoiser in relative L2, over HIO and ER passes by the
+    flip-tolerant bar (99.9th percentile of the voxel differences; a voxel
+    near zero may take the other HIO branch under any rounding difference,
+    reconstruction.c:680-695); the admission estimate counts the ten
     buffers."""
     np_, nx = 128, 128
     rng = np.random.default_rng(11)
     proj = rng.standard_normal((41, nz, nx))
 
-    def run(prec):
-        p = hip.make_params(nx, nz, np_, -60, 60, 3, padangle=1.0, weight=1, nhio=2, ner=2, precision=prec)
+    def run(prec, nhio):
+        p = hip.make_params(nx, nz, np_, -60, 60, 3, padangle=1.0, weight=1, nhio=nhio, ner=4 - nhio,
+                            precision=prec)
         with hip.Reconstruction(p) as job:
             job.load_projections(proj)
             job.initial()
             r = job.iterate(4)
             return r, job.volume(), hip.estimate_bytes(p)
 
-    r_keep, v_keep, b_keep = run(hip.FP32)
-    monkeypatch.setenv("NUDIM_KEEP_P", "0")
-    r_one, v_one, b_one = run(hip.FP32)
-    monkeypatch.delenv("NUDIM_KEEP_P")
-    r64, v64, _ = run(hip.FP64_PLAIN)
-    assert b_keep - b_one >= 10 * np_ * np_ * nz * 8
-    assert rel(v_keep, v_one) < 1e-5
-    assert rel(v_keep, v64) < 1e-4
-    assert np.max(np.abs(r_keep - r64)) < 1e-5
+    for nhio in (0, 2):
+        r_keep, v_keep, b_keep = run(hip.FP32, nhio)
+        monkeypatch.setenv("NUDIM_KEEP_P", "0")
+        r_one, v_one, b_one = run(hip.FP32, nhio)
+        monkeypatch.delenv("NUDIM_KEEP_P")
+        r64, v64, _ = run(hip.FP64_PLAIN, nhio)
+        assert b_keep - b_one >= 10 * np_ * np_ * nz * 8
+        assert rel(v_keep, v_one) < 1e-5
+        if nhio == 0:
+            assert rel(v_keep, v64) < 1e-4
+        else:
+            assert np.quantile(np.abs(v_keep - v64), 0.999) < 1e-4 * np.abs(v64).max()
+        assert np.max(np.abs(r_keep - r64)) < 1e-5
 
 
 def test_toeplitz_plane_chunks(hip, orc, monkeypatch):
