@@ -55,9 +55,9 @@ def kc_names(np_, nz, n, W):
     zf = "k_zfft2<float,%d,%d> (z transforms)" % ZFFT2_SPLIT[nz] if nz in ZFFT2_SPLIT else "k_zfft_fwd/inv<float>"
     gather = ("k_gather_chunk<float,%d> (NFFT adjoint)" % W if nz % 64 == 0
               else "k_gather_w<float,%d> (NFFT adjoint)" % W)
-    return {1: toep(1, "Toeplitz column pass"), 2: "k_interp_rsums<float,%d> (R sums at the known nodes)" % W, 3: gather, 4: grid,
+    return {1: toep(1, "Toeplitz column pass, <p,Tp>"), 2: "k_interp_rsums<float,%d> (R sums at the known nodes)" % W, 3: gather, 4: grid,
             5: "k_tv_march<float>" if nz % 4 == 0 else "k_tv_fused<float>", 6: zf,
-            7: toep(0, "Toeplitz row forward + CG update"), 8: toep(2, "Toeplitz row inverse + <p,q>"),
+            7: toep(0, "Toeplitz row forward, p = z + beta p"), 8: toep(2, "Toeplitz row inverse, z -= alpha q"),
             9: toep(3, "two-input column pass, CG start"), 10: "k_cg_zupdate<float>"}
 
 

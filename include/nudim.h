@@ -140,12 +140,15 @@ int nudim_estimate_bytes(const nudim_params *p, size_t *bytes);
  * reference fixes n = 2 nextpow2(np), 14 taps, src/reconstruction.c:393-405.) */
 int nudim_autotune(const nudim_params *p, int *n, int *W);
 /* kernel timing with HIP events on the job's stream during nudim_iterate:
- * class 0 off, 1 Toeplitz column pass, 2 NFFT interpolation, 3 NFFT gather,
+ * class 0 off, 1 Toeplitz column pass (with the <p, T p> column sums),
+ * 2 NFFT interpolation (both kernels of the patch form), 3 NFFT gather,
  * 4 other line FFT passes, 5 TV sweeps, 6 z-axis transforms, 7 Toeplitz row
- * forward pass (with the CG vector update), 8 Toeplitz row inverse pass
- * (with the <p, q> row sums), 9 two-input column pass of the projected CG
- * start, 10 CG vector update z -= alpha q, 11 anisotropic diffusion (one
- * timed launch group per call: all 80 sweeps).  Selecting a
+ * forward pass (with the CG vector update p = z + beta p), 8 Toeplitz row
+ * inverse pass (with z -= alpha q and the |z|^2 row sums), 9 two-input
+ * column pass of the projected CG start, 10 CG vector update z -= alpha q
+ * as a kernel of its own (only with NUDIM_CG_UPDATE=kernel), 11
+ * anisotropic diffusion (one timed launch group per call: all 80 sweeps).
+ * Selecting a
  * class resets the counters.  (No reference counterpart: the reference only
  * prints total wall time, src/reconstruction-parallel.c:881-883.) */
 int nudim_profile(nudim_ctx *ctx, int kernel_class);

@@ -992,8 +992,10 @@ extern "C" int nudim_estimate_bytes(const nudim_params *p, size_t *bytes) {
     // sums, at most one entry per patch band pair a window meets
     auto entries = [&](int pr, int pc) { return M * (size_t)((W + pr - 2) / pr + 1) * ((W + pc - 2) / pc + 1); };
     const int pr = kPatchR, pc = kPatchC;
+    // (the gather's split patches: at most 2 E / 128 segment sums of pr pc cells)
     if (P % 64 == 0 && n % pc == 0 && n >= W + pc)
-      b += entries(pr, pc) * (cb == 16 ? sizeof(PatchRec<double>) : sizeof(PatchRec<float>)) + (size_t)n * n / (pr * pc) * 8;
+      b += entries(pr, pc) * (cb == 16 ? sizeof(PatchRec<double>) : sizeof(PatchRec<float>)) + (size_t)n * n / (pr * pc) * 16 +
+           2 * entries(pr, pc) / 128 * pr * pc * P * cb;
     if (P % 64 == 0 && n % (2 * pc) == 0 && n >= W + 2 * pc)
       b += entries(2 * pr, 2 * pc) * (P * cb + 4 + 4 * (cb == 16 ? sizeof(PatchRec<double>) : sizeof(PatchRec<float>))) +
            (size_t)n * n / (4 * pr * pc) * 8;

@@ -77,13 +77,14 @@ struct NfftGeom {
   // a block, and the blocks of an XCD, read the same grid cells from L2
   DevBuf perm;
   // patch lists of the patch-form kernels (spread.hpp; used when P % 64 == 0):
-  // the gather's 4 x 8-cell patches (order, heaviest first, entry ranges,
-  // PatchRec<T> records) and the interpolation's 8 x 16-cell block patches
-  // (order of the nonempty ones, heaviest first, entry ranges, four
-  // PatchRec<T> quarter records per entry, each node's entries)
+  // the gather's 4 x 8-cell patches (entry-list segments, longest first, with
+  // the split patches' (patch, first slot, count) triples; PatchRec<T>
+  // records) and the interpolation's 8 x 16-cell block patches (segments of
+  // the nonempty ones, four PatchRec<T> quarter records per entry, each
+  // node's entries)
   bool has_patches = false, has_blocks = false;
-  int npatch = 0, nbitem = 0, nbentry = 0;
-  DevBuf porder, pptr, grec, border, bptr, irec, nptr, nent;
+  int npatch = 0, ngseg = 0, ncomb = 0, nslot = 0, niseg = 0, nbentry = 0;
+  DevBuf gseg, gcomb, grec, iseg, irec, nptr, nent;
   size_t bytes() const;
 };
 template <typename T>
@@ -97,6 +98,7 @@ struct NfftOp {
   bool patched = false;                  // patch-form gather in use (P % 64 == 0)
   bool blocked = false;                  // block-patch interpolation in use (P % 64 == 0)
   DevBuf part;                           // the interpolation's partial sums [entry][P]
+  DevBuf gpart;                          // the gather's segment sums [slot][patch cells][P]
   DevBuf X, grid;                        // [N][n][P], [n][n][P]
   size_t bytes() const;
   void init(int N, int n, int W, const double *x, int M, int P);

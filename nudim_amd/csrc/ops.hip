@@ -224,7 +224,39 @@ void LineFft<T>::run(const cx<T> *in, cx<T> *out, int nlines, const LineIO &li, 
 template <typename T>
 size_t NfftGeom<T>::bytes() const {
   return fft.tw.bytes + c0.bytes + c1.bytes + psi0.bytes + psi1.bytes + dec.bytes + cell_ptr.bytes + ent.bytes + perm.bytes +
-         porder.bytes + pptr.bytes + grec.bytes + border.bytes + bptr.bytes + irec.bytes + nptr.bytes + nent.bytes;
+         gseg.bytes + gcomb.bytes + grec.bytes + iseg.bytes + irec.bytes + nptr.bytes + nent.bytes;
+}
+
+// Entry lists longer than smax are cut into nearly equal segments of at
+// most that many entries, each its own work item (a wavefront or block walks
+// one segment), so that no item walks the hundreds of entries of the
+// patches where all rays meet.  seg: the segments, longest first (ties in
+// patch order), empty patches last when kept; the segments of a split patch
+// get partial-sum slots, and comb their (patch, first slot, count).
+static void patch_segments(const PatchLists &pl, bool keep_empty, int smax, std::vector<PatchSeg> *seg,
+                           std::vector<int> *comb, int *nslot) {
+  seg->clear();
+  comb->clear();
+  int slot = 0;
+  const int npatch = (int)pl.ptr.size() - 1;
+  for (int q = 0; q < npatch; q++) {
+    const int e0 = pl.ptr[q], cnt = pl.ptr[q + 1] - e0;
+    if (cnt == 0) {
+      if (keep_empty) seg->push_back(PatchSeg{q, e0, e0, -1});
+      continue;
+    }
+    const int ns = (cnt + smax - 1) / smax;
+    if (ns == 1) {
+      seg->push_back(PatchSeg{q, e0, e0 + cnt, -1});
+      continue;
+    }
+    comb->insert(comb->end(), {q, slot, ns});
+    for (int k = 0; k < ns; k++)
+      seg->push_back(PatchSeg{q, e0 + (int)((long long)k * cnt / ns), e0 + (int)((long long)(k + 1) * cnt / ns), slot++});
+  }
+  std::stable_sort(seg->begin(), seg->end(),
+                   [](const PatchSeg &a, const PatchSeg &b) { return a.e1 - a.e0 > b.e1 - b.e0; });
+  *nslot = slot;
 }
 
 template <typename T>
@@ -275,8 +307,16 @@ static std::shared_ptr<NfftGeom<T>> build_nfft_geometry(int N, int n, int W, con
     g->has_patches = g_spread_patch && build_patch_lists(t, kPatchR, kPatchC, &pl);
     if (g->has_patches) {
       g->npatch = (int)pl.order.size();
-      upload(g->porder, pl.order);
-      upload(g->pptr, pl.ptr);
+      std::vector<PatchSeg> sg;
+      std::vector<int> comb;
+      // the gather's split patches cost a combine pass: segments of up to
+      // 128 entries, or of the entries per wavefront slot of the GPU (3072)
+      // when that is more, where the longest lists start first anyway
+      patch_segments(pl, true, std::max<int>(128, (int)(pl.ent.size() / 3072)), &sg, &comb, &g->nslot);
+      g->ngseg = (int)sg.size();
+      g->ncomb = (int)comb.size() / 3;
+      upload(g->gseg, sg);
+      if (!comb.empty()) upload(g->gcomb, comb);
       // one spare record at the end (RecChunk reads pieces past the last)
       std::vector<PatchRec<T>> gr(pl.ent.size() + 1);
       for (size_t e = 0; e < pl.ent.size(); e++) {
@@ -293,10 +333,14 @@ static std::shared_ptr<NfftGeom<T>> build_nfft_geometry(int N, int n, int W, con
     PatchLists bl;
     g->has_blocks = g->has_patches && build_patch_lists(t, 2 * kPatchR, 2 * kPatchC, &bl);
     if (g->has_blocks) {
-      g->nbitem = bl.nonempty;
       g->nbentry = (int)bl.ent.size();
-      upload(g->border, bl.order);
-      upload(g->bptr, bl.ptr);
+      std::vector<PatchSeg> sg;
+      std::vector<int> comb;
+      int nslot = 0;
+      patch_segments(bl, false, 128, &sg, &comb, &nslot);
+      for (auto &q : sg) q.slot = -1;
+      g->niseg = (int)sg.size();
+      if (!sg.empty()) upload(g->iseg, sg);
       std::vector<PatchRec<T>> ir((bl.ent.size() + 1) * 4);
       for (size_t e = 0; e < bl.ent.size(); e++) {
         const int m = (int)(bl.ent[e] & 0xFFFFFu), k0 = (int)((bl.ent[e] >> 20) & 63u), k1 = (int)(bl.ent[e] >> 26);
@@ -371,7 +415,7 @@ std::shared_ptr<const NfftGeom<T>> nfft_geometry(int N, int n, int W, const doub
 
 template <typename T>
 size_t NfftOp<T>::bytes() const {
-  return (g ? g->bytes() : 0) + part.bytes + X.bytes + grid.bytes;
+  return (g ? g->bytes() : 0) + part.bytes + gpart.bytes + X.bytes + grid.bytes;
 }
 
 template <typename T>
@@ -381,6 +425,7 @@ void NfftOp<T>::init(int N_, int n_, int W_, const double *x, int M_, int P_) {
   patched = g->has_patches && (P % 64 == 0);
   blocked = g->has_blocks && (P % 64 == 0);
   if (blocked) part.alloc((size_t)std::max(g->nbentry, 1) * P * sizeof(cx<T>));
+  if (patched && g->nslot > 0) gpart.alloc((size_t)g->nslot * kPatchR * kPatchC * P * sizeof(cx<T>));
   X.alloc((size_t)N * n * P * sizeof(cx<T>));
   grid.alloc((size_t)n * n * P * sizeof(cx<T>));
 }
@@ -451,10 +496,10 @@ void NfftOp<T>::trafo(const cx<T> *fhat, cx<T> *f, hipStream_t st, const int *se
     // algorithmic bytes: grid read once, node values written, node coordinates
     ProfScope ps(KC_INTERP, st, ((double)n * n * sizeof(cx<T>) + (double)M * sizeof(cx<T>) + 2.0 * M * sizeof(T)) * P);
     const int G = P / 64;
-    if (g->nbitem > 0)
-      hipLaunchKernelGGL(k_interp_block<T>, dim3((unsigned)((size_t)(g->nbitem + 7) / 8 * 8 * G)), dim3(kThreads), 0, st,
-                         grid.as<cx<T>>(), part.as<cx<T>>(), g->border.template as<int>(), g->bptr.template as<int>(),
-                         g->irec.template as<PatchRec<T>>(), n, P, g->nbitem);
+    if (g->niseg > 0)
+      hipLaunchKernelGGL(k_interp_block<T>, dim3((unsigned)((size_t)(g->niseg + 7) / 8 * 8 * G)), dim3(kThreads), 0, st,
+                         grid.as<cx<T>>(), part.as<cx<T>>(), g->iseg.template as<PatchSeg>(),
+                         g->irec.template as<PatchRec<T>>(), n, P, g->niseg);
     hipLaunchKernelGGL(k_interp_sum<T>, dim3(cdiv((size_t)M * G, 4)), dim3(kThreads), 0, st, part.as<cx<T>>(), f,
                        g->nptr.template as<int>(), g->nent.template as<int>(), M, P);
     NUDIM_HIP(hipGetLastError());
@@ -498,9 +543,12 @@ void NfftOp<T>::adjoint(const cx<T> *f, cx<T> *fhat, hipStream_t st) {
   // algorithmic bytes: node values read, node ids of the inverse map, grid written
   ProfScope ps(KC_GATHER, st, ((double)M * sizeof(cx<T>) + 4.0 * M + (double)ncell * sizeof(cx<T>)) * P);
   if (patched) {
-    hipLaunchKernelGGL(k_gather_patch<T>, dim3(cdiv((size_t)g->npatch * (P / 64), 4)), dim3(kThreads), 0, st, f,
-                       grid.as<cx<T>>(), g->porder.template as<int>(), g->pptr.template as<int>(),
-                       g->grec.template as<PatchRec<T>>(), n, P, g->npatch);
+    hipLaunchKernelGGL(k_gather_patch<T>, dim3(cdiv((size_t)g->ngseg * (P / 64), 4)), dim3(kThreads), 0, st, f,
+                       grid.as<cx<T>>(), gpart.as<cx<T>>(), g->gseg.template as<PatchSeg>(),
+                       g->grec.template as<PatchRec<T>>(), n, P, g->ngseg);
+    if (g->ncomb > 0)
+      hipLaunchKernelGGL(k_gather_combine<T>, dim3(cdiv((size_t)g->ncomb * (P / 64), 4)), dim3(kThreads), 0, st,
+                         gpart.as<cx<T>>(), grid.as<cx<T>>(), g->gcomb.template as<int>(), n, P, g->ncomb);
   } else {
     cx<T> *gp = grid.as<cx<T>>();
     const int *cp = g->cell_ptr.template as<int>();

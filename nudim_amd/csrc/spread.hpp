@@ -116,24 +116,33 @@ __device__ __forceinline__ void gather_add(cx<T> (&acc)[kPatchR][kPatchC], const
   gather_col<7>(acc, c, k, sv);
 }
 
+// One segment of a patch's entry list: the patch, entries [e0, e1) and the
+// partial-sum slot (-1: the patch's only segment, which writes the grid).
+struct PatchSeg {
+  int pid, e0, e1, slot;
+};
+
 // g[cell][p] = sum over the cell's (node, t0, t1) of
 //              psi0[node][t0] (psi1[node][t1] s[node][p]), in node order.
+// A segment of a patch split into several (its entry list longer than one
+// wavefront should walk alone) writes its sum to part[slot][cell][p] for
+// k_gather_combine.
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
-k_gather_patch(const cx<T> *__restrict__ s, cx<T> *__restrict__ g, const int *__restrict__ order,
-               const int *__restrict__ pptr, const PatchRec<T> *__restrict__ rec, int n, int P, int npatch) {
+k_gather_patch(const cx<T> *__restrict__ s, cx<T> *__restrict__ g, cx<T> *__restrict__ part,
+               const PatchSeg *__restrict__ seg, const PatchRec<T> *__restrict__ rec, int n, int P, int nseg) {
   constexpr int PR = kPatchR, PC = kPatchC;
   int item, p;
   patch_wave(P >> 6, item, p);
-  if (item >= npatch) return;
-  const int pid = order[item];
-  const int npc = n / PC, a = pid / npc, b = pid - a * npc;
+  if (item >= nseg) return;
+  const PatchSeg sg = seg[item];
+  const int npc = n / PC, a = sg.pid / npc, b = sg.pid - a * npc;
   cx<T> acc[PR][PC];
 #pragma unroll
   for (int i = 0; i < PR; i++)
 #pragma unroll
     for (int j = 0; j < PC; j++) acc[i][j] = mk((T)0, (T)0);
-  const int e0 = pptr[pid], cnt = pptr[pid + 1] - e0;
+  const int e0 = sg.e0, cnt = sg.e1 - sg.e0;
   for (int c0 = 0; c0 < cnt; c0 += 32) {
     const int m = min(32, cnt - c0);
     RecChunk<T, 32> ch;
@@ -151,10 +160,50 @@ k_gather_patch(const cx<T> *__restrict__ s, cx<T> *__restrict__ g, const int *__
     }
     if (k < m) gather_add(acc, ch, k, sa);
   }
+  if (sg.slot < 0) {
 #pragma unroll
-  for (int i = 0; i < PR; i++)
+    for (int i = 0; i < PR; i++)
 #pragma unroll
-    for (int j = 0; j < PC; j++) g[((size_t)(a * PR + i) * n + (size_t)(b * PC + j)) * P + p] = acc[i][j];
+      for (int j = 0; j < PC; j++) g[((size_t)(a * PR + i) * n + (size_t)(b * PC + j)) * P + p] = acc[i][j];
+  } else {
+#pragma unroll
+    for (int i = 0; i < PR; i++)
+#pragma unroll
+      for (int j = 0; j < PC; j++) part[((size_t)sg.slot * PR * PC + i * PC + j) * P + p] = acc[i][j];
+  }
+}
+
+// The patches split into segments: cell = the segments' partial sums added
+// in segment order.  comb holds (patch, first slot, count) triples; the
+// wavefront layout of patch_wave.
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_gather_combine(const cx<T> *__restrict__ part, cx<T> *__restrict__ g, const int *__restrict__ comb, int n, int P,
+                 int ncomb) {
+  constexpr int PR = kPatchR, PC = kPatchC;
+  int item, p;
+  patch_wave(P >> 6, item, p);
+  if (item >= ncomb) return;
+  const int pid = comb[3 * item], slot0 = comb[3 * item + 1], nseg = comb[3 * item + 2];
+  const int npc = n / PC, a = pid / npc, b = pid - a * npc;
+  // a patch row (PC cells) at a time, each segment's PC loads in flight together
+#pragma unroll
+  for (int i = 0; i < PR; i++) {
+    const cx<T> *src = part + ((size_t)slot0 * PR * PC + i * PC) * P + p;
+    cx<T> acc[PC];
+#pragma unroll
+    for (int j = 0; j < PC; j++) acc[j] = src[(size_t)j * P];
+    for (int k = 1; k < nseg; k++) {
+      src += (size_t)PR * PC * P;
+      cx<T> v[PC];
+#pragma unroll
+      for (int j = 0; j < PC; j++) v[j] = src[(size_t)j * P];
+#pragma unroll
+      for (int j = 0; j < PC; j++) acc[j] = acc[j] + v[j];
+    }
+#pragma unroll
+    for (int j = 0; j < PC; j++) g[((size_t)(a * PR + i) * n + (size_t)(b * PC + j)) * P + p] = acc[j];
+  }
 }
 
 template <int I, typename T>
@@ -167,9 +216,9 @@ __device__ __forceinline__ void interp_row(cx<T> &acc, const cx<T> (&v)[kPatchR]
   acc = acc + c.template a0<I>(k) * racc;
 }
 
-// Interpolation, first half.  A block takes one 8 x 16-cell block patch of
-// the order (its nonempty ones) and 64 planes.  Wavefront w holds the 4 x 8
-// quarter (rows 4 (w / 2) ..,
+// Interpolation, first half.  A block takes one segment (a range of at most
+// a few hundred entries) of an 8 x 16-cell block patch and 64 planes.
+// Wavefront w holds the 4 x 8 quarter (rows 4 (w / 2) ..,
 // columns 8 (w % 2) ..) in registers and reads quarter-record w of each
 // entry (node < 0: the quarter is outside the node's window).  Per entry it
 // forms sum over the quarter's rows of a0[i] (sum over the row's cells of
@@ -178,20 +227,21 @@ __device__ __forceinline__ void interp_row(cx<T> &acc, const cx<T> (&v)[kPatchR]
 // through LDS eight entries at a time, give part[e][p].
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
-k_interp_block(const cx<T> *__restrict__ g, cx<T> *__restrict__ part, const int *__restrict__ order,
-               const int *__restrict__ pptr, const PatchRec<T> *__restrict__ rec, int n, int P, int nitem) {
+k_interp_block(const cx<T> *__restrict__ g, cx<T> *__restrict__ part, const PatchSeg *__restrict__ seg,
+               const PatchRec<T> *__restrict__ rec, int n, int P, int nitem) {
   constexpr int PR = kPatchR, PC = kPatchC, KB = 8;
   __shared__ cx<T> red[KB][4][64];
   // block b runs on XCD b % 8 and is its (b / 8)-th block there: XCD x takes
-  // items x, x + 8, .. of the order, each over its G plane groups in turn
-  // (the blocks reading one item's records share an L2)
+  // segments x, x + 8, .. of the order, each over its G plane groups in turn
+  // (the blocks reading one segment's records share an L2)
   const int G = P >> 6, i8 = (int)blockIdx.x >> 3;
   const int item = (i8 / G) * 8 + (int)(blockIdx.x & 7);
   if (item >= nitem) return;
   const int lane = (int)(threadIdx.x & 63);
   const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int p = (i8 % G) * 64 + lane;
-  const int pid = order[item];
+  const PatchSeg sg = seg[item];
+  const int pid = sg.pid;
   const int nbc = n / (2 * PC), a = pid / nbc, b = pid - a * nbc;
   const int r0 = a * 2 * PR + (w >> 1) * PR, q0 = b * 2 * PC + (w & 1) * PC;
   cx<T> v[PR][PC];
@@ -199,7 +249,7 @@ k_interp_block(const cx<T> *__restrict__ g, cx<T> *__restrict__ part, const int 
   for (int i = 0; i < PR; i++)
 #pragma unroll
     for (int j = 0; j < PC; j++) v[i][j] = g[((size_t)(r0 + i) * n + (size_t)(q0 + j)) * P + p];
-  const int e0 = pptr[pid], cnt = pptr[pid + 1] - e0;
+  const int e0 = sg.e0, cnt = sg.e1 - sg.e0;
   for (int c0 = 0; c0 < cnt; c0 += KB) {
     const int m = min(KB, cnt - c0);
     RecChunk<T, KB> ch;
