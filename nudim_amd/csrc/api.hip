@@ -378,6 +378,9 @@ struct Job : JobBase {
     if (form == NUDIM_CG_TOEPLITZ) toep.init(N, g.x.data(), g.w.empty() ? nullptr : g.w.data(), M, Pt, st);
     cg.init(N, M, Pt, form, &op, form == NUDIM_CG_TOEPLITZ ? &toep : nullptr, dw.as<double>());
     proj_start = projected_start(form, N);
+    // passes or solves that interpolate: the NFFT CGNR, and the pass without
+    // the projected start (A fh for the projector)
+    if (form != NUDIM_CG_TOEPLITZ || !proj_start) op.prepare_trafo();
     if (proj_start) {
       if (keep_on((size_t)10 * NN * sizeof(cx<T>))) cg.keep_directions(10);
       // kernel of T_U: the weights of the nodes the projector does not
@@ -996,8 +999,10 @@ extern "C" int nudim_estimate_bytes(const nudim_params *p, size_t *bytes) {
     if (P % 64 == 0 && n % pc == 0 && n >= W + pc)
       b += entries(pr, pc) * (cb == 16 ? sizeof(PatchRec<double>) : sizeof(PatchRec<float>)) + (size_t)n * n / (pr * pc) * 16 +
            2 * entries(pr, pc) / 128 * pr * pc * P * cb;
+    // (the partial sums only for jobs that interpolate: Job's prepare_trafo)
+    const bool interp = form != NUDIM_CG_TOEPLITZ || !projected_start(form, (int)N);
     if (P % 64 == 0 && n % (2 * pc) == 0 && n >= W + 2 * pc)
-      b += entries(2 * pr, 2 * pc) * (P * cb + 4 + 4 * (cb == 16 ? sizeof(PatchRec<double>) : sizeof(PatchRec<float>))) +
+      b += entries(2 * pr, 2 * pc) * ((interp ? P * cb : 0) + 4 + 4 * (cb == 16 ? sizeof(PatchRec<double>) : sizeof(PatchRec<float>))) +
            (size_t)n * n / (4 * pr * pc) * 8;
   }
   if (form == NUDIM_CG_TOEPLITZ) b += (size_t)4 * N * N * (cb + rb) + (size_t)N * 2 * N * P * cb + NN * cb + 2 * N * P * 8;

@@ -109,6 +109,9 @@ struct NfftOp {
   void trafo_rsums(const cx<T> *fhat, const cx<T> *meas, const int *sel, int msel, double *part1, double *part2,
                    int nblocks, hipStream_t st);
   void grid_fft(const cx<T> *fhat, hipStream_t st);  // deconvolve, pad, 2D FFT into grid
+  // allocate what trafo needs beyond init (the patch form's partial sums; a
+  // first trafo does it too, which a graph capture must not contain)
+  void prepare_trafo();
   void adjoint(const cx<T> *f, cx<T> *fhat, hipStream_t st);
 };
 

@@ -424,7 +424,8 @@ void NfftOp<T>::init(int N_, int n_, int W_, const double *x, int M_, int P_) {
   g = nfft_geometry<T>(N, n, W, x, M);
   patched = g->has_patches && (P % 64 == 0);
   blocked = g->has_blocks && (P % 64 == 0);
-  if (blocked) part.alloc((size_t)std::max(g->nbentry, 1) * P * sizeof(cx<T>));
+  // the interpolation's partial sums are allocated by its first call: FP32
+  // jobs run the Toeplitz operator and may never interpolate
   if (patched && g->nslot > 0) gpart.alloc((size_t)g->nslot * kPatchR * kPatchC * P * sizeof(cx<T>));
   X.alloc((size_t)N * n * P * sizeof(cx<T>));
   grid.alloc((size_t)n * n * P * sizeof(cx<T>));
@@ -488,11 +489,17 @@ void NfftOp<T>::trafo_rsums(const cx<T> *fhat, const cx<T> *meas, const int *sel
 }
 
 template <typename T>
+void NfftOp<T>::prepare_trafo() {
+  if (blocked && !part.p) part.alloc((size_t)std::max(g->nbentry, 1) * P * sizeof(cx<T>));
+}
+
+template <typename T>
 void NfftOp<T>::trafo(const cx<T> *fhat, cx<T> *f, hipStream_t st, const int *sel, int msel) {
   grid_fft(fhat, st);
   int PCs = plane_group_log2(P);
   const int Mc = sel ? msel : M;
   if (blocked && !sel) {
+    prepare_trafo();
     // algorithmic bytes: grid read once, node values written, node coordinates
     ProfScope ps(KC_INTERP, st, ((double)n * n * sizeof(cx<T>) + (double)M * sizeof(cx<T>) + 2.0 * M * sizeof(T)) * P);
     const int G = P / 64;
