@@ -991,19 +991,24 @@ extern "C" int nudim_estimate_bytes(const nudim_params *p, size_t *bytes) {
   b += (size_t)N * n * P * cb + (size_t)n * n * P * cb;           // NFFT X, grid
   b += (size_t)M * W * W * (4 + rb) + (size_t)n * n * 4 + M * W * 16;  // gather maps, window tables
   {
-    // patch lists (records of both kernels) and the interpolation's partial
-    // sums, at most one entry per patch band pair a window meets
+    // patch lists: at most one entry per pair of patch bands a window meets
     auto entries = [&](int pr, int pc) { return M * (size_t)((W + pr - 2) / pr + 1) * ((W + pc - 2) / pc + 1); };
     const int pr = kPatchR, pc = kPatchC;
-    // (the gather's split patches: at most 2 E / 128 segment sums of pr pc cells)
-    if (P % 64 == 0 && n % pc == 0 && n >= W + pc)
-      b += entries(pr, pc) * (cb == 16 ? sizeof(PatchRec<double>) : sizeof(PatchRec<float>)) + (size_t)n * n / (pr * pc) * 16 +
-           2 * entries(pr, pc) / 128 * pr * pc * P * cb;
-    // (the partial sums only for jobs that interpolate: Job's prepare_trafo)
+    const size_t rec = cb == 16 ? sizeof(PatchRec<double>) : sizeof(PatchRec<float>);
+    // the patch tables belong to the geometry (built at every plane count
+    // where the grid allows them); the gather's segment sums (split patches:
+    // at most 2 E / 128 of pr pc cells) and the interpolation's partial sums
+    // (only for jobs that interpolate: Job's prepare_trafo) to plane counts
+    // that are multiples of 64
     const bool interp = form != NUDIM_CG_TOEPLITZ || !projected_start(form, (int)N);
-    if (P % 64 == 0 && n % (2 * pc) == 0 && n >= W + 2 * pc)
-      b += entries(2 * pr, 2 * pc) * ((interp ? P * cb : 0) + 4 + 4 * (cb == 16 ? sizeof(PatchRec<double>) : sizeof(PatchRec<float>))) +
-           (size_t)n * n / (4 * pr * pc) * 8;
+    if (n % pc == 0 && n >= W + pc) {
+      b += entries(pr, pc) * (rec + 16) + (size_t)n * n / (pr * pc) * 16 + (size_t)M * 4;
+      if (P % 64 == 0) b += 2 * entries(pr, pc) / 128 * pr * pc * P * cb;
+    }
+    if (n % (2 * pc) == 0 && n >= W + 2 * pc) {
+      b += entries(2 * pr, 2 * pc) * (8 + 4 * rec) + (size_t)n * n / (4 * pr * pc) * 16;
+      if (P % 64 == 0 && interp) b += entries(2 * pr, 2 * pc) * P * cb;
+    }
   }
   if (form == NUDIM_CG_TOEPLITZ) b += (size_t)4 * N * N * (cb + rb) + (size_t)N * 2 * N * P * cb + NN * cb + 2 * N * P * 8;
   if (projected_start(form, (int)N))  // T_U, X2, b_K, R partials, known list; c_K and the quadratic form's partials

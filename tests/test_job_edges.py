@@ -190,3 +190,23 @@ def test_diffusion_plane_in_lds_equals_hyperplane_launches(hip, monkeypatch, pre
             job.initial()
             out.append((job.iterate(2), job.volume()))
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("prec,np_,nz,denoise", [(0, 64, 64, 0), (1, 64, 64, 1), (0, 96, 48, 1), (1, 128, 128, 1),
+                                                 (0, 128, 128, 0), (1, 96, 96, 0)])
+def test_estimate_bounds_device_bytes(hip, orc, prec, np_, nz, denoise):
+    """The admission estimate (what the job server budgets with) is at least
+    what a job holds on the device after set-up and passes: the patch-form
+    spreading tables and partial sums (plane counts that are multiples of
+    64), the Toeplitz operators and the per-cell maps otherwise."""
+    rng = np.random.default_rng(np_ + nz)
+    proj = rng.standard_normal((41, nz, np_))
+    p = hip.make_params(np_, nz, np_, -60, 60, 3, padangle=1.0, weight=1, nhio=1, ner=1, denoise=bool(denoise),
+                        method=hip.TV, height=np_ // 2, center=np_ // 2, precision=prec)
+    est = hip.estimate_bytes(p)
+    with hip.Reconstruction(p) as job:
+        job.load_projections(proj)
+        job.initial()
+        job.iterate(2)
+        held = job.device_bytes()
+    assert 0 < held <= est
