@@ -993,21 +993,23 @@ extern "C" int nudim_estimate_bytes(const nudim_params *p, size_t *bytes) {
   {
     // patch lists: at most one entry per pair of patch bands a window meets
     auto entries = [&](int pr, int pc) { return M * (size_t)((W + pr - 2) / pr + 1) * ((W + pc - 2) / pc + 1); };
-    const int pr = kPatchR, pc = kPatchC;
+    const int gr = kGatherR, gc = kGatherC, pr = kPatchR, pc = kPatchC;
+    const size_t grec = cb == 16 ? sizeof(PatchRec<double, kGatherR, kGatherC>) : sizeof(PatchRec<float, kGatherR, kGatherC>);
     const size_t rec = cb == 16 ? sizeof(PatchRec<double>) : sizeof(PatchRec<float>);
     // the patch tables belong to the geometry (built at every plane count
     // where the grid allows them); the gather's segment sums (split patches:
-    // at most 2 E / 128 of pr pc cells) and the interpolation's partial sums
+    // at most 2 E / 128 of gr gc cells) and the interpolation's partial sums
     // (only for jobs that interpolate: Job's prepare_trafo) to plane counts
     // that are multiples of 64
     const bool interp = form != NUDIM_CG_TOEPLITZ || !projected_start(form, (int)N);
-    if (n % pc == 0 && n >= W + pc) {
-      b += entries(pr, pc) * (rec + 16) + (size_t)n * n / (pr * pc) * 16 + (size_t)M * 4;
-      if (P % 64 == 0) b += 2 * entries(pr, pc) / 128 * pr * pc * P * cb;
-    }
-    if (n % (2 * pc) == 0 && n >= W + 2 * pc) {
-      b += entries(2 * pr, 2 * pc) * (8 + 4 * rec) + (size_t)n * n / (4 * pr * pc) * 16;
-      if (P % 64 == 0 && interp) b += entries(2 * pr, 2 * pc) * P * cb;
+    if (n % gr == 0 && n % gc == 0 && n >= W + std::max(gr, gc)) {
+      // records, node lists, segments (padded to the XCD sectors: at most twice the patches)
+      b += entries(gr, gc) * (grec + 16) + (size_t)2 * n * n / (gr * gc) * 16 + (size_t)M * 4;
+      if (P % 64 == 0) b += 2 * entries(gr, gc) / 128 * gr * gc * P * cb;
+      if (n % (2 * pc) == 0 && n % (2 * pr) == 0 && n >= W + 2 * pc) {
+        b += entries(2 * pr, 2 * pc) * (8 + 4 * rec) + (size_t)n * n / (4 * pr * pc) * 16;
+        if (P % 64 == 0 && interp) b += entries(2 * pr, 2 * pc) * P * cb;
+      }
     }
   }
   if (form == NUDIM_CG_TOEPLITZ) b += (size_t)4 * N * N * (cb + rb) + (size_t)N * 2 * N * P * cb + NN * cb + 2 * N * P * 8;

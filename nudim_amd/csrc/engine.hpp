@@ -77,9 +77,9 @@ struct NfftGeom {
   // a block, and the blocks of an XCD, read the same grid cells from L2
   DevBuf perm;
   // patch lists of the patch-form kernels (spread.hpp; used when P % 64 == 0):
-  // the gather's 4 x 8-cell patches (entry-list segments, longest first, with
-  // the split patches' (patch, first slot, count) triples; PatchRec<T>
-  // records) and the interpolation's 8 x 16-cell block patches (segments of
+  // the gather's 8 x 4-cell patches (entry-list segments dealt to the XCDs by
+  // angular sector, longest first in each, with the split patches' (patch,
+  // first slot, count) triples; PatchRec<T, 8, 4> records) and the interpolation's 8 x 16-cell block patches (segments of
   // the nonempty ones, four PatchRec<T> quarter records per entry, each
   // node's entries)
   bool has_patches = false, has_blocks = false;

@@ -259,6 +259,21 @@ static void patch_segments(const PatchLists &pl, bool keep_empty, int smax, std:
   *nslot = slot;
 }
 
+// the gather's entry records, with one spare record at the end (RecChunk
+// reads pieces past the last)
+template <typename T, int R, int C>
+static void gather_records(const PatchLists &pl, DevBuf *out) {
+  std::vector<PatchRec<T, R, C>> gr(pl.ent.size() + 1);
+  for (size_t e = 0; e < pl.ent.size(); e++) {
+    const int m = (int)(pl.ent[e] & 0xFFFFFu), k0 = (int)((pl.ent[e] >> 20) & 63u), k1 = (int)(pl.ent[e] >> 26);
+    const double *a0 = pl.psi0p.data() + (size_t)m * pl.l0 + k0, *a1 = pl.psi1p.data() + (size_t)m * pl.l1 + k1;
+    gr[e].node = m;
+    for (int i = 0; i < R; i++) gr[e].a0[i] = (T)a0[i];
+    for (int j = 0; j < C; j++) gr[e].w1[j] = (T)a1[j];
+  }
+  upload(*out, gr);
+}
+
 template <typename T>
 static std::shared_ptr<NfftGeom<T>> build_nfft_geometry(int N, int n, int W, const double *x, int M) {
   NUDIM_CHECK(N % 2 == 0 && n >= N && n % 2 == 0, "NFFT sizes must be even with n >= N");
@@ -302,9 +317,9 @@ static std::shared_ptr<NfftGeom<T>> build_nfft_geometry(int N, int n, int W, con
     upload(g->perm, perm);
   }
   {
-    // gather: 4 x 8-cell patches, one wavefront each
+    // gather: kGatherR x kGatherC-cell patches, one wavefront each
     PatchLists pl;
-    g->has_patches = g_spread_patch && build_patch_lists(t, kPatchR, kPatchC, &pl);
+    g->has_patches = g_spread_patch && build_patch_lists(t, kGatherR, kGatherC, &pl);
     if (g->has_patches) {
       g->npatch = (int)pl.order.size();
       std::vector<PatchSeg> sg;
@@ -313,20 +328,34 @@ static std::shared_ptr<NfftGeom<T>> build_nfft_geometry(int N, int n, int W, con
       // 128 entries, or of the entries per wavefront slot of the GPU (3072)
       // when that is more, where the longest lists start first anyway
       patch_segments(pl, true, std::max<int>(128, (int)(pl.ent.size() / 3072)), &sg, &comb, &g->nslot);
+      {
+        // XCD x (blocks b with b % 8 = x) takes the patches of angular sector
+        // x around the grid's origin (frequency 0 at cell 0, periodic), each
+        // sector in the heavy-first order: position 8 k + x holds sector x's
+        // k-th segment, the shorter sectors padded with pid = -1.  A node's
+        // patches then meet in one XCD's L2 at about the same time.
+        std::vector<PatchSeg> sec[8];
+        const int npc = n / kGatherC;
+        for (const PatchSeg &q : sg) {
+          const int a = q.pid / npc, b = q.pid - a * npc;
+          int cy = a * kGatherR + kGatherR / 2, cx = b * kGatherC + kGatherC / 2;
+          if (cy >= n / 2) cy -= n;
+          if (cx >= n / 2) cx -= n;
+          const double ang = std::atan2((double)cy, (double)cx);
+          const int x = std::min(7, std::max(0, (int)std::floor((ang + M_PI) / (2 * M_PI) * 8)));
+          sec[x].push_back(q);
+        }
+        size_t len = 0;
+        for (auto &v : sec) len = std::max(len, v.size());
+        sg.assign(8 * len, PatchSeg{-1, 0, 0, -1});
+        for (int x = 0; x < 8; x++)
+          for (size_t k = 0; k < sec[x].size(); k++) sg[8 * k + x] = sec[x][k];
+      }
       g->ngseg = (int)sg.size();
       g->ncomb = (int)comb.size() / 3;
       upload(g->gseg, sg);
       if (!comb.empty()) upload(g->gcomb, comb);
-      // one spare record at the end (RecChunk reads pieces past the last)
-      std::vector<PatchRec<T>> gr(pl.ent.size() + 1);
-      for (size_t e = 0; e < pl.ent.size(); e++) {
-        const int m = (int)(pl.ent[e] & 0xFFFFFu), k0 = (int)((pl.ent[e] >> 20) & 63u), k1 = (int)(pl.ent[e] >> 26);
-        const double *a0 = pl.psi0p.data() + (size_t)m * pl.l0 + k0, *a1 = pl.psi1p.data() + (size_t)m * pl.l1 + k1;
-        gr[e].node = m;
-        for (int i = 0; i < kPatchR; i++) gr[e].a0[i] = (T)a0[i];
-        for (int j = 0; j < kPatchC; j++) gr[e].w1[j] = (T)a1[j];
-      }
-      upload(g->grec, gr);
+      gather_records<T, kGatherR, kGatherC>(pl, &g->grec);
     }
     // interpolation: 8 x 16-cell block patches, one block each; quarter q of
     // an entry is the 4 x 8 part at rows 4 (q / 2) .., columns 8 (q % 2) ..
@@ -426,7 +455,7 @@ void NfftOp<T>::init(int N_, int n_, int W_, const double *x, int M_, int P_) {
   blocked = g->has_blocks && (P % 64 == 0);
   // the interpolation's partial sums are allocated by its first call: FP32
   // jobs run the Toeplitz operator and may never interpolate
-  if (patched && g->nslot > 0) gpart.alloc((size_t)g->nslot * kPatchR * kPatchC * P * sizeof(cx<T>));
+  if (patched && g->nslot > 0) gpart.alloc((size_t)g->nslot * kGatherR * kGatherC * P * sizeof(cx<T>));
   X.alloc((size_t)N * n * P * sizeof(cx<T>));
   grid.alloc((size_t)n * n * P * sizeof(cx<T>));
 }
@@ -550,12 +579,15 @@ void NfftOp<T>::adjoint(const cx<T> *f, cx<T> *fhat, hipStream_t st) {
   // algorithmic bytes: node values read, node ids of the inverse map, grid written
   ProfScope ps(KC_GATHER, st, ((double)M * sizeof(cx<T>) + 4.0 * M + (double)ncell * sizeof(cx<T>)) * P);
   if (patched) {
-    hipLaunchKernelGGL(k_gather_patch<T>, dim3(cdiv((size_t)g->ngseg * (P / 64), 4)), dim3(kThreads), 0, st, f,
-                       grid.as<cx<T>>(), gpart.as<cx<T>>(), g->gseg.template as<PatchSeg>(),
-                       g->grec.template as<PatchRec<T>>(), n, P, g->ngseg);
+    // 8 XCD sequences of ngseg / 8 items, G wavefronts each
+    const unsigned gb = 8 * (unsigned)cdiv((size_t)g->ngseg / 8 * (P / 64), 4);
+    hipLaunchKernelGGL((k_gather_patch<T, kGatherR, kGatherC>), dim3(gb), dim3(kThreads), 0, st, f, grid.as<cx<T>>(),
+                       gpart.as<cx<T>>(), g->gseg.template as<PatchSeg>(),
+                       g->grec.template as<PatchRec<T, kGatherR, kGatherC>>(), n, P, g->ngseg);
     if (g->ncomb > 0)
-      hipLaunchKernelGGL(k_gather_combine<T>, dim3(cdiv((size_t)g->ncomb * (P / 64), 4)), dim3(kThreads), 0, st,
-                         gpart.as<cx<T>>(), grid.as<cx<T>>(), g->gcomb.template as<int>(), n, P, g->ncomb);
+      hipLaunchKernelGGL((k_gather_combine<T, kGatherR, kGatherC>), dim3(cdiv((size_t)g->ncomb * (P / 64), 4)),
+                         dim3(kThreads), 0, st, gpart.as<cx<T>>(), grid.as<cx<T>>(), g->gcomb.template as<int>(), n,
+                         P, g->ncomb);
   } else {
     cx<T> *gp = grid.as<cx<T>>();
     const int *cp = g->cell_ptr.template as<int>();

@@ -1,16 +1,19 @@
 // spread.hpp -- patch form of the NFFT window kernels (interpolation and
 // gather adjoint) for plane counts that are multiples of 64.
 //
-// A wavefront owns a patch of 4 x 8 grid cells and 64 planes (lane = plane),
-// the patch held in registers, and walks the patch's entry list: the nodes
-// whose window meets the patch, in node order (PatchLists in geom.hpp).
-// Every per-node quantity is wave-uniform -- the node and its window weights
-// over the patch arrive 32 (or 8) records at a time with vector loads and
-// are broadcast with readlane -- and each node value is one coalesced
-// 64-plane load per patch instead of one per tap.  The whole patch takes
-// part in every entry: weights outside the node's window are zero.
+// A wavefront owns a patch of grid cells (8 x 4 in the gather, 4 x 8 in the
+// interpolation) and 64 planes (lane = plane), the patch held in registers,
+// and walks the patch's entry list: the nodes whose window meets the patch,
+// in node order (PatchLists in geom.hpp).  Every per-node quantity is
+// wave-uniform -- the node and its window weights over the patch arrive 32
+// (or 8) records at a time with vector loads and are broadcast with
+// readlane -- and each node value is one coalesced 64-plane load per patch
+// instead of one per tap.  The whole patch takes part in every entry:
+// weights outside the node's window are zero.
 //
-// The gather writes every grid cell exactly once and needs no partial sums.
+// The gather writes every grid cell exactly once and needs no partial sums
+// (patches with long lists are split into segments whose sums a second pass
+// adds in segment order).
 // The interpolation reads every grid cell once; a block of four wavefronts
 // covers an 8 x 16-cell block patch, adds its quarters' sums through LDS and
 // leaves one partial sum per (block patch, node) entry, which k_interp_sum
@@ -23,22 +26,38 @@
 
 #include "kernels.hpp"
 
+#include <type_traits>
+
 namespace nudim {
 
-// patch shape: 4 x 8 cells, 32 complex values per lane
+// patch shapes (rows x columns), 32 complex values per lane: the
+// interpolation's quarter patches and the gather's patches (per entry, four
+// columns of w1[j] s[node] and 32 products into the rows)
 constexpr int kPatchR = 4, kPatchC = 8;
+constexpr int kGatherR = 8, kGatherC = 4;
 
 // One entry of a patch's list as the kernels read it: the node and its
-// window weights over the patch, a0[i] = psi0[node][t0] for patch row i and
-// w1[j] = psi1[node][t1] for patch column j (zero outside the window), in
+// window weights over an R x C patch, a0[i] = psi0[node][t0] for patch row i
+// and w1[j] = psi1[node][t1] for patch column j (zero outside the window), in
 // the kernel's precision.
-template <typename WT>
+template <typename WT, int R = kPatchR, int C = kPatchC>
 struct PatchRec {
   int node, pad[3];
-  WT a0[kPatchR];
-  WT w1[kPatchC];
+  WT a0[R];
+  WT w1[C];
 };
 static_assert(sizeof(PatchRec<double>) == 112 && sizeof(PatchRec<float>) == 64, "record layout");
+static_assert(sizeof(PatchRec<double, kGatherR, kGatherC>) == 112 && sizeof(PatchRec<float, kGatherR, kGatherC>) == 64,
+              "record layout");
+
+// f(integral_constant<int, I>) for I = B .. E - 1, unrolled at compile time
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
 
 // Wavefront q = 4 blockIdx.x + w (w = threadIdx.x / 64) takes item q / G of
 // an order and planes 64 (q % G) ..; G = P / 64 plane groups.  With G = 4 a
@@ -57,12 +76,13 @@ __device__ __forceinline__ void patch_wave(int G, int &item, int &p) {
 // ordinary vector loads (a chunk costs one memory latency instead of one per
 // entry) and each field is broadcast with readlane.  A record array carries
 // one spare record at its end for the pieces read past the last record.
-template <typename WT, int K>
+template <typename WT, int K, int R = kPatchR, int C = kPatchC>
 struct RecChunk {
-  static constexpr int NP = (int)sizeof(PatchRec<WT>) / 16, NG = 64 / K, NJ = (NP + NG - 1) / NG;
-  static_assert(sizeof(PatchRec<WT>) % 16 == 0 && NG * NJ - NP <= NP, "pieces past a record stay in the next");
+  using Rec = PatchRec<WT, R, C>;
+  static constexpr int NP = (int)sizeof(Rec) / 16, NG = 64 / K, NJ = (NP + NG - 1) / NG;
+  static_assert(sizeof(Rec) % 16 == 0 && NG * NJ - NP <= NP, "pieces past a record stay in the next");
   uint4 q[NJ];
-  __device__ __forceinline__ void load(const PatchRec<WT> *__restrict__ r, int m, int stride) {
+  __device__ __forceinline__ void load(const Rec *__restrict__ r, int m, int stride) {
     const int lane = (int)(threadIdx.x & 63);
     const uint4 *rp = reinterpret_cast<const uint4 *>(r + (size_t)min(lane % K, m - 1) * stride) + lane / K;
 #pragma unroll
@@ -87,33 +107,18 @@ struct RecChunk {
   template <int I>
   __device__ __forceinline__ WT a0(int k) const { return val<4 + I * (int)sizeof(WT) / 4>(k); }
   template <int J>
-  __device__ __forceinline__ WT w1(int k) const { return val<4 + (kPatchR + J) * (int)sizeof(WT) / 4>(k); }
+  __device__ __forceinline__ WT w1(int k) const { return val<4 + (R + J) * (int)sizeof(WT) / 4>(k); }
 };
-
-template <int J, typename T>
-__device__ __forceinline__ void gather_col(cx<T> (&acc)[kPatchR][kPatchC], const RecChunk<T, 32> &c, int k,
-                                           cx<T> sv) {
-  const cx<T> cs = c.template w1<J>(k) * sv;
-#define NUDIM_GROW(I) acc[I][J] = acc[I][J] + c.template a0<I>(k) * cs;
-  NUDIM_GROW(0) NUDIM_GROW(1) NUDIM_GROW(2) NUDIM_GROW(3)
-#undef NUDIM_GROW
-}
 
 // one entry (record k of the chunk) added into the patch: column by column,
 // w1[j] s[node] then a0[i] times that into each row (the weights outside the
 // window are zero)
-template <typename T>
-__device__ __forceinline__ void gather_add(cx<T> (&acc)[kPatchR][kPatchC], const RecChunk<T, 32> &c, int k,
-                                           cx<T> sv) {
-  static_assert(kPatchR == 4 && kPatchC == 8, "unrolled for 4 x 8 patches");
-  gather_col<0>(acc, c, k, sv);
-  gather_col<1>(acc, c, k, sv);
-  gather_col<2>(acc, c, k, sv);
-  gather_col<3>(acc, c, k, sv);
-  gather_col<4>(acc, c, k, sv);
-  gather_col<5>(acc, c, k, sv);
-  gather_col<6>(acc, c, k, sv);
-  gather_col<7>(acc, c, k, sv);
+template <typename T, int R, int C>
+__device__ __forceinline__ void gather_add(cx<T> (&acc)[R][C], const RecChunk<T, 32, R, C> &c, int k, cx<T> sv) {
+  static_for<0, C>([&](auto J) {
+    const cx<T> cs = c.template w1<J>(k) * sv;
+    static_for<0, R>([&](auto I) { acc[I][J] = acc[I][J] + c.template a0<I>(k) * cs; });
+  });
 }
 
 // One segment of a patch's entry list: the patch, entries [e0, e1) and the
@@ -127,15 +132,25 @@ struct PatchSeg {
 // A segment of a patch split into several (its entry list longer than one
 // wavefront should walk alone) writes its sum to part[slot][cell][p] for
 // k_gather_combine.
-template <typename T>
+//
+// The segments arrive dealt to the 8 XCDs: block b runs on XCD x = b % 8 and
+// its wavefronts take the XCD's items il = 0, 1, .. (4 (b / 8) + w = G il +
+// plane group), item 8 il + x of seg; seg places the segments of angular
+// sector x of the grid (around its origin, heavy-first inside the sector) at
+// the positions of XCD x, so that the patches sharing a node's value meet in
+// one L2.  Padding items have pid < 0.
+template <typename T, int PR, int PC>
 __global__ void __launch_bounds__(kThreads)
 k_gather_patch(const cx<T> *__restrict__ s, cx<T> *__restrict__ g, cx<T> *__restrict__ part,
-               const PatchSeg *__restrict__ seg, const PatchRec<T> *__restrict__ rec, int n, int P, int nseg) {
-  constexpr int PR = kPatchR, PC = kPatchC;
-  int item, p;
-  patch_wave(P >> 6, item, p);
+               const PatchSeg *__restrict__ seg, const PatchRec<T, PR, PC> *__restrict__ rec, int n, int P,
+               int nseg) {
+  const int G = P >> 6, x = (int)(blockIdx.x & 7);
+  const int lq = __builtin_amdgcn_readfirstlane((int)((blockIdx.x >> 3) * 4 + (threadIdx.x >> 6)));
+  const int il = lq / G, item = il * 8 + x;
+  const int p = (lq - il * G) * 64 + (int)(threadIdx.x & 63);
   if (item >= nseg) return;
   const PatchSeg sg = seg[item];
+  if (sg.pid < 0) return;
   const int npc = n / PC, a = sg.pid / npc, b = sg.pid - a * npc;
   cx<T> acc[PR][PC];
 #pragma unroll
@@ -145,20 +160,25 @@ k_gather_patch(const cx<T> *__restrict__ s, cx<T> *__restrict__ g, cx<T> *__rest
   const int e0 = sg.e0, cnt = sg.e1 - sg.e0;
   for (int c0 = 0; c0 < cnt; c0 += 32) {
     const int m = min(32, cnt - c0);
-    RecChunk<T, 32> ch;
+    RecChunk<T, 32, PR, PC> ch;
     ch.load(rec + e0 + c0, m, 1);
-    // two node values in flight: entries k and k + 1 in sa and sb; each is
-    // reloaded (two entries ahead, clamped to the chunk's last) after its use
+    // three node values in flight: entries k, k + 1, k + 2 in sa, sb, sc;
+    // each is reloaded (three entries ahead, clamped to the chunk's last)
+    // after its use
     cx<T> sa = s[(size_t)ch.node(0) * P + p];
     cx<T> sb = s[(size_t)ch.node(min(1, m - 1)) * P + p];
+    cx<T> sc = s[(size_t)ch.node(min(2, m - 1)) * P + p];
     int k = 0;
-    for (; k + 2 <= m; k += 2) {
+    for (; k + 3 <= m; k += 3) {
       gather_add(acc, ch, k, sa);
-      sa = s[(size_t)ch.node(min(k + 2, m - 1)) * P + p];
+      sa = s[(size_t)ch.node(min(k + 3, m - 1)) * P + p];
       gather_add(acc, ch, k + 1, sb);
-      sb = s[(size_t)ch.node(min(k + 3, m - 1)) * P + p];
+      sb = s[(size_t)ch.node(min(k + 4, m - 1)) * P + p];
+      gather_add(acc, ch, k + 2, sc);
+      sc = s[(size_t)ch.node(min(k + 5, m - 1)) * P + p];
     }
     if (k < m) gather_add(acc, ch, k, sa);
+    if (k + 1 < m) gather_add(acc, ch, k + 1, sb);
   }
   if (sg.slot < 0) {
 #pragma unroll
@@ -176,11 +196,10 @@ k_gather_patch(const cx<T> *__restrict__ s, cx<T> *__restrict__ g, cx<T> *__rest
 // The patches split into segments: cell = the segments' partial sums added
 // in segment order.  comb holds (patch, first slot, count) triples; the
 // wavefront layout of patch_wave.
-template <typename T>
+template <typename T, int PR, int PC>
 __global__ void __launch_bounds__(kThreads)
 k_gather_combine(const cx<T> *__restrict__ part, cx<T> *__restrict__ g, const int *__restrict__ comb, int n, int P,
                  int ncomb) {
-  constexpr int PR = kPatchR, PC = kPatchC;
   int item, p;
   patch_wave(P >> 6, item, p);
   if (item >= ncomb) return;
