@@ -646,8 +646,8 @@ struct Job : JobBase {
         tv(img);
       if (nn == dump_nn && dump_after) copy_to_host_ref(img, dump_after);
     }
-    hipLaunchKernelGGL(k_constraint<T>, dim3(cdiv(Vt, 256)), dim3(256), 0, st, img, img_last, N, P, prm.nx,
-                       nn < prm.nhio ? 1 : 0, prm.config_height, prm.nheight, prm.ncenter, K);
+    hipLaunchKernelGGL(k_constraint<T>, dim3(cdiv((size_t)N * P, 256), N, K), dim3(256), 0, st, img, img_last, N, P,
+                       prm.nx, nn < prm.nhio ? 1 : 0, prm.config_height, prm.nheight, prm.ncenter);
     NUDIM_HIP(hipGetLastError());
   }
 
@@ -1431,8 +1431,8 @@ static void constraint_parity(double *img, const double *last, int nz, int np, i
                               int nc) {
   VolTmp<T> a(img, nz, np, np), b(last, nz, np, np);
   NUDIM_HIP(hipStreamSynchronize(b.ts.s));
-  hipLaunchKernelGGL(k_constraint<T>, dim3(cdiv(a.V, 256)), dim3(256), 0, a.ts.s, a.u.template as<T>(),
-                     b.u.template as<T>(), np, nz, nx, is_hio, ch, nh, nc, 1);
+  hipLaunchKernelGGL(k_constraint<T>, dim3(cdiv((size_t)np * nz, 256), np, 1), dim3(256), 0, a.ts.s,
+                     a.u.template as<T>(), b.u.template as<T>(), np, nz, nx, is_hio, ch, nh, nc);
   NUDIM_HIP(hipGetLastError());
   a.out(img);
 }

@@ -1058,19 +1058,27 @@ k_diffusion_plane(T *__restrict__ u, int ny, int nx, int nz, int niter, double l
 // density constraint (reconstruction.c:672-700) on u[y][x][z]
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void k_constraint(T *__restrict__ u, const T *__restrict__ last, int np, int nz, int nx, int is_hio,
-                             int config_height, int nheight, int ncenter, int K = 1) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long V = (long long)np * np * nz;  // K volumes of a batch one after another
-  if (i >= V * K) return;
-  long long yx = (i % V) / nz;
-  int yi = (int)(yx / np), xi = (int)(yx % np);
+__global__ void __launch_bounds__(256)
+k_constraint(T *__restrict__ u, const T *__restrict__ last, int np, int nz, int nx, int is_hio, int config_height,
+             int nheight, int ncenter) {
+  // grid (rows of np nz values / 256, np, volumes): block y is row yi of
+  // volume blockIdx.z, threads run over (x, z) of the row
+  const int yi = (int)blockIdx.y, rowlen = np * nz;
+  const int r = (int)(blockIdx.x * 256 + threadIdx.x);
+  if (r >= rowlen) return;
+  const size_t i = ((size_t)blockIdx.z * np + yi) * rowlen + r;
   bool inside;
-  if (config_height)
-    inside = (yi >= (2 * ncenter - nheight) / 2 && yi <= (2 * ncenter + nheight) / 2);
-  else
-    inside = (yi >= (np - nx) / 2) && (yi < (np + nx) / 2) && (xi >= (np - nx) / 2) && (xi < (np + nx) / 2);
-  T v = u[i];
+  if (config_height) {
+    inside = yi >= (2 * ncenter - nheight) / 2 && yi <= (2 * ncenter + nheight) / 2;
+  } else {
+    const int xi = r / nz;
+    inside = yi >= (np - nx) / 2 && yi < (np + nx) / 2 && xi >= (np - nx) / 2 && xi < (np + nx) / 2;
+  }
+  if (!inside && !is_hio) {  // ER outside the support: 0 whatever the value
+    u[i] = (T)0;
+    return;
+  }
+  const T v = u[i];
   if (!inside || v < (T)0) u[i] = is_hio ? last[i] - (T)0.9 * v : (T)0;
 }
 
