@@ -424,6 +424,113 @@ k_fft2(const cx<T> *in, cx<T> *out, LineIO li, LineIO lo, const cx<T> *__restric
     if (HOUT || kk[k2] < lo.cnt) dst[(long long)kk[k2] * lo.sk] = b[k2];
 }
 
+// ---- lane-pair line kernel (FP64) ----------------------------------------------------
+//
+// In FP64 a lane holding all N1 = 32 values of a step-A line position (and
+// the DFT's temporaries) needs more than 256 registers, and the complex
+// exchange of 8 planes 148 KB of LDS: one wavefront per SIMD.  Here two
+// adjacent lanes share each step-A line position and each step-B column k1
+// and hold half of its values; one radix-2 step across the pair, the values
+// swapped with a lane shuffle, splits each DFT into two of half length by
+// output parity:
+//   X[2k]     = DFT_{R/2}(x[i] + x[i + R/2])[k]                (lane 0)
+//   X[2k + 1] = DFT_{R/2}((x[i] - x[i + R/2]) w_R^{S i})[k]    (lane 1)
+// with lane h holding inputs i + h R/2 (i < R/2).  The exchange goes through
+// a real buffer (real parts, then imaginary parts).
+
+// v[i] = input i + h R/2 on entry; on return v[s] = output 2 s + h
+template <typename T, int R, int S>
+__device__ __forceinline__ void dft_pair(cx<T> *v, int h) {
+  constexpr int Q = R / 2;
+#pragma unroll
+  for (int i = 0; i < Q; i++) {
+    cx<T> o;
+    o.x = __shfl_xor(v[i].x, 1);
+    o.y = __shfl_xor(v[i].y, 1);
+    const cx<T> d = h ? o - v[i] : v[i] + o;
+    v[i] = h ? mul_wconst<T, S, R>(d, i) : d;
+  }
+  dft_reg<T, Q, S>(v);
+}
+
+// One transform of sign S per line, full input and output (positions past
+// cnt are zero on input and not stored on output).  Threads 2 max(N1, N2) G:
+// lane h = tid & 1, plane g, line position r (step A) or k1 (step B) =
+// tid >> (log2 G + 1).  LDS: the real exchange buffer [N2][N1 G + G] and the
+// n twiddles.
+template <typename T, int N1, int N2, int S>
+__global__ void __launch_bounds__(512)
+k_fft2p(const cx<T> *in, cx<T> *out, LineIO li, LineIO lo, const cx<T> *__restrict__ tw, int Gs, int nchunk) {
+  constexpr int n = N1 * N2, A1 = N1 / 2, A2 = N2 / 2;
+  static_assert(N1 % 2 == 0 && N2 % 2 == 0, "even factors");
+  extern __shared__ unsigned char smem_raw[];
+  T *lx = reinterpret_cast<T *>(smem_raw);
+  const int G = 1 << Gs, G1 = G - 1;
+  const int rowA = N1 * G + G;
+  cx<T> *stw = reinterpret_cast<cx<T> *>(lx + N2 * rowA);  // N2 rowA is even: 16-byte aligned
+  const int lb = (int)xcd_block(blockIdx.x, gridDim.x);
+  const int o = lb / nchunk, chunk = lb - o * nchunk;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < n; i += blockDim.x) stw[i] = tw[i];
+  const int h = tid & 1, g = (tid >> 1) & G1, r = tid >> (Gs + 1);
+  const bool actA = r < N2, actB = r < N1;
+
+  // ---- step A: positions r + N2 (h A1 + i), N1-point DFT over the pair ----
+  cx<T> a[A1];
+  if (actA) {
+    const cx<T> *src = in + (long long)o * li.so + (long long)chunk * G + g;
+    const T so_scale = li.scale_o ? (T)li.scale_o[o] : (T)1;
+    int kk[A1];
+#pragma unroll
+    for (int i = 0; i < A1; i++) kk[i] = wrap_pos(r + N2 * (h * A1 + i) - li.shift, n);
+#pragma unroll
+    for (int i = 0; i < A1; i++) {
+      cx<T> v = mk((T)0, (T)0);
+      if (kk[i] < li.cnt) v = src[(long long)kk[i] * li.sk];
+      a[i] = v;
+    }
+    if (li.scale_k) {
+#pragma unroll
+      for (int i = 0; i < A1; i++) {
+        const T sc = kk[i] < li.cnt ? (T)li.scale_k[kk[i]] : (T)0;
+        a[i] = (sc * so_scale) * a[i];
+      }
+    } else if (li.scale_o) {
+#pragma unroll
+      for (int i = 0; i < A1; i++) a[i] = so_scale * a[i];
+    }
+    dft_pair<T, N1, S>(a, h);
+  }
+  __syncthreads();  // twiddle table complete
+  if (actA) {
+#pragma unroll
+    for (int s = 0; s < A1; s++) {
+      cx<T> w = stw[r * (2 * s + h)];
+      if (S > 0) w = conjx(w);
+      a[s] = a[s] * w;
+    }
+  }
+  // ---- step B: k1 = r, lane h holds j2 = h A2 + i ----
+  cx<T> b[A2];
+  xchg_at<T, A1, A2, true>(
+      lx, a, actA, [&](int s) { return r * rowA + (2 * s + h) * G + g; }, b, actB,
+      [&](int i) { return (h * A2 + i) * rowA + r * G + g; });
+  if (!actB) return;
+  dft_pair<T, N2, S>(b, h);
+  cx<T> *dst = out + (long long)o * lo.so + (long long)chunk * G + g;
+  const T oo_scale = lo.scale_o ? (T)lo.scale_o[o] : (T)1;
+#pragma unroll
+  for (int s = 0; s < A2; s++) {
+    const int kk = wrap_pos(r + N1 * (2 * s + h) - lo.shift, n);
+    if (kk < lo.cnt) {
+      cx<T> v = b[s];
+      if (lo.scale_k) v = ((T)lo.scale_k[kk] * oo_scale) * v;
+      else if (lo.scale_o) v = oo_scale * v;
+      dst[(long long)kk * lo.sk] = v;
+    }
+  }
+}
+
 // ---- Toeplitz passes ---------------------------------------------------------------
 //
 // The circulant embedding of an N x N plane into n x n, n = 2N, holds the

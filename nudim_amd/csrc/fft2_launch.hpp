@@ -20,6 +20,12 @@ inline bool toep_pairs_on() {
   return !(e && std::string(e) == "0");
 }
 
+// FP64 generic-form lines on lane pairs (NUDIM_FFT64_PAIRS=0: the one-lane kernel)
+inline bool fft2_pairs64_on() {
+  static const bool on = !(getenv("NUDIM_FFT64_PAIRS") && std::string(getenv("NUDIM_FFT64_PAIRS")) == "0");
+  return on;
+}
+
 template <typename T, int N1, int N2>
 bool launch_shape(const cx<T> *in, cx<T> *out, int nlines, const LineIO &li, const LineIO &lo, int sign, int P,
                   hipStream_t st, const cx<T> *tw, const cx<T> *mid, long long mid_so) {
@@ -31,6 +37,32 @@ bool launch_shape(const cx<T> *in, cx<T> *out, int nlines, const LineIO &li, con
   // the half-input 48-point stage runs on lane pairs in 512-thread blocks
   // (as the Toeplitz row passes of n = 1536)
   const bool pairs = N1 == 48 && hin && toep_pairs_on();
+  if constexpr (sizeof(T) == 8 && N1 >= 24 && N1 % 2 == 0 && N2 % 2 == 0) {
+    // FP64 lines of the generic form on lane pairs (k_fft2p)
+    if (!hin && !hout && fft2_pairs64_on()) {
+      constexpr int TL2 = 2 * (N1 > N2 ? N1 : N2);
+      // lines that write most of their positions far apart (the trafo's
+      // grid columns: one row of n P elements between positions) in
+      // 512-thread blocks of 8 planes, for 128-byte runs per position; the
+      // others in 256-thread blocks of 4 planes, three blocks per CU.  At
+      // 256^3 in FP64 (one lane per line: 4 planes, 1 block per CU): trafo
+      // column pass 2267 -> 2104 us (2477 with 4 planes), adjoint column
+      // pass 2612 -> 2256 us (2411 with 8), row passes 718 -> 527 us
+      const bool far = lo.sk > P && 2 * lo.cnt > n;
+      const int tmax2 = far ? 512 : 256;
+      int gs = 0;
+      while ((TL2 << (gs + 1)) <= tmax2 && P % (2 << gs) == 0) gs++;
+      auto lds2 = [&](int q) { return (size_t)N2 * (N1 * (1 << q) + (1 << q)) * sizeof(T) + (size_t)n * sizeof(cx<T>); };
+      while (gs > 0 && lds2(gs) > 160 * 1024) gs--;
+      const int nch = P >> gs;
+      const dim3 gr2((unsigned)((long long)nlines * nch)), bl2(TL2 << gs);
+      if (sign < 0)
+        hipLaunchKernelGGL((k_fft2p<T, N1, N2, -1>), gr2, bl2, lds2(gs), st, in, out, li, lo, tw, gs, nch);
+      else
+        hipLaunchKernelGGL((k_fft2p<T, N1, N2, +1>), gr2, bl2, lds2(gs), st, in, out, li, lo, tw, gs, nch);
+      return true;
+    }
+  }
   const int TL = pairs ? std::max(N1, 2 * N2) : N1, tmax = pairs ? 512 : 256;
   int cap = std::max(1, std::min(tmax, g_fft2_threads * tmax / 256) / TL);
   int Gs = 0;
