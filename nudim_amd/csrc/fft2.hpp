@@ -459,7 +459,7 @@ __device__ __forceinline__ void dft_pair(cx<T> *v, int h) {
 // tid >> (log2 G + 1).  LDS: the real exchange buffer [N2][N1 G + G] and the
 // n twiddles.
 template <typename T, int N1, int N2, int S>
-__global__ void __launch_bounds__(512)
+__global__ void __launch_bounds__(256)
 k_fft2p(const cx<T> *in, cx<T> *out, LineIO li, LineIO lo, const cx<T> *__restrict__ tw, int Gs, int nchunk) {
   constexpr int n = N1 * N2, A1 = N1 / 2, A2 = N2 / 2;
   static_assert(N1 % 2 == 0 && N2 % 2 == 0, "even factors");

@@ -41,17 +41,9 @@ bool launch_shape(const cx<T> *in, cx<T> *out, int nlines, const LineIO &li, con
     // FP64 lines of the generic form on lane pairs (k_fft2p)
     if (!hin && !hout && fft2_pairs64_on()) {
       constexpr int TL2 = 2 * (N1 > N2 ? N1 : N2);
-      // lines that write most of their positions far apart (the trafo's
-      // grid columns: one row of n P elements between positions) in
-      // 512-thread blocks of 8 planes, for 128-byte runs per position; the
-      // others in 256-thread blocks of 4 planes, three blocks per CU.  At
-      // 256^3 in FP64 (one lane per line: 4 planes, 1 block per CU): trafo
-      // column pass 2267 -> 2104 us (2477 with 4 planes), adjoint column
-      // pass 2612 -> 2256 us (2411 with 8), row passes 718 -> 527 us
-      const bool far = lo.sk > P && 2 * lo.cnt > n;
-      const int tmax2 = far ? 512 : 256;
+      // 256-thread blocks (4 planes at N1 = N2 = 32), three per CU
       int gs = 0;
-      while ((TL2 << (gs + 1)) <= tmax2 && P % (2 << gs) == 0) gs++;
+      while ((TL2 << (gs + 1)) <= 256 && P % (2 << gs) == 0) gs++;
       auto lds2 = [&](int q) { return (size_t)N2 * (N1 * (1 << q) + (1 << q)) * sizeof(T) + (size_t)n * sizeof(cx<T>); };
       while (gs > 0 && lds2(gs) > 160 * 1024) gs--;
       const int nch = P >> gs;

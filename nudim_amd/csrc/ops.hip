@@ -476,14 +476,16 @@ template <typename T>
 void NfftOp<T>::grid_fft(const cx<T> *fhat, hipStream_t st) {
   const long long NP = (long long)N * P, nP = (long long)n * P;
   const double *d = g->dec.template as<double>();
-  // rows: deconvolve, place at (k - N/2) mod n, FFT along x
-  LineIO li{NP, P, N, -N / 2, d, d};
-  LineIO lo{nP, P, n, 0, nullptr, nullptr};
-  g->fft.run(fhat, X.as<cx<T>>(), N, li, lo, -1, P, st);
-  // columns: N non-zero rows at (k - N/2) mod n, FFT along y
-  LineIO ci{P, nP, N, -N / 2, nullptr, nullptr};
-  LineIO co{P, nP, n, 0, nullptr, nullptr};
-  g->fft.run(X.as<cx<T>>(), grid.as<cx<T>>(), n, ci, co, -1, P, st);
+  // the N columns of fhat first (deconvolved, placed at (k - N/2) mod n,
+  // FFT along y): X[y][kx]; then the n rows, whose positions lie P elements
+  // apart, write the grid (256^3 FP64: 2.68 -> 2.03 ms against columns
+  // last, whose positions lie a grid row apart)
+  LineIO ci{P, NP, N, -N / 2, d, d};
+  LineIO co{P, NP, n, 0, nullptr, nullptr};
+  g->fft.run(fhat, X.as<cx<T>>(), N, ci, co, -1, P, st);
+  LineIO ri{NP, P, N, -N / 2, nullptr, nullptr};
+  LineIO ro{nP, P, n, 0, nullptr, nullptr};
+  g->fft.run(X.as<cx<T>>(), grid.as<cx<T>>(), n, ri, ro, -1, P, st);
 }
 
 template <typename T>
@@ -613,12 +615,15 @@ void NfftOp<T>::adjoint(const cx<T> *f, cx<T> *fhat, hipStream_t st) {
   }
   NUDIM_HIP(hipGetLastError());
   const double *d = g->dec.template as<double>();
-  LineIO ci{P, nP, n, 0, nullptr, nullptr};
-  LineIO co{P, nP, N, -N / 2, nullptr, nullptr};
-  g->fft.run(grid.as<cx<T>>(), X.as<cx<T>>(), n, ci, co, +1, P, st);
+  // the rows of the grid first (the n x n side, positions P elements apart;
+  // its columns' positions lie a grid row apart, a slower access), N
+  // outputs kept: X[y][kx]; then the N columns with the deconvolution
   LineIO ri{nP, P, n, 0, nullptr, nullptr};
-  LineIO ro{NP, P, N, -N / 2, d, d};
-  g->fft.run(X.as<cx<T>>(), fhat, N, ri, ro, +1, P, st);
+  LineIO ro{NP, P, N, -N / 2, nullptr, nullptr};
+  g->fft.run(grid.as<cx<T>>(), X.as<cx<T>>(), n, ri, ro, +1, P, st);
+  LineIO ci{P, NP, n, 0, nullptr, nullptr};
+  LineIO co{P, NP, N, -N / 2, d, d};
+  g->fft.run(X.as<cx<T>>(), fhat, N, ci, co, +1, P, st);
 }
 
 // ---------------------------------------------------------------------------
