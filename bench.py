@@ -695,7 +695,7 @@ def main():
         # gathers per pass): HIP events on the job's stream, one pass each
         n64, w64 = NFFT_CHOICE["f64"]["n"], NFFT_CHOICE["f64"]["W"]
         for kc, what in ((2, "interpolation: k_interp_block<double> + k_interp_sum<double>, W=%d, n=%d"),
-                         (3, "gather adjoint: k_gather_patch<double>, W=%d, n=%d")):
+                         (3, "gather adjoint: k_gather_patch<double,8,4>, W=%d, n=%d")):
             _, _, (kms, klaunch, kbytes) = timed(j64, 1, kc)
             if not klaunch:
                 continue
