@@ -347,9 +347,15 @@ static std::shared_ptr<NfftGeom<T>> build_nfft_geometry(int N, int n, int W, con
         }
         size_t len = 0;
         for (auto &v : sec) len = std::max(len, v.size());
-        sg.assign(8 * len, PatchSeg{-1, 0, 0, -1});
-        for (int x = 0; x < 8; x++)
-          for (size_t k = 0; k < sec[x].size(); k++) sg[8 * k + x] = sec[x][k];
+        if (8 * len <= sg.size() + sg.size() / 2) {
+          sg.assign(8 * len, PatchSeg{-1, 0, 0, -1});
+          for (int x = 0; x < 8; x++)
+            for (size_t k = 0; k < sec[x].size(); k++) sg[8 * k + x] = sec[x][k];
+        } else {
+          // node sets crowded into a few sectors (not the polar grids of a
+          // job) keep the global heavy-first order, dealt round robin
+          sg.resize((sg.size() + 7) / 8 * 8, PatchSeg{-1, 0, 0, -1});
+        }
       }
       g->ngseg = (int)sg.size();
       g->ncomb = (int)comb.size() / 3;

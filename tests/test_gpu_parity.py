@@ -64,7 +64,7 @@ def test_nfft_fp32(hip, orc, polar32, n, W, tol):
                                           (True, 120, 14, 1e-12)])
 def test_nfft_patch_forms(hip, orc, fp64, n, W, tol):
     """64 planes on a grid whose edge is a multiple of 16 take the patch-form
-    kernels (4 x 8-cell patches in registers: the gather writes each cell
+    kernels (8 x 4-cell patches in registers: the gather writes each cell
     once, the interpolation adds per-block-patch partial sums per node); 63
     planes take the per-cell gather and the per-node interpolation.  The two
     forms agree to rounding and both match the NDFT (the dense centre
@@ -88,6 +88,26 @@ def test_nfft_patch_forms(hip, orc, fp64, n, W, tol):
     for p in (0, 37, 63):
         assert rel(a64[p], orc.ndft_adjoint(x, g[p], N)) < tol
         assert rel(f64[p], orc.ndft_trafo(x, fh[p])) < tol
+
+
+@pytest.mark.parametrize("fp64", [True, False])
+def test_nfft_patch_gather_crowded_nodes(hip, orc, fp64):
+    """Nodes crowded into one corner of the frequency square (one angular
+    sector of the grid): the gather keeps its global heavy-first order
+    instead of dealing sectors to the XCDs, and equals the per-cell form
+    and the NDFT; uniformly spread nodes take the sector order and agree
+    the same way."""
+    N = 64
+    rng = np.random.default_rng(23)
+    for x in (0.05 + 0.3 * rng.random((3000, 2)), rng.random((3000, 2)) - 0.5):
+        M = x.shape[0]
+        g = rng.standard_normal((64, M)) + 1j * rng.standard_normal((64, M))
+        prec = hip.FP64_PLAIN if fp64 else hip.FP32
+        a64 = hip.nfft_adjoint(x, g, N, precision=prec)
+        a63 = hip.nfft_adjoint(x, g[:63], N, precision=prec)
+        assert rel(a64[:63], a63) < (1e-14 if fp64 else 1e-6)
+        for p in (0, 63):
+            assert rel(a64[p], orc.ndft_adjoint(x, g[p], N)) < (1e-10 if fp64 else 2e-5)
 
 
 @pytest.mark.parametrize("N", [64, 192])
