@@ -593,7 +593,7 @@ void NfftOp<T>::adjoint(const cx<T> *f, cx<T> *fhat, hipStream_t st) {
                        gpart.as<cx<T>>(), g->gseg.template as<PatchSeg>(),
                        g->grec.template as<PatchRec<T, kGatherR, kGatherC>>(), n, P, g->ngseg);
     if (g->ncomb > 0)
-      hipLaunchKernelGGL((k_gather_combine<T, kGatherR, kGatherC>), dim3(cdiv((size_t)g->ncomb * (P / 64), 4)),
+      hipLaunchKernelGGL((k_gather_combine<T, kGatherR, kGatherC>), dim3(cdiv((size_t)g->ncomb * kGatherR * (P / 64), 4)),
                          dim3(kThreads), 0, st, gpart.as<cx<T>>(), grid.as<cx<T>>(), g->gcomb.template as<int>(), n,
                          P, g->ncomb);
   } else {

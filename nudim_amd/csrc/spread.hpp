@@ -194,35 +194,40 @@ k_gather_patch(const cx<T> *__restrict__ s, cx<T> *__restrict__ g, cx<T> *__rest
 }
 
 // The patches split into segments: cell = the segments' partial sums added
-// in segment order.  comb holds (patch, first slot, count) triples; the
-// wavefront layout of patch_wave.
+// in segment order.  comb holds (patch, first slot, count) triples; one
+// wavefront per (patch, patch row) and plane group (item = triple PR + row,
+// the wavefront layout of patch_wave), two segments' loads in flight.
 template <typename T, int PR, int PC>
 __global__ void __launch_bounds__(kThreads)
 k_gather_combine(const cx<T> *__restrict__ part, cx<T> *__restrict__ g, const int *__restrict__ comb, int n, int P,
                  int ncomb) {
   int item, p;
   patch_wave(P >> 6, item, p);
-  if (item >= ncomb) return;
-  const int pid = comb[3 * item], slot0 = comb[3 * item + 1], nseg = comb[3 * item + 2];
+  if (item >= ncomb * PR) return;
+  const int c = item / PR, i = item - c * PR;
+  const int pid = comb[3 * c], slot0 = comb[3 * c + 1], nseg = comb[3 * c + 2];
   const int npc = n / PC, a = pid / npc, b = pid - a * npc;
-  // a patch row (PC cells) at a time, each segment's PC loads in flight together
+  const cx<T> *src = part + ((size_t)slot0 * PR * PC + i * PC) * P + p;
+  const size_t ss = (size_t)PR * PC * P;  // one segment's partial sums
+  cx<T> acc[PC];
 #pragma unroll
-  for (int i = 0; i < PR; i++) {
-    const cx<T> *src = part + ((size_t)slot0 * PR * PC + i * PC) * P + p;
-    cx<T> acc[PC];
+  for (int j = 0; j < PC; j++) acc[j] = src[(size_t)j * P];
+  int k = 1;
+  for (; k + 2 <= nseg; k += 2) {
+    cx<T> v[PC], w[PC];
 #pragma unroll
-    for (int j = 0; j < PC; j++) acc[j] = src[(size_t)j * P];
-    for (int k = 1; k < nseg; k++) {
-      src += (size_t)PR * PC * P;
-      cx<T> v[PC];
+    for (int j = 0; j < PC; j++) v[j] = src[k * ss + (size_t)j * P];
 #pragma unroll
-      for (int j = 0; j < PC; j++) v[j] = src[(size_t)j * P];
+    for (int j = 0; j < PC; j++) w[j] = src[(k + 1) * ss + (size_t)j * P];
 #pragma unroll
-      for (int j = 0; j < PC; j++) acc[j] = acc[j] + v[j];
-    }
-#pragma unroll
-    for (int j = 0; j < PC; j++) g[((size_t)(a * PR + i) * n + (size_t)(b * PC + j)) * P + p] = acc[j];
+    for (int j = 0; j < PC; j++) acc[j] = (acc[j] + v[j]) + w[j];
   }
+  if (k < nseg) {
+#pragma unroll
+    for (int j = 0; j < PC; j++) acc[j] = acc[j] + src[k * ss + (size_t)j * P];
+  }
+#pragma unroll
+  for (int j = 0; j < PC; j++) g[((size_t)(a * PR + i) * n + (size_t)(b * PC + j)) * P + p] = acc[j];
 }
 
 template <int I, typename T>
