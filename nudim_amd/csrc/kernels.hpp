@@ -372,6 +372,22 @@ __global__ void k_xpby_planes(cx<T> *__restrict__ pv, const cx<T> *__restrict__ 
   pv[i] = b * pv[i] + z[i];
 }
 
+// f += alpha_p p ; p = z + beta_p p   (the NFFT-form CG step's two updates of
+// p's old value in one pass; the same arithmetic as k_axpy_planes then
+// k_xpby_planes)
+template <typename T>
+__global__ void k_fxp_planes(cx<T> *__restrict__ f, cx<T> *__restrict__ pv, const cx<T> *__restrict__ z,
+                             const double *__restrict__ alpha, const double *__restrict__ beta, long long total,
+                             int P) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  int p = (int)(i % P);
+  const T a = (T)alpha[p], b = (T)beta[p];
+  const cx<T> po = pv[i];
+  f[i] = f[i] + a * po;
+  pv[i] = b * po + z[i];
+}
+
 // r -= alpha_p v ; t = w .* r   (rows = M)
 template <typename T>
 __global__ void k_resid_update(cx<T> *__restrict__ r, const cx<T> *__restrict__ v, const double *__restrict__ alpha,

@@ -979,7 +979,7 @@ void Cgnr<T>::init(int N_, int M_, int P_, int form_, NfftOp<T> *op_, ToeplitzOp
   v.alloc(MM);
   t.alloc(MM);
   dot_z.alloc(P * sizeof(double));
-  coef.alloc(P * sizeof(double));
+  coef.alloc(2 * P * sizeof(double));  // alpha, then beta of the NFFT-form step
   sc.alloc(P * sizeof(double));
   red.init(P);
 }
@@ -1109,12 +1109,13 @@ void Cgnr<T>::solve(const cx<T> *y, cx<T> *fhat, int nsteps, bool f0_zero, hipSt
     op->trafo(pp, vp, st);
     red.dot<T>(vp, w, M, s, st);
     hipLaunchKernelGGL(k_cg_scalars, dim3(cdiv(P, 64)), dim3(64), 0, st, dz, cf, s, P, 1);
-    hipLaunchKernelGGL(k_axpy_planes<T>, dim3(bN), dim3(256), 0, st, fhat, pp, cf, 1.0, NN, P);
     hipLaunchKernelGGL(k_resid_update<T>, dim3(bM), dim3(256), 0, st, rp, vp, cf, w, tp, MM, P);
     op->adjoint(tp, zp, st);
     red.dot<T>(zp, nullptr, (long long)N * N, s, st);
-    hipLaunchKernelGGL(k_cg_scalars, dim3(cdiv(P, 64)), dim3(64), 0, st, dz, cf, s, P, 2);
-    hipLaunchKernelGGL(k_xpby_planes<T>, dim3(bN), dim3(256), 0, st, pp, zp, cf, NN, P);
+    // beta into the second half of coef; f += alpha p and p = z + beta p
+    // of p's old value together (f is not read in between)
+    hipLaunchKernelGGL(k_cg_scalars, dim3(cdiv(P, 64)), dim3(64), 0, st, dz, cf + P, s, P, 2);
+    hipLaunchKernelGGL(k_fxp_planes<T>, dim3(bN), dim3(256), 0, st, fhat, pp, zp, cf, cf + P, NN, P);
     NUDIM_HIP(hipGetLastError());
   }
 }
