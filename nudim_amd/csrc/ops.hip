@@ -342,15 +342,15 @@ static std::shared_ptr<NfftGeom<T>> build_nfft_geometry(int N, int n, int W, con
           if (cy >= n / 2) cy -= n;
           if (cx >= n / 2) cx -= n;
           const double ang = std::atan2((double)cy, (double)cx);
-          const int x = std::min(7, std::max(0, (int)std::floor((ang + M_PI) / (2 * M_PI) * 8)));
-          sec[x].push_back(q);
+          const int xcd = std::min(7, std::max(0, (int)std::floor((ang + M_PI) / (2 * M_PI) * 8)));
+          sec[xcd].push_back(q);
         }
         size_t len = 0;
         for (auto &v : sec) len = std::max(len, v.size());
         if (8 * len <= sg.size() + sg.size() / 2) {
           sg.assign(8 * len, PatchSeg{-1, 0, 0, -1});
-          for (int x = 0; x < 8; x++)
-            for (size_t k = 0; k < sec[x].size(); k++) sg[8 * k + x] = sec[x][k];
+          for (int xcd = 0; xcd < 8; xcd++)
+            for (size_t k = 0; k < sec[xcd].size(); k++) sg[8 * k + xcd] = sec[xcd][k];
         } else {
           // node sets crowded into a few sectors (not the polar grids of a
           // job) keep the global heavy-first order, dealt round robin
