@@ -110,6 +110,23 @@ def test_nfft_patch_gather_crowded_nodes(hip, orc, fp64):
             assert rel(a64[p], orc.ndft_adjoint(x, g[p], N)) < (1e-10 if fp64 else 2e-5)
 
 
+@pytest.mark.parametrize("N,n", [(96, 384), (160, 512), (256, 768), (384, 1024), (600, 1536)])
+def test_nfft_fp64_lane_pair_lengths(hip, orc, N, n):
+    """FP64 grids of 24x16, 32x16, 32x24, 32x32 and 48x32 points (n > 2N:
+    the general line form on lane pairs, k_fft2p): trafo and adjoint of two
+    planes at 400 random nodes against the NDFT."""
+    rng = np.random.default_rng(N)
+    x = rng.random((400, 2)) - 0.5
+    P = 2
+    fh = rng.standard_normal((P, N, N)) + 1j * rng.standard_normal((P, N, N))
+    g = rng.standard_normal((P, 400)) + 1j * rng.standard_normal((P, 400))
+    f = hip.nfft_trafo(x, fh, n=n, W=14, precision=hip.FP64_PLAIN)
+    a = hip.nfft_adjoint(x, g, N, n=n, W=14, precision=hip.FP64_PLAIN)
+    for p in range(P):
+        assert rel(f[p], orc.ndft_trafo(x, fh[p])) < 1e-11
+        assert rel(a[p], orc.ndft_adjoint(x, g[p], N)) < 1e-11
+
+
 @pytest.mark.parametrize("N", [64, 192])
 def test_nfft_centred_half_lengths(hip, orc, N):
     """n = 2N on the fixed-length FFT kernels (16 x 8, 24 x 16): the row and
